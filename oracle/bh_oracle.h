@@ -1,0 +1,146 @@
+/*
+ * CPU oracle for the Barnes-Hut force step — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's CPU algorithm (LucasFurer/MasterThesisLucas), used as
+ * the checker of the HIP path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or the timed CPU baseline, never as the product path.
+ * Each function names the reference file:line it follows. Arithmetic is FP64 like the reference.
+ *
+ * Pinning (what checks each function beyond retyping it):
+ *   orc_morton32/64      closed-form bit interleave vs brute force, hand boundary cases, and
+ *                        exhaustive per-axis checks in tests/test_oracle.py
+ *   orc_stable_sort_u64  invariants (non-decreasing keys, permutation, ties by index)
+ *   orc_get_depth        the depth table quoted for N = 65,536 / 70,000 / 1,048,576 / 4,194,304
+ *   orc_fixed_tree       brute-force recount of every slot on small inputs; root M0 = N,
+ *                        root COM = mean, leaf M2 = direct second moment
+ *   orc_adaptive_tree    brute-force cell counts; topology equal to orc_quadtree_build's
+ *   orc_quadtree_*       bit-exact against the compiled reference (oracle/_ref) when present
+ *   orc_bh_traverse      theta=0 equals the direct sum; one far cell equals one monopole;
+ *                        against orc_quadtree_forces and oracle/_ref
+ *   orc_direct           closed forms (two-body, symmetric rings), Newton's third law
+ *   orc_tsne_*           finite differences of the KL cost; Z invariant
+ *   orc_mlp_*            finite differences; Adam closed form
+ */
+#ifndef BH_ORACLE_H
+#define BH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_KERNEL_GRAVITY = 0, ORC_KERNEL_TSNE = 1 };
+/* acceptance / near-field semantics */
+enum {
+    ORC_SEM_QUADTREE = 0, /* nBodySolverBH.h:56-89: l/|d| < theta; children<=1 -> PP over all */
+    ORC_SEM_MORTON = 1    /* nBodySolverFMM_SYM_MORTON.h:272-313: d^2 > (l/theta)^2; PP at leaves */
+};
+
+/* ---- Morton codes: nBodySolverFMM_SYM_MORTON.h:538-561 ---- */
+void orc_morton32(const double* x, const double* y, int64_t n, double minx, double miny, double L,
+                  uint32_t* code);
+/* same rule with `bits` bits per axis (bits <= 32): q = min(floor(2^bits * RN((p-min)/L)), 2^bits-1) */
+void orc_morton64(const double* x, const double* y, int64_t n, double minx, double miny, double L,
+                  int bits, uint64_t* code);
+void orc_quantize_axis(const double* v, int64_t n, double vmin, double L, int bits, uint64_t* q);
+
+/* ---- sort: stable by (key, original index); perm[i] = original index of i-th element ---- */
+void orc_stable_sort_u64(const uint64_t* key, int64_t n, uint32_t* perm);
+
+/* ---- fixed-depth Morton tree: nBodySolverFMM_SYM_MORTON.h:106-151, 328-470 ---- */
+int orc_get_depth(int max_children, int64_t n);
+int64_t orc_fixed_tree_slots(int depth);
+void orc_fixed_tree(const double* xs, const double* ys, int64_t n, int depth, double minx,
+                    double miny, double maxx, double maxy, uint32_t* first_child,
+                    uint32_t* first_particle, uint32_t* count, double* bb_cx, double* bb_cy,
+                    double* bb_len, double* com_x, double* com_y, double* m0, double* m2);
+
+/* ---- adaptive (leaf-capacity) tree over sorted Morton codes, pre-order layout ----
+ * Cells at level k are the top 2k bits of a code with `bits` bits per axis. A cell is split
+ * while it holds more than leaf_cap particles (quadtree.h:98) and level < bits.
+ * Returns the node count, or -1 if cap_nodes is too small. */
+int64_t orc_adaptive_tree(const uint64_t* codes_sorted, int64_t n, int bits, int leaf_cap,
+                          int64_t cap_nodes, int32_t* level, uint32_t* first, uint32_t* count,
+                          uint32_t* skip, uint32_t* nchild);
+/* pre-order form of the fixed-depth tree: every non-empty cell of level <= depth */
+int64_t orc_fixed_preorder(const uint32_t* codes_sorted, int64_t n, int depth, int64_t cap_nodes,
+                           int32_t* level, uint32_t* first, uint32_t* count, uint32_t* skip,
+                           uint32_t* nchild);
+/* mass and centre of mass per pre-order node (plain FP64 sums over each node's range) */
+void orc_node_moments(const double* xs, const double* ys, const double* ms, int64_t nnodes,
+                      const uint32_t* first, const uint32_t* count, double* cx, double* cy,
+                      double* mass, double* m2 /* 3 per node: xx, xy, yy (mass weighted) */);
+
+/* ---- Barnes-Hut traversal over a pre-order tree (sorted particles) ----
+ * nBodySolverBH.h:56-89 (semantics QUADTREE) / nBodySolverFMM_SYM_MORTON.h:272-313 (MORTON),
+ * with the t-SNE kernels of nBodySolverBH.h:94-116 or the Plummer-softened gravity kernel.
+ * Node side at level k = root_side * 2^-k. pp[k] marks nodes whose near field is PP.
+ * sinks: sorted indices to evaluate (NULL = all n); outputs indexed like sinks. */
+void orc_bh_traverse(const double* xs, const double* ys, const double* ms, int64_t n,
+                     const int32_t* level, const uint32_t* first, const uint32_t* count,
+                     const uint32_t* skip, const uint8_t* pp, const double* cx, const double* cy,
+                     const double* mass, int64_t nnodes, double root_side, int semantics,
+                     int kernel, double G, double eps2, double theta, const int64_t* sinks,
+                     int64_t nsinks, double* out_ax, double* out_ay, double* out_tot,
+                     uint64_t* counters /* [pn, pp] */, int nthreads);
+
+/* ---- FP64 compensated direct sum (nBodySolverNaive.h:53-140), self excluded by index ---- */
+void orc_direct(const double* x, const double* y, const double* m, int64_t n, int kernel, double G,
+                double eps2, const int64_t* sinks, int64_t nsinks, double* out_ax, double* out_ay,
+                double* out_tot, int nthreads);
+
+/* ---- QuadTree restatement: quadtree.h:29-171 + nBodySolverBH.h:33-116 ----
+ * Build from positions (input order), FP64 with the FP32 rounding of returned child
+ * mass/COM (quadtree.h:96,146). Traversal accumulates into out_ax/out_ay (+=) and returns the
+ * running total like solveNbody (single thread: bit-compatible order). */
+typedef struct orc_quadtree orc_quadtree;
+orc_quadtree* orc_quadtree_build(const double* x, const double* y, int64_t n, int max_children);
+void orc_quadtree_free(orc_quadtree* t);
+int64_t orc_quadtree_node_count(const orc_quadtree* t);
+/* per node, pre-order (children in HH,HL,LH,LL order): depth, count, nchildren, low/high x/y,
+ * total mass (float widened), com */
+void orc_quadtree_export(const orc_quadtree* t, int32_t* depth, uint32_t* count, uint32_t* nchild,
+                         double* lowx, double* lowy, double* highx, double* highy, double* mass,
+                         double* comx, double* comy);
+/* sorted occupant order of the leaves/pp cells is not needed by callers: occupant sets are
+ * reported as min/max original index per node for topology comparisons */
+void orc_quadtree_occupants(const orc_quadtree* t, uint32_t* node_first, uint32_t* occ_index);
+double orc_quadtree_forces(const orc_quadtree* t, const double* x, const double* y, int kernel,
+                           double G, double m, double eps2, double theta, double* out_ax,
+                           double* out_ay, uint64_t* counters, int nthreads);
+
+/* ---- integrators ---- */
+/* kick-drift-kick leapfrog with BH forces (adaptive tree, QUADTREE semantics, or fixed depth
+ * MORTON semantics), FP64 state, one tree rebuild per step (gravitysim.h:99-121 order). */
+void orc_leapfrog(double* x, double* y, double* vx, double* vy, const double* m, int64_t n,
+                  int steps, double dt, double G, double eps2, double theta, int semantics,
+                  int leaf_cap, int bits, int nthreads);
+
+/* ---- t-SNE: tsne.h:179-233, 287-301, 356-381, 383-460 ---- */
+double orc_tsne_theta(double min_theta, double max_theta, int iteration);
+/* attraction (adds to dx/dy): exag * sum_j P_ij w_ij (y_j - y_i) */
+void orc_tsne_attraction(const double* x, const double* y, int64_t n, const uint32_t* row_ptr,
+                         const uint32_t* col, const double* val, double exag, double* dx,
+                         double* dy);
+/* KL divergence sum_{i!=j} P_ij log2(P_ij/Q_ij) with exact FP64 Z (threaded) */
+double orc_tsne_kl(const double* x, const double* y, int64_t n, const uint32_t* row_ptr,
+                   const uint32_t* col, const double* val, int nthreads);
+double orc_tsne_z(const double* x, const double* y, int64_t n, int nthreads);
+
+/* ---- MLP (10 -> H -> H -> 2, SiLU), FP64 ---- */
+void orc_mlp_forward(const double* X, int64_t B, int din, int h, int dout, const double* W1,
+                     const double* b1, const double* W2, const double* b2, const double* W3,
+                     const double* b3, double* Y);
+/* mean-squared-error loss and parameter gradients */
+double orc_mlp_loss_grad(const double* X, const double* T, int64_t B, int din, int h, int dout,
+                         const double* W1, const double* b1, const double* W2, const double* b2,
+                         const double* W3, const double* b3, double* gW1, double* gb1,
+                         double* gW2, double* gb2, double* gW3, double* gb3);
+void orc_adam(double* p, const double* g, double* mom1, double* mom2, int64_t n, double lr,
+              double beta1, double beta2, double eps, int64_t t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,374 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and of the compiled reference (_ref/libref_bh.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg as the checker / CPU baseline. The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_bh.so")
+
+KERNEL_GRAVITY = 0
+KERNEL_TSNE = 1
+SEM_QUADTREE = 0
+SEM_MORTON = 1
+
+_lib = None
+_ref = None
+
+_d = C.POINTER(C.c_double)
+_u32 = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+_u8 = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        I64, I32, D = C.c_int64, C.c_int, C.c_double
+        sig = {
+            "orc_morton32": (None, [_vp, _vp, I64, D, D, D, _vp]),
+            "orc_morton64": (None, [_vp, _vp, I64, D, D, D, I32, _vp]),
+            "orc_quantize_axis": (None, [_vp, I64, D, D, I32, _vp]),
+            "orc_stable_sort_u64": (None, [_vp, I64, _vp]),
+            "orc_get_depth": (I32, [I32, I64]),
+            "orc_fixed_tree_slots": (I64, [I32]),
+            "orc_fixed_tree": (None, [_vp, _vp, I64, I32, D, D, D, D] + [_vp] * 10),
+            "orc_adaptive_tree": (I64, [_vp, I64, I32, I32, I64] + [_vp] * 5),
+            "orc_fixed_preorder": (I64, [_vp, I64, I32, I64] + [_vp] * 5),
+            "orc_node_moments": (None, [_vp, _vp, _vp, I64, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "orc_bh_traverse": (None, [_vp, _vp, _vp, I64] + [_vp] * 8 + [I64, D, I32, I32, D, D, D,
+                                                                          _vp, I64, _vp, _vp, _vp,
+                                                                          _vp, I32]),
+            "orc_direct": (None, [_vp, _vp, _vp, I64, I32, D, D, _vp, I64, _vp, _vp, _vp, I32]),
+            "orc_quadtree_build": (_vp, [_vp, _vp, I64, I32]),
+            "orc_quadtree_free": (None, [_vp]),
+            "orc_quadtree_node_count": (I64, [_vp]),
+            "orc_quadtree_export": (None, [_vp] + [_vp] * 10),
+            "orc_quadtree_occupants": (None, [_vp, _vp, _vp]),
+            "orc_quadtree_forces": (D, [_vp, _vp, _vp, I32, D, D, D, D, _vp, _vp, _vp, I32]),
+            "orc_leapfrog": (None, [_vp] * 5 + [I64, I32, D, D, D, D, I32, I32, I32, I32]),
+            "orc_tsne_theta": (D, [D, D, I32]),
+            "orc_tsne_attraction": (None, [_vp, _vp, I64, _vp, _vp, _vp, D, _vp, _vp]),
+            "orc_tsne_kl": (D, [_vp, _vp, I64, _vp, _vp, _vp, I32]),
+            "orc_tsne_z": (D, [_vp, _vp, I64, I32]),
+            "orc_mlp_forward": (None, [_vp, I64, I32, I32, I32] + [_vp] * 7),
+            "orc_mlp_loss_grad": (D, [_vp, _vp, I64, I32, I32, I32] + [_vp] * 12),
+            "orc_adam": (None, [_vp, _vp, _vp, _vp, I64, D, D, D, D, I64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"compiled reference missing: {REF_PATH} (run `make -C oracle ref`)")
+        L = C.CDLL(REF_PATH)
+        I64, I32, D = C.c_int64, C.c_int, C.c_double
+        L.ref_bh_solve.restype = D
+        L.ref_bh_solve.argtypes = [_vp, _vp, I64, I32, D, D, D, D, I32, _vp, _vp, _vp, _vp]
+        L.ref_quadtree_export.restype = I64
+        L.ref_quadtree_export.argtypes = [_vp, _vp, I64, I32] + [_vp] * 10
+        L.ref_naive_tsne.restype = D
+        L.ref_naive_tsne.argtypes = [_vp, _vp, I64, _vp, _vp]
+        _ref = L
+    return _ref
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------------------------
+def morton32(x, y, minx, miny, L):
+    x, y = _f64(x), _f64(y)
+    out = np.empty(x.size, np.uint32)
+    lib().orc_morton32(_p(x), _p(y), x.size, float(minx), float(miny), float(L), _p(out))
+    return out
+
+
+def morton64(x, y, minx, miny, L, bits=32):
+    x, y = _f64(x), _f64(y)
+    out = np.empty(x.size, np.uint64)
+    lib().orc_morton64(_p(x), _p(y), x.size, float(minx), float(miny), float(L), int(bits), _p(out))
+    return out
+
+
+def quantize_axis(v, vmin, L, bits):
+    v = _f64(v)
+    out = np.empty(v.size, np.uint64)
+    lib().orc_quantize_axis(_p(v), v.size, float(vmin), float(L), int(bits), _p(out))
+    return out
+
+
+def stable_sort(keys):
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    perm = np.empty(k.size, np.uint32)
+    lib().orc_stable_sort_u64(_p(k), k.size, _p(perm))
+    return perm
+
+
+def get_depth(max_children, n):
+    return int(lib().orc_get_depth(int(max_children), int(n)))
+
+
+def quadtree_side(minx, miny, maxx, maxy):
+    """Root side of the reference QuadTree: ((min + largest) + 1e-4) - min (quadtree.h:45-47)."""
+    largest = max(maxx - minx, maxy - miny)
+    return ((minx + largest) + 0.0001) - minx
+
+
+def fixed_tree(xs, ys, depth, minx, miny, maxx, maxy):
+    xs, ys = _f64(xs), _f64(ys)
+    slots = int(lib().orc_fixed_tree_slots(int(depth)))
+    out = dict(
+        first_child=np.empty(slots, np.uint32), first_particle=np.empty(slots, np.uint32),
+        count=np.empty(slots, np.uint32), bb_cx=np.empty(slots), bb_cy=np.empty(slots),
+        bb_len=np.empty(slots), com_x=np.empty(slots), com_y=np.empty(slots), m0=np.empty(slots),
+        m2=np.empty(3 * slots))
+    lib().orc_fixed_tree(_p(xs), _p(ys), xs.size, int(depth), float(minx), float(miny), float(maxx),
+                         float(maxy), *[_p(out[k]) for k in ("first_child", "first_particle", "count",
+                                                             "bb_cx", "bb_cy", "bb_len", "com_x",
+                                                             "com_y", "m0", "m2")])
+    out["m2"] = out["m2"].reshape(slots, 3)
+    return out
+
+
+def _pre_arrays(cap):
+    return dict(level=np.empty(cap, np.int32), first=np.empty(cap, np.uint32),
+                count=np.empty(cap, np.uint32), skip=np.empty(cap, np.uint32),
+                nchild=np.empty(cap, np.uint32))
+
+
+def adaptive_tree(codes_sorted, bits=32, leaf_cap=16):
+    c = np.ascontiguousarray(codes_sorted, dtype=np.uint64)
+    cap = 3 * c.size + 64
+    a = _pre_arrays(cap)
+    nn = lib().orc_adaptive_tree(_p(c), c.size, int(bits), int(leaf_cap), cap, _p(a["level"]),
+                                 _p(a["first"]), _p(a["count"]), _p(a["skip"]), _p(a["nchild"]))
+    if nn < 0:
+        raise RuntimeError("node capacity exceeded")
+    return {k: v[:nn].copy() for k, v in a.items()}
+
+
+def fixed_preorder(codes32_sorted, depth):
+    c = np.ascontiguousarray(codes32_sorted, dtype=np.uint32)
+    cap = int(lib().orc_fixed_tree_slots(int(depth))) + 64
+    cap = min(cap, c.size * (depth + 1) + 64)
+    a = _pre_arrays(cap)
+    nn = lib().orc_fixed_preorder(_p(c), c.size, int(depth), cap, _p(a["level"]), _p(a["first"]),
+                                  _p(a["count"]), _p(a["skip"]), _p(a["nchild"]))
+    if nn < 0:
+        raise RuntimeError("node capacity exceeded")
+    return {k: v[:nn].copy() for k, v in a.items()}
+
+
+def node_moments(xs, ys, ms, tree, with_m2=False):
+    xs, ys = _f64(xs), _f64(ys)
+    ms = None if ms is None else _f64(ms)
+    nn = tree["first"].size
+    cx, cy, mass = np.empty(nn), np.empty(nn), np.empty(nn)
+    m2 = np.empty(3 * nn) if with_m2 else None
+    lib().orc_node_moments(_p(xs), _p(ys), _p(ms), nn, _p(tree["first"]), _p(tree["count"]),
+                           _p(cx), _p(cy), _p(mass), _p(m2))
+    out = dict(cx=cx, cy=cy, mass=mass)
+    if with_m2:
+        out["m2"] = m2.reshape(nn, 3)
+    return out
+
+
+def bh_traverse(xs, ys, ms, tree, mom, root_side, semantics, kernel, G=1.0, eps2=0.0, theta=0.5,
+                sinks=None, pp=None, nthreads=8):
+    xs, ys = _f64(xs), _f64(ys)
+    ms = None if ms is None else _f64(ms)
+    if pp is None:
+        pp = (tree["nchild"] <= 1).astype(np.uint8)
+    pp = np.ascontiguousarray(pp, dtype=np.uint8)
+    ns = xs.size if sinks is None else len(sinks)
+    sk = None if sinks is None else np.ascontiguousarray(sinks, dtype=np.int64)
+    ax, ay, tot = np.empty(ns), np.empty(ns), np.empty(ns)
+    cnt = np.zeros(2, np.uint64)
+    lib().orc_bh_traverse(_p(xs), _p(ys), _p(ms), xs.size, _p(tree["level"]), _p(tree["first"]),
+                          _p(tree["count"]), _p(tree["skip"]), _p(pp), _p(mom["cx"]),
+                          _p(mom["cy"]), _p(mom["mass"]), tree["first"].size, float(root_side),
+                          int(semantics), int(kernel), float(G), float(eps2), float(theta), _p(sk),
+                          ns, _p(ax), _p(ay), _p(tot), _p(cnt), int(nthreads))
+    return ax, ay, tot, cnt
+
+
+def direct(x, y, m, kernel, G=1.0, eps2=0.0, sinks=None, nthreads=8):
+    x, y = _f64(x), _f64(y)
+    m = None if m is None else _f64(m)
+    ns = x.size if sinks is None else len(sinks)
+    sk = None if sinks is None else np.ascontiguousarray(sinks, dtype=np.int64)
+    ax, ay, tot = np.empty(ns), np.empty(ns), np.empty(ns)
+    lib().orc_direct(_p(x), _p(y), _p(m), x.size, int(kernel), float(G), float(eps2), _p(sk), ns,
+                     _p(ax), _p(ay), _p(tot), int(nthreads))
+    return ax, ay, tot
+
+
+class QuadTree:
+    """Restated reference QuadTree (quadtree.h) + traverseBH (nBodySolverBH.h)."""
+
+    def __init__(self, x, y, max_children=16):
+        self.x, self.y = _f64(x), _f64(y)
+        self.h = lib().orc_quadtree_build(_p(self.x), _p(self.y), self.x.size, int(max_children))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_quadtree_free(self.h)
+            self.h = None
+
+    def export(self):
+        nn = int(lib().orc_quadtree_node_count(self.h))
+        d = dict(depth=np.empty(nn, np.int32), count=np.empty(nn, np.uint32),
+                 nchild=np.empty(nn, np.uint32), lowx=np.empty(nn), lowy=np.empty(nn),
+                 highx=np.empty(nn), highy=np.empty(nn), mass=np.empty(nn), comx=np.empty(nn),
+                 comy=np.empty(nn))
+        lib().orc_quadtree_export(self.h, *[_p(d[k]) for k in ("depth", "count", "nchild", "lowx",
+                                                               "lowy", "highx", "highy", "mass",
+                                                               "comx", "comy")])
+        return d
+
+    def occupants(self):
+        nn = int(lib().orc_quadtree_node_count(self.h))
+        nf = np.empty(nn, np.uint32)
+        occ = np.empty(self.x.size, np.uint32)
+        lib().orc_quadtree_occupants(self.h, _p(nf), _p(occ))
+        return nf, occ
+
+    def forces(self, kernel, G=1.0, m=1.0, eps2=0.0, theta=0.5, nthreads=1):
+        ax, ay = np.zeros(self.x.size), np.zeros(self.x.size)
+        cnt = np.zeros(2, np.uint64)
+        tot = lib().orc_quadtree_forces(self.h, _p(self.x), _p(self.y), int(kernel), float(G),
+                                        float(m), float(eps2), float(theta), _p(ax), _p(ay),
+                                        _p(cnt), int(nthreads))
+        return ax, ay, tot, cnt
+
+
+def leapfrog(x, y, vx, vy, m, steps, dt, G, eps2, theta, semantics=SEM_QUADTREE, leaf_cap=16,
+             bits=32, nthreads=8):
+    x, y, vx, vy = (_f64(a).copy() for a in (x, y, vx, vy))
+    m = None if m is None else _f64(m)
+    lib().orc_leapfrog(_p(x), _p(y), _p(vx), _p(vy), _p(m), x.size, int(steps), float(dt), float(G),
+                       float(eps2), float(theta), int(semantics), int(leaf_cap), int(bits),
+                       int(nthreads))
+    return x, y, vx, vy
+
+
+def tsne_theta(min_theta, max_theta, iteration):
+    return float(lib().orc_tsne_theta(float(min_theta), float(max_theta), int(iteration)))
+
+
+def tsne_attraction(x, y, row_ptr, col, val, exag):
+    x, y = _f64(x), _f64(y)
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    cl = np.ascontiguousarray(col, np.uint32)
+    vl = _f64(val)
+    dx, dy = np.zeros(x.size), np.zeros(x.size)
+    lib().orc_tsne_attraction(_p(x), _p(y), x.size, _p(rp), _p(cl), _p(vl), float(exag), _p(dx),
+                              _p(dy))
+    return dx, dy
+
+
+def tsne_z(x, y, nthreads=8):
+    x, y = _f64(x), _f64(y)
+    return float(lib().orc_tsne_z(_p(x), _p(y), x.size, int(nthreads)))
+
+
+def tsne_kl(x, y, row_ptr, col, val, nthreads=8):
+    x, y = _f64(x), _f64(y)
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    cl = np.ascontiguousarray(col, np.uint32)
+    vl = _f64(val)
+    return float(lib().orc_tsne_kl(_p(x), _p(y), x.size, _p(rp), _p(cl), _p(vl), int(nthreads)))
+
+
+def mlp_forward(X, params):
+    X = _f64(X)
+    B, din = X.shape
+    W1, b1, W2, b2, W3, b3 = (_f64(params[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3"))
+    h, dout = W1.shape[1], W3.shape[1]
+    Y = np.empty((B, dout))
+    lib().orc_mlp_forward(_p(X), B, din, h, dout, _p(W1), _p(b1), _p(W2), _p(b2), _p(W3), _p(b3),
+                          _p(Y))
+    return Y
+
+
+def mlp_loss_grad(X, T, params):
+    X, T = _f64(X), _f64(T)
+    B, din = X.shape
+    W1, b1, W2, b2, W3, b3 = (_f64(params[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3"))
+    h, dout = W1.shape[1], W3.shape[1]
+    g = {k: np.empty_like(v) for k, v in (("W1", W1), ("b1", b1), ("W2", W2), ("b2", b2),
+                                          ("W3", W3), ("b3", b3))}
+    loss = lib().orc_mlp_loss_grad(_p(X), _p(T), B, din, h, dout, _p(W1), _p(b1), _p(W2), _p(b2),
+                                   _p(W3), _p(b3), _p(g["W1"]), _p(g["b1"]), _p(g["W2"]),
+                                   _p(g["b2"]), _p(g["W3"]), _p(g["b3"]))
+    return float(loss), g
+
+
+def adam(p, g, m1, m2, lr, beta1, beta2, eps, t):
+    for a in (p, m1, m2):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    g = _f64(g)
+    lib().orc_adam(_p(p), _p(g), _p(m1), _p(m2), p.size, float(lr), float(beta1), float(beta2),
+                   float(eps), int(t))
+
+
+# ---------------------------------------------------------------------------------------------
+def ref_bh_solve(x, y, kernel, G=1.0, m=1.0, eps2=0.0, theta=0.5, max_children=16):
+    x, y = _f64(x), _f64(y)
+    ax, ay = np.empty(x.size), np.empty(x.size)
+    tb, ts = C.c_double(0), C.c_double(0)
+    tot = ref().ref_bh_solve(_p(x), _p(y), x.size, int(kernel), float(G), float(m), float(eps2),
+                             float(theta), int(max_children), _p(ax), _p(ay), C.byref(tb),
+                             C.byref(ts))
+    return ax, ay, tot, tb.value, ts.value
+
+
+def ref_quadtree(x, y, max_children=16):
+    x, y = _f64(x), _f64(y)
+    R = ref()
+    nn = int(R.ref_quadtree_export(_p(x), _p(y), x.size, int(max_children), *([None] * 10)))
+    d = dict(depth=np.empty(nn, np.int32), count=np.empty(nn, np.uint32),
+             nchild=np.empty(nn, np.uint32), mass=np.empty(nn), comx=np.empty(nn),
+             comy=np.empty(nn), lowx=np.empty(nn), lowy=np.empty(nn), highx=np.empty(nn),
+             highy=np.empty(nn))
+    R.ref_quadtree_export(_p(x), _p(y), x.size, int(max_children),
+                          *[_p(d[k]) for k in ("depth", "count", "nchild", "mass", "comx", "comy",
+                                               "lowx", "lowy", "highx", "highy")])
+    return d
+
+
+def ref_naive_tsne(x, y):
+    x, y = _f64(x), _f64(y)
+    ax, ay = np.empty(x.size), np.empty(x.size)
+    tot = ref().ref_naive_tsne(_p(x), _p(y), x.size, _p(ax), _p(ay))
+    return ax, ay, tot
