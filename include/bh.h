@@ -1,0 +1,143 @@
+/*
+ * bh.h — C ABI of the MI355X (gfx950) Barnes-Hut force step.
+ *
+ * Drop-in boundary for the reference's solver plug-in interface
+ *   template <typename T> class NBodySolver            (src/nbodysolvers/cpu/nBodySolver.h:9-23)
+ *     virtual void updateTree(std::vector<T>& points, glm::dvec2 minPos, glm::dvec2 maxPos);
+ *     virtual void solveNbody(double& total, std::vector<T>& points);
+ *     virtual std::vector<VertexPos2Col3> getNodesBufferData(int level);
+ *     int maxChildren; double theta; double cell_size;
+ * and of its GPU sibling NBodySolverGpu<T>::timeStep/getParticles
+ *   (src/nbodysolvers/gpu/nBodySOlverGpu.cuh:9-33).
+ * Plain C types only; no exceptions cross this boundary. A context owns all device memory,
+ * never permutes caller memory, and returns results in the caller's particle order.
+ * A context is not thread-safe; distinct contexts may be used from different threads.
+ * Every call is synchronous on return unless its name ends in _async.
+ */
+#ifndef MTL_BH_H
+#define MTL_BH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bh_ctx bh_ctx;
+
+/* The reference reports no errors (void returns, CUDA errors unchecked,
+ * cudaHelper.cuh:22-31); these status codes replace that. */
+typedef enum bh_status {
+    BH_OK = 0,
+    BH_ERR_ARG = 1,      /* invalid argument (null pointer, n > n_max, bad config) */
+    BH_ERR_HIP = 2,      /* HIP runtime error; see bh_last_error */
+    BH_ERR_OOM = 3,      /* device allocation failed */
+    BH_ERR_CAPACITY = 4, /* a fixed capacity was exceeded */
+    BH_ERR_STATE = 5     /* call out of order (e.g. forces before a build) */
+} bh_status;
+
+/* interaction kernels: the reference injects them as std::function (nBodySolverBH.h:20-31) */
+typedef enum bh_kernel {
+    BH_KERNEL_GRAVITY = 0, /* a = G sum m_j (x_j - x)/(|x_j - x|^2 + eps^2)^(3/2) */
+    BH_KERNEL_TSNE = 1     /* TSNEBHPNKernel / TSNEBHPPKernel (nBodySolverBH.h:94-116) */
+} bh_kernel;
+
+typedef enum bh_tree_mode {
+    /* QuadTree (quadtree.h:29-171) + traverseBH (nBodySolverBH.h:56-89): adaptive, split while
+     * > max_children, root square from the min corner with side max extent + 1e-4, a node with
+     * at most one child is summed particle by particle over all its occupants. */
+    BH_TREE_QUADTREE = 0,
+    /* NBodySolverFMM_SYM_MORTON (nBodySolverFMM_SYM_MORTON.h:79-151, 272-313, 328-561):
+     * 32-bit Morton codes, fixed depth getDepth(max_children, N), point-node traversal
+     * accepting d^2 > (l/theta)^2, particle-particle at the leaves. */
+    BH_TREE_MORTON_FIXED = 1
+} bh_tree_mode;
+
+typedef struct bh_config {
+    uint32_t n_max;       /* particle capacity of the context */
+    int32_t kernel;       /* bh_kernel */
+    int32_t tree_mode;    /* bh_tree_mode */
+    int32_t max_children; /* NBodySolver::maxChildren, leaf capacity (reference: 16, tsne.h:104) */
+    int32_t fixed_depth;  /* MORTON_FIXED depth; 0 = getDepth(max_children, n) per build */
+    int32_t code_bits;    /* QUADTREE: Morton bits per axis, 16..32 (0 = 32) */
+    float G;              /* gravitational constant */
+    float softening;      /* Plummer softening eps */
+    double theta;         /* NBodySolver::theta, default opening angle */
+} bh_config;
+
+/* Fill cfg with the reference defaults: QuadTree mode, max_children 16, theta 0.5,
+ * t-SNE kernel, 32 code bits. */
+bh_status bh_config_default(bh_config* cfg);
+
+/* Construct / destroy a solver (the reference's `new NBodySolverBH<...>(...)`, tsne.h:104-122;
+ * device buffers are sized for cfg->n_max like NBodySolverGpuBH's constructor,
+ * nBodySolverGpuBH.cu:489-555). */
+bh_status bh_create(bh_ctx** out, const bh_config* cfg);
+void bh_destroy(bh_ctx* ctx);
+const char* bh_last_error(const bh_ctx* ctx);
+
+/* Upload particles in the caller's order (host pointers). m, vx, vy may be NULL:
+ * masses default to 1 (the reference trees count particles, quadtree.h:163), velocities to 0.
+ * The copy of a TsnePoint2D vector's positions that updateTree sees (tsne.h:202). */
+bh_status bh_set_particles(bh_ctx* ctx, uint32_t n, const float* x, const float* y,
+                           const float* m, const float* vx, const float* vy);
+/* same with device pointers (inputs already resident in HBM) */
+bh_status bh_set_particles_device(bh_ctx* ctx, uint32_t n, const float* x, const float* y,
+                                  const float* m, const float* vx, const float* vy);
+/* particle state in the caller's order (host pointers; any may be NULL) */
+bh_status bh_get_particles(bh_ctx* ctx, float* x, float* y, float* vx, float* vy);
+
+/* NBodySolver::updateTree(points, minPos, maxPos) (nBodySolver.h:18). bounds = {minx, miny,
+ * maxx, maxy} as passed by the caller (tsne.h:260-270) or NULL to compute them on the device.
+ * MORTON_FIXED honours caller bounds (nBodySolverFMM_SYM_MORTON.h:79-95); QUADTREE ignores
+ * them like NBodySolverBH::updateTree (nBodySolverBH.h:43-46). */
+bh_status bh_build(bh_ctx* ctx, const double* bounds);
+
+/* NBodySolver::solveNbody(total, points) (nBodySolver.h:17, nBodySolverBH.h:33-41):
+ * forces for the last built tree at opening angle theta. ax/ay (host, caller order, may be
+ * NULL) receive the interaction sums (t-SNE: the unnormalised repulsive derivative; gravity:
+ * the acceleration). total receives the FP64 sum of the per-particle totals (t-SNE: Z;
+ * gravity: sum of particle potentials). */
+bh_status bh_forces(bh_ctx* ctx, double theta, float* ax, float* ay, double* total);
+/* same with device output pointers; total may be NULL */
+bh_status bh_forces_device(bh_ctx* ctx, double theta, float* d_ax, float* d_ay, double* total);
+
+/* Leapfrog kick-drift-kick for `steps` steps, rebuilding the tree every step (the order of
+ * the reference's time step: update points, rebuild, solve; tsne_buffers.h:125-168).
+ * Positions and velocities stay on the device; read them with bh_get_particles. */
+bh_status bh_leapfrog(bh_ctx* ctx, int steps, float dt, double theta);
+
+/* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
+/* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */
+bh_status bh_get_bounds(bh_ctx* ctx, double* bounds, double* root_side);
+/* sorted Morton codes (createMortonCode, nBodySolverFMM_SYM_MORTON.h:538-552; 64-bit slots
+ * holding 32-bit codes in MORTON_FIXED mode) and the sort permutation (original indices in
+ * sorted order, integer_sort at :89 with ties by original index) */
+bh_status bh_get_morton(bh_ctx* ctx, uint64_t* codes_sorted, uint32_t* perm);
+/* node count, deepest leaf level and leaves at the depth limit holding > max_children */
+bh_status bh_tree_info(bh_ctx* ctx, uint32_t* n_nodes, uint32_t* max_level,
+                       uint32_t* overfull_leaves);
+/* the pre-order tree: level, first sorted particle, count, skip, flags (bit0 leaf, bit1
+ * particle-particle near field), centre of mass, mass; arrays of n_nodes (any may be NULL) */
+bh_status bh_get_tree(bh_ctx* ctx, int32_t* level, uint32_t* first, uint32_t* count,
+                      uint32_t* skip, uint8_t* flags, float* cx, float* cy, float* mass);
+/* MORTON_FIXED only: the reference's slot arrays (firstChildIndex, firstParticleIndex,
+ * particleIndexAmount of NodeFMM_MORTON_2D, nodeFMM_MORTON_2D.h:14-16) for all
+ * (4^(depth+1)-1)/3 slots */
+bh_status bh_get_tree_slots(bh_ctx* ctx, int32_t* depth, uint64_t* n_slots, uint32_t* first_child,
+                            uint32_t* first_particle, uint32_t* count);
+/* interaction counters of the last force evaluation: particle-node and particle-particle */
+bh_status bh_counters(bh_ctx* ctx, uint64_t* pn, uint64_t* pp);
+
+/* ---- timing (HIP events on the context's stream) ---- */
+/* phases: 0 bounds+morton, 1 sort, 2 gather, 3 tree, 4 upward, 5 traversal, 6 integrate */
+#define BH_NUM_PHASES 7
+bh_status bh_profile_enable(bh_ctx* ctx, int enable);
+/* accumulated milliseconds and launch counts per phase since the last reset */
+bh_status bh_profile_read(bh_ctx* ctx, double* ms, uint64_t* launches, int reset);
+bh_status bh_synchronize(bh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTL_BH_H */

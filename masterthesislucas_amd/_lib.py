@@ -1,0 +1,89 @@
+"""ctypes binding of the C ABI in include/bh.h (libmtl_bh.so, built in-tree).
+
+There is no CPU fallback: if the library is missing or the GPU is absent, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmtl_bh.so")
+
+BH_OK = 0
+STATUS_NAMES = {0: "BH_OK", 1: "BH_ERR_ARG", 2: "BH_ERR_HIP", 3: "BH_ERR_OOM",
+                4: "BH_ERR_CAPACITY", 5: "BH_ERR_STATE"}
+
+KERNEL_GRAVITY = 0
+KERNEL_TSNE = 1
+TREE_QUADTREE = 0
+TREE_MORTON_FIXED = 1
+NUM_PHASES = 7
+PHASE_NAMES = ("bounds+morton", "sort", "gather", "tree", "upward", "traversal", "integrate")
+
+
+class BHConfig(C.Structure):
+    _fields_ = [("n_max", C.c_uint32), ("kernel", C.c_int32), ("tree_mode", C.c_int32),
+                ("max_children", C.c_int32), ("fixed_depth", C.c_int32),
+                ("code_bits", C.c_int32), ("G", C.c_float), ("softening", C.c_float),
+                ("theta", C.c_double)]
+
+
+class BHError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_vp = C.c_void_p
+_u32, _i32, _u64, _d, _f = C.c_uint32, C.c_int32, C.c_uint64, C.c_double, C.c_float
+
+# name -> (restype, argtypes); the exported symbol set of include/bh.h
+SIGNATURES = {
+    "bh_config_default": (_i32, [C.POINTER(BHConfig)]),
+    "bh_create": (_i32, [C.POINTER(_vp), C.POINTER(BHConfig)]),
+    "bh_destroy": (None, [_vp]),
+    "bh_last_error": (C.c_char_p, [_vp]),
+    "bh_set_particles": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "bh_set_particles_device": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "bh_get_particles": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "bh_build": (_i32, [_vp, _vp]),
+    "bh_forces": (_i32, [_vp, _d, _vp, _vp, _vp]),
+    "bh_forces_device": (_i32, [_vp, _d, _vp, _vp, _vp]),
+    "bh_leapfrog": (_i32, [_vp, C.c_int, _f, _d]),
+    "bh_get_bounds": (_i32, [_vp, _vp, _vp]),
+    "bh_get_morton": (_i32, [_vp, _vp, _vp]),
+    "bh_tree_info": (_i32, [_vp, _vp, _vp, _vp]),
+    "bh_get_tree": (_i32, [_vp] + [_vp] * 8),
+    "bh_get_tree_slots": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "bh_counters": (_i32, [_vp, _vp, _vp]),
+    "bh_profile_enable": (_i32, [_vp, C.c_int]),
+    "bh_profile_read": (_i32, [_vp, _vp, _vp, C.c_int]),
+    "bh_synchronize": (_i32, [_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libmtl_bh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, ctx=None):
+    if status != BH_OK:
+        msg = ""
+        if ctx:
+            m = load().bh_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise BHError(status, msg)

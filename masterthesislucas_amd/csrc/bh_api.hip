@@ -1,0 +1,671 @@
+// C ABI of the Barnes-Hut force step (declarations and reference citations in include/bh.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bh.h"
+#include "bh_common.h"
+#include "bh_launch.h"
+
+using namespace bh;
+
+namespace {
+
+struct PhaseTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
+    double ms = 0.0;
+    uint64_t launches = 0;
+};
+
+// getDepth (nBodySolverFMM_SYM_MORTON.h:106-115), N/max_children an int division; 0 when
+// N < max_children (the reference casts -inf to unsigned there).
+int reference_depth(int max_children, uint32_t n) {
+    const int q = (int)n / max_children;
+    if (q <= 0) return 0;
+    const double circle_area = M_PI / 4.0;
+    const double depth_real = std::log((1.0 / circle_area) * (double)q) / std::log(4.0);
+    const double c = std::ceil(depth_real);
+    return c < 0 ? 0 : (int)c;
+}
+
+}  // namespace
+
+struct bh_ctx {
+    bh_config cfg{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    uint32_t n = 0;
+    int cur = 0;
+    bool built = false;
+    bool have_acc = false;  // ax/ay valid for the current order (leapfrog)
+    bool wide = true;       // 64-bit codes
+    int B = 32;             // code bits per axis
+    int depth = 0;          // MORTON_FIXED depth of the last build
+    bool keys_in_b = false;
+    uint32_t cap_nodes = 0;
+    int max_fixed_depth = 0;
+
+    float *x[2]{}, *y[2]{}, *m[2]{}, *vx[2]{}, *vy[2]{};
+    uint32_t* id[2]{};
+    float4* pos4 = nullptr;
+    float *ax = nullptr, *ay = nullptr, *tot = nullptr;
+    float *out_a = nullptr, *out_b = nullptr, *out_c = nullptr, *out_d = nullptr;
+    void *keys_a = nullptr, *keys_b = nullptr;
+    uint32_t *vals_a = nullptr, *vals_b = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t* scan_part = nullptr;
+    uint8_t *leaf_lvl = nullptr, *lcp1 = nullptr;
+    uint32_t *cnt = nullptr, *noff = nullptr;
+    Node* nodes = nullptr;
+    uint32_t *parent = nullptr, *arrived = nullptr;
+    float4* m2 = nullptr;
+    BuildParams* prm = nullptr;
+    float4* bounds_part = nullptr;
+    uint32_t* bounds_counter = nullptr;
+    double* red_part = nullptr;
+    uint32_t* red_counter = nullptr;
+    double* dsum = nullptr;
+    unsigned long long* counters = nullptr;
+    uint32_t *slot_a = nullptr, *slot_b = nullptr, *slot_c = nullptr;
+    uint64_t slot_cap = 0;
+
+    BuildParams* h_prm = nullptr;  // pinned
+    double* h_dsum = nullptr;
+    unsigned long long* h_counters = nullptr;
+
+    bool profile = false;
+    PhaseTimer phases[BH_NUM_PHASES];
+
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+bh_status fail(bh_ctx* c, bh_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return st;
+}
+
+bh_status hip_check(bh_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return BH_OK;
+    return fail(c, e == hipErrorOutOfMemory ? BH_ERR_OOM : BH_ERR_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+template <typename T>
+bh_status dalloc(bh_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T) ? count * sizeof(T) : 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return hip_check(c, e, "hipMalloc");
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return BH_OK;
+}
+
+void phase_begin(bh_ctx* c, int ph) {
+    if (!c->profile) return;
+    PhaseTimer& t = c->phases[ph];
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!t.spare.empty()) {
+        ev = t.spare.back();
+        t.spare.pop_back();
+    } else {
+        (void)hipEventCreate(&ev.first);
+        (void)hipEventCreate(&ev.second);
+    }
+    (void)hipEventRecord(ev.first, c->stream);
+    t.pending.push_back(ev);
+}
+
+void phase_end(bh_ctx* c, int ph) {
+    if (!c->profile) return;
+    PhaseTimer& t = c->phases[ph];
+    (void)hipEventRecord(t.pending.back().second, c->stream);
+    t.launches++;
+}
+
+void phase_collect(bh_ctx* c) {
+    for (int p = 0; p < BH_NUM_PHASES; ++p) {
+        PhaseTimer& t = c->phases[p];
+        for (auto& ev : t.pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(ev.second) == hipSuccess &&
+                hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess)
+                t.ms += ms;
+            t.spare.push_back(ev);
+        }
+        t.pending.clear();
+    }
+}
+
+bh_status launch_status(bh_ctx* c, const char* what) {
+    return hip_check(c, hipGetLastError(), what);
+}
+
+int effective_depth(const bh_ctx* c) {
+    return c->cfg.fixed_depth > 0 ? c->cfg.fixed_depth : reference_depth(c->cfg.max_children, c->n);
+}
+
+bh_status do_build(bh_ctx* c, const double* bounds) {
+    const uint32_t n = c->n;
+    if (n == 0) return fail(c, BH_ERR_STATE, "no particles");
+    hipStream_t s = c->stream;
+    const bool quadtree = c->cfg.tree_mode == BH_TREE_QUADTREE;
+    const int cur = c->cur;
+    int fixed = -1;
+    if (!quadtree) {
+        fixed = effective_depth(c);
+        if (fixed > 16) return fail(c, BH_ERR_ARG, "fixed depth %d exceeds 16 (32-bit codes)", fixed);
+        if (fixed > c->max_fixed_depth)
+            return fail(c, BH_ERR_CAPACITY, "fixed depth %d above the capacity planned (%d)", fixed,
+                        c->max_fixed_depth);
+    }
+
+    phase_begin(c, 0);
+    launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->bounds_counter, c->prm,
+                  quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
+    launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a, c->wide);
+    phase_end(c, 0);
+
+    phase_begin(c, 1);
+    bool in_b = false;
+    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, 2 * c->B, c->wide, c->hist,
+                      c->scan_part, &in_b);
+    phase_end(c, 1);
+    c->keys_in_b = in_b;
+    const void* codes = in_b ? c->keys_b : c->keys_a;
+    const uint32_t* perm = in_b ? c->vals_b : c->vals_a;
+
+    phase_begin(c, 2);
+    const int nxt = 1 - cur;
+    launch_gather(s, perm, n, c->x[cur], c->y[cur], c->m[cur], c->vx[cur], c->vy[cur], c->id[cur],
+                  c->x[nxt], c->y[nxt], c->m[nxt], c->vx[nxt], c->vy[nxt], c->id[nxt], c->pos4);
+    phase_end(c, 2);
+    c->cur = nxt;
+
+    if (!quadtree) c->depth = fixed;
+    phase_begin(c, 3);
+    launch_tree(s, codes, c->wide, n, c->B, c->cfg.max_children, fixed, quadtree ? 1 : 0,
+                c->leaf_lvl, c->lcp1, c->cnt, c->noff, c->scan_part, c->prm, c->nodes, c->parent,
+                c->arrived, c->m2, c->cap_nodes);
+    phase_end(c, 3);
+
+    phase_begin(c, 4);
+    launch_upward(s, c->pos4, c->nodes, c->parent, c->arrived, c->m2, c->prm, c->cap_nodes);
+    phase_end(c, 4);
+    bh_status st = launch_status(c, "build launch");
+    if (st != BH_OK) return st;
+    c->built = true;
+    c->have_acc = false;
+    return BH_OK;
+}
+
+bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
+    if (!c->built) return fail(c, BH_ERR_STATE, "forces requested before a tree was built");
+    if (!(theta >= 0.0)) return fail(c, BH_ERR_ARG, "theta must be >= 0");
+    hipStream_t s = c->stream;
+    launch_fill_u32(s, (uint32_t*)c->counters, 4, 0u);
+    const float inv_theta2 = (float)(1.0 / (theta * theta));
+    const float eps = c->cfg.softening;
+    const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
+    phase_begin(c, 5);
+    launch_traverse(s, grav ? kKernelGravity : kKernelTsne, c->pos4, c->nodes, c->prm, c->n,
+                    inv_theta2, eps * eps, grav ? c->cfg.G : 1.0f, c->ax, c->ay,
+                    want_total ? c->tot : nullptr, c->counters);
+    phase_end(c, 5);
+    if (want_total)
+        launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->red_counter, c->dsum);
+    c->have_acc = true;
+    return launch_status(c, "traverse launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+bh_status bh_config_default(bh_config* cfg) {
+    if (!cfg) return BH_ERR_ARG;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->n_max = 0;
+    cfg->kernel = BH_KERNEL_TSNE;
+    cfg->tree_mode = BH_TREE_QUADTREE;
+    cfg->max_children = 16;
+    cfg->fixed_depth = 0;
+    cfg->code_bits = 32;
+    cfg->G = 1.0f;
+    cfg->softening = 0.0f;
+    cfg->theta = 0.5;
+    return BH_OK;
+}
+
+const char* bh_last_error(const bh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void bh_destroy(bh_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->h_prm) (void)hipHostFree(c->h_prm);
+    if (c->h_dsum) (void)hipHostFree(c->h_dsum);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    for (auto& t : c->phases) {
+        for (auto& e : t.pending) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (auto& e : t.spare) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
+    if (!out || !cfg) return BH_ERR_ARG;
+    *out = nullptr;
+    if (cfg->n_max == 0 || cfg->n_max > (1u << 30)) return BH_ERR_ARG;
+    if (cfg->kernel != BH_KERNEL_GRAVITY && cfg->kernel != BH_KERNEL_TSNE) return BH_ERR_ARG;
+    if (cfg->tree_mode != BH_TREE_QUADTREE && cfg->tree_mode != BH_TREE_MORTON_FIXED)
+        return BH_ERR_ARG;
+    if (cfg->max_children < 1 || cfg->max_children > kMaxLeafCap) return BH_ERR_ARG;
+    if (cfg->fixed_depth < 0 || cfg->fixed_depth > 16) return BH_ERR_ARG;
+    const int bits = cfg->code_bits ? cfg->code_bits : 32;
+    if (bits < 16 || bits > 32) return BH_ERR_ARG;
+
+    bh_ctx* c = new (std::nothrow) bh_ctx();
+    if (!c) return BH_ERR_OOM;
+    c->cfg = *cfg;
+    c->cfg.code_bits = bits;
+    c->wide = cfg->tree_mode == BH_TREE_QUADTREE;
+    c->B = c->wide ? bits : 16;
+
+    bh_status st;
+#define TRY(expr)                 \
+    do {                          \
+        st = (expr);              \
+        if (st != BH_OK) {        \
+            *out = c;             \
+            return st;            \
+        }                         \
+    } while (0)
+    TRY(hip_check(c, hipGetDevice(&c->device), "hipGetDevice"));
+    TRY(hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"));
+    const uint32_t N = cfg->n_max;
+    if (c->wide) {
+        // cells of one level are disjoint and an internal cell holds > max_children particles:
+        // at most N leaves plus B * N / (max_children + 1) internal cells
+        const uint64_t capn = (uint64_t)N + (uint64_t)c->B * N / (uint64_t)(cfg->max_children + 1) + 64;
+        if (capn > 0xFFFFFFF0ull) {
+            *out = c;
+            return fail(c, BH_ERR_CAPACITY, "node capacity overflows 32 bits");
+        }
+        c->cap_nodes = (uint32_t)capn;
+    } else {
+        const int dcap = std::max(cfg->fixed_depth, reference_depth(cfg->max_children, N));
+        c->max_fixed_depth = std::max(dcap, 1);
+        const uint64_t slots = ((1ull << (2 * (c->max_fixed_depth + 1))) - 1) / 3;
+        const uint64_t bound = (uint64_t)(c->max_fixed_depth + 1) * N + 1;
+        c->cap_nodes = (uint32_t)std::min(slots, bound);
+        c->slot_cap = slots;
+    }
+    for (int k = 0; k < 2; ++k) {
+        TRY(dalloc(c, &c->x[k], N));
+        TRY(dalloc(c, &c->y[k], N));
+        TRY(dalloc(c, &c->m[k], N));
+        TRY(dalloc(c, &c->vx[k], N));
+        TRY(dalloc(c, &c->vy[k], N));
+        TRY(dalloc(c, &c->id[k], N));
+    }
+    TRY(dalloc(c, &c->pos4, N));
+    TRY(dalloc(c, &c->ax, N));
+    TRY(dalloc(c, &c->ay, N));
+    TRY(dalloc(c, &c->tot, N));
+    TRY(dalloc(c, &c->out_a, N));
+    TRY(dalloc(c, &c->out_b, N));
+    TRY(dalloc(c, &c->out_c, N));
+    TRY(dalloc(c, &c->out_d, N));
+    const size_t ksz = c->wide ? sizeof(uint64_t) : sizeof(uint32_t);
+    {
+        void* q;
+        TRY(hip_check(c, hipMalloc(&q, (size_t)N * ksz), "hipMalloc keys"));
+        c->allocs.push_back(q);
+        c->keys_a = q;
+        TRY(hip_check(c, hipMalloc(&q, (size_t)N * ksz), "hipMalloc keys"));
+        c->allocs.push_back(q);
+        c->keys_b = q;
+    }
+    TRY(dalloc(c, &c->vals_a, N));
+    TRY(dalloc(c, &c->vals_b, N));
+    const uint32_t nh = sort_blocks(N) * 256u;
+    TRY(dalloc(c, &c->hist, nh));
+    TRY(dalloc(c, &c->scan_part, std::max(scan_partials_size(N), scan_partials_size(nh)) + 256));
+    TRY(dalloc(c, &c->leaf_lvl, N));
+    TRY(dalloc(c, &c->lcp1, N));
+    TRY(dalloc(c, &c->cnt, N));
+    TRY(dalloc(c, &c->noff, N));
+    TRY(dalloc(c, &c->nodes, c->cap_nodes));
+    TRY(dalloc(c, &c->parent, c->cap_nodes));
+    TRY(dalloc(c, &c->arrived, c->cap_nodes));
+    TRY(dalloc(c, &c->prm, 1));
+    TRY(dalloc(c, &c->bounds_part, kMaxBoundsBlocks));
+    TRY(dalloc(c, &c->bounds_counter, 1));
+    TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
+    TRY(dalloc(c, &c->red_counter, 1));
+    TRY(dalloc(c, &c->dsum, 1));
+    TRY(dalloc(c, &c->counters, 2));
+    if (!c->wide) {
+        TRY(dalloc(c, &c->slot_a, c->slot_cap));
+        TRY(dalloc(c, &c->slot_b, c->slot_cap));
+        TRY(dalloc(c, &c->slot_c, c->slot_cap));
+    }
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_prm, sizeof(BuildParams), 0), "hipHostMalloc"));
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_dsum, sizeof(double), 0), "hipHostMalloc"));
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_counters, 2 * sizeof(unsigned long long), 0),
+                  "hipHostMalloc"));
+    launch_fill_u32(c->stream, c->bounds_counter, 1, 0u);
+    launch_fill_u32(c->stream, c->red_counter, 1, 0u);
+    launch_fill_u32(c->stream, (uint32_t*)c->prm, sizeof(BuildParams) / 4, 0u);
+    TRY(hip_check(c, hipStreamSynchronize(c->stream), "init"));
+#undef TRY
+    *out = c;
+    return BH_OK;
+}
+
+static bh_status set_particles(bh_ctx* c, uint32_t n, const float* x, const float* y,
+                               const float* m, const float* vx, const float* vy, bool device) {
+    if (!c || !x || !y) return BH_ERR_ARG;
+    if (n > c->cfg.n_max) return fail(c, BH_ERR_ARG, "n=%u exceeds n_max=%u", n, c->cfg.n_max);
+    if ((vx == nullptr) != (vy == nullptr)) return fail(c, BH_ERR_ARG, "vx and vy go together");
+    hipStream_t s = c->stream;
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (device) {
+        bh_status st = hip_check(c, hipDeviceSynchronize(), "sync before device upload");
+        if (st != BH_OK) return st;
+    }
+    c->cur = 0;
+    c->n = n;
+    c->built = false;
+    c->have_acc = false;
+    if (n == 0) return BH_OK;
+    const size_t b = (size_t)n * sizeof(float);
+    bh_status st = hip_check(c, hipMemcpyAsync(c->x[0], x, b, kind, s), "upload x");
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(c->y[0], y, b, kind, s), "upload y");
+    if (st != BH_OK) return st;
+    if (m)
+        st = hip_check(c, hipMemcpyAsync(c->m[0], m, b, kind, s), "upload m");
+    else
+        launch_fill_f32(s, c->m[0], n, 1.0f);
+    if (st != BH_OK) return st;
+    if (vx) {
+        st = hip_check(c, hipMemcpyAsync(c->vx[0], vx, b, kind, s), "upload vx");
+        if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(c->vy[0], vy, b, kind, s), "upload vy");
+        if (st != BH_OK) return st;
+    } else {
+        launch_fill_f32(s, c->vx[0], n, 0.0f);
+        launch_fill_f32(s, c->vy[0], n, 0.0f);
+    }
+    launch_iota(s, c->id[0], n);
+    return hip_check(c, hipStreamSynchronize(s), "upload");
+}
+
+bh_status bh_set_particles(bh_ctx* c, uint32_t n, const float* x, const float* y, const float* m,
+                           const float* vx, const float* vy) {
+    return set_particles(c, n, x, y, m, vx, vy, false);
+}
+
+bh_status bh_set_particles_device(bh_ctx* c, uint32_t n, const float* x, const float* y,
+                                  const float* m, const float* vx, const float* vy) {
+    return set_particles(c, n, x, y, m, vx, vy, true);
+}
+
+bh_status bh_get_particles(bh_ctx* c, float* x, float* y, float* vx, float* vy) {
+    if (!c) return BH_ERR_ARG;
+    const uint32_t n = c->n;
+    if (n == 0) return BH_OK;
+    hipStream_t s = c->stream;
+    const int cur = c->cur;
+    launch_unsort2(s, c->id[cur], n, c->x[cur], c->y[cur], c->out_a, c->out_b);
+    launch_unsort2(s, c->id[cur], n, c->vx[cur], c->vy[cur], c->out_c, c->out_d);
+    const size_t b = (size_t)n * sizeof(float);
+    bh_status st = BH_OK;
+    if (x) st = hip_check(c, hipMemcpyAsync(x, c->out_a, b, hipMemcpyDeviceToHost, s), "x");
+    if (st == BH_OK && y) st = hip_check(c, hipMemcpyAsync(y, c->out_b, b, hipMemcpyDeviceToHost, s), "y");
+    if (st == BH_OK && vx) st = hip_check(c, hipMemcpyAsync(vx, c->out_c, b, hipMemcpyDeviceToHost, s), "vx");
+    if (st == BH_OK && vy) st = hip_check(c, hipMemcpyAsync(vy, c->out_d, b, hipMemcpyDeviceToHost, s), "vy");
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(s), "get particles");
+}
+
+bh_status bh_build(bh_ctx* c, const double* bounds) {
+    if (!c) return BH_ERR_ARG;
+    bh_status st = do_build(c, bounds);
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), "build");
+}
+
+static bh_status forces_out(bh_ctx* c, double theta, float* ax, float* ay, double* total,
+                            bool device) {
+    if (!c) return BH_ERR_ARG;
+    bh_status st = do_forces(c, theta, total != nullptr);
+    if (st != BH_OK) return st;
+    hipStream_t s = c->stream;
+    const uint32_t n = c->n;
+    const int cur = c->cur;
+    const size_t b = (size_t)n * sizeof(float);
+    if (ax || ay) {
+        if (device) {
+            launch_unsort2(s, c->id[cur], n, c->ax, c->ay, ax ? ax : c->out_a, ay ? ay : c->out_b);
+        } else {
+            launch_unsort2(s, c->id[cur], n, c->ax, c->ay, c->out_a, c->out_b);
+            if (ax) st = hip_check(c, hipMemcpyAsync(ax, c->out_a, b, hipMemcpyDeviceToHost, s), "ax");
+            if (st == BH_OK && ay)
+                st = hip_check(c, hipMemcpyAsync(ay, c->out_b, b, hipMemcpyDeviceToHost, s), "ay");
+            if (st != BH_OK) return st;
+        }
+    }
+    if (total) {
+        st = hip_check(c, hipMemcpyAsync(c->h_dsum, c->dsum, sizeof(double), hipMemcpyDeviceToHost, s),
+                       "total");
+        if (st != BH_OK) return st;
+    }
+    st = hip_check(c, hipStreamSynchronize(s), "forces");
+    if (st != BH_OK) return st;
+    if (total) *total = *c->h_dsum;
+    return BH_OK;
+}
+
+bh_status bh_forces(bh_ctx* c, double theta, float* ax, float* ay, double* total) {
+    return forces_out(c, theta, ax, ay, total, false);
+}
+
+bh_status bh_forces_device(bh_ctx* c, double theta, float* d_ax, float* d_ay, double* total) {
+    return forces_out(c, theta, d_ax, d_ay, total, true);
+}
+
+bh_status bh_leapfrog(bh_ctx* c, int steps, float dt, double theta) {
+    if (!c || steps < 0) return BH_ERR_ARG;
+    if (c->n == 0) return fail(c, BH_ERR_STATE, "no particles");
+    bh_status st;
+    if (!c->have_acc) {
+        st = do_build(c, nullptr);
+        if (st == BH_OK) st = do_forces(c, theta, false);
+        if (st != BH_OK) return st;
+    }
+    const float hdt = 0.5f * dt;
+    for (int k = 0; k < steps; ++k) {
+        const int cur = c->cur;
+        phase_begin(c, 6);
+        launch_kick_drift(c->stream, c->x[cur], c->y[cur], c->vx[cur], c->vy[cur], c->ax, c->ay,
+                          c->n, hdt, dt);
+        phase_end(c, 6);
+        st = do_build(c, nullptr);
+        if (st == BH_OK) st = do_forces(c, theta, false);
+        if (st != BH_OK) return st;
+        const int nc = c->cur;
+        phase_begin(c, 6);
+        launch_kick(c->stream, c->vx[nc], c->vy[nc], c->ax, c->ay, c->n, hdt);
+        phase_end(c, 6);
+    }
+    st = launch_status(c, "leapfrog launch");
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), "leapfrog");
+}
+
+bh_status bh_get_bounds(bh_ctx* c, double* bounds, double* root_side) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
+    bh_status st = hip_check(c, hipMemcpy(c->h_prm, c->prm, sizeof(BuildParams), hipMemcpyDeviceToHost),
+                             "params");
+    if (st != BH_OK) return st;
+    auto d = [](uint64_t b) {
+        double v;
+        memcpy(&v, &b, 8);
+        return v;
+    };
+    if (bounds) {
+        bounds[0] = d(c->h_prm->minx64);
+        bounds[1] = d(c->h_prm->miny64);
+        bounds[2] = c->h_prm->maxx;
+        bounds[3] = c->h_prm->maxy;
+    }
+    if (root_side) *root_side = d(c->h_prm->side64);
+    return BH_OK;
+}
+
+bh_status bh_get_morton(bh_ctx* c, uint64_t* codes_sorted, uint32_t* perm) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
+    const uint32_t n = c->n;
+    bh_status st = BH_OK;
+    const void* keys = c->keys_in_b ? c->keys_b : c->keys_a;
+    const uint32_t* vals = c->keys_in_b ? c->vals_b : c->vals_a;
+    if (codes_sorted) {
+        if (c->wide) {
+            st = hip_check(c, hipMemcpy(codes_sorted, keys, (size_t)n * 8, hipMemcpyDeviceToHost), "codes");
+        } else {
+            std::vector<uint32_t> tmp(n);
+            st = hip_check(c, hipMemcpy(tmp.data(), keys, (size_t)n * 4, hipMemcpyDeviceToHost), "codes");
+            for (uint32_t i = 0; i < n; ++i) codes_sorted[i] = tmp[i];
+        }
+    }
+    if (st == BH_OK && perm) {
+        // ids of the sorted state are the caller's original indices
+        st = hip_check(c, hipMemcpy(perm, c->id[c->cur], (size_t)n * 4, hipMemcpyDeviceToHost), "perm");
+    }
+    (void)vals;
+    return st;
+}
+
+bh_status bh_tree_info(bh_ctx* c, uint32_t* n_nodes, uint32_t* max_level, uint32_t* overfull) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
+    bh_status st = hip_check(c, hipMemcpy(c->h_prm, c->prm, sizeof(BuildParams), hipMemcpyDeviceToHost),
+                             "params");
+    if (st != BH_OK) return st;
+    if (n_nodes) *n_nodes = c->h_prm->n_nodes;
+    if (max_level) *max_level = c->h_prm->max_level;
+    if (overfull) *overfull = c->h_prm->overfull_leaves;
+    return BH_OK;
+}
+
+bh_status bh_get_tree(bh_ctx* c, int32_t* level, uint32_t* first, uint32_t* count, uint32_t* skip,
+                      uint8_t* flags, float* cx, float* cy, float* mass) {
+    uint32_t nn = 0;
+    bh_status st = bh_tree_info(c, &nn, nullptr, nullptr);
+    if (st != BH_OK) return st;
+    std::vector<Node> h(nn);
+    if (nn) {
+        st = hip_check(c, hipMemcpy(h.data(), c->nodes, (size_t)nn * sizeof(Node), hipMemcpyDeviceToHost),
+                       "nodes");
+        if (st != BH_OK) return st;
+    }
+    for (uint32_t k = 0; k < nn; ++k) {
+        if (level) level[k] = (int32_t)(h[k].meta & kMetaLevelMask);
+        if (first) first[k] = h[k].first;
+        if (count) count[k] = h[k].count;
+        if (skip) skip[k] = h[k].skip;
+        if (flags) flags[k] = (uint8_t)(((h[k].meta & kMetaLeaf) ? 1 : 0) | ((h[k].meta & kMetaPP) ? 2 : 0));
+        if (cx) cx[k] = h[k].cx;
+        if (cy) cy[k] = h[k].cy;
+        if (mass) mass[k] = h[k].mass;
+    }
+    return BH_OK;
+}
+
+bh_status bh_get_tree_slots(bh_ctx* c, int32_t* depth, uint64_t* n_slots, uint32_t* first_child,
+                            uint32_t* first_particle, uint32_t* count) {
+    if (!c) return BH_ERR_ARG;
+    if (c->wide) return fail(c, BH_ERR_STATE, "slot layout exists only in MORTON_FIXED mode");
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
+    const uint64_t slots = ((1ull << (2 * (c->depth + 1))) - 1) / 3;
+    if (depth) *depth = c->depth;
+    if (n_slots) *n_slots = slots;
+    if (!first_child && !first_particle && !count) return BH_OK;
+    const void* keys = c->keys_in_b ? c->keys_b : c->keys_a;
+    launch_tree_slots(c->stream, (const uint32_t*)keys, c->nodes, c->prm, c->cap_nodes, c->depth,
+                      c->slot_a, c->slot_b, c->slot_c, slots);
+    bh_status st = launch_status(c, "slots");
+    if (st != BH_OK) return st;
+    const size_t b = slots * 4;
+    if (first_child) st = hip_check(c, hipMemcpyAsync(first_child, c->slot_a, b, hipMemcpyDeviceToHost, c->stream), "slots");
+    if (st == BH_OK && first_particle)
+        st = hip_check(c, hipMemcpyAsync(first_particle, c->slot_b, b, hipMemcpyDeviceToHost, c->stream), "slots");
+    if (st == BH_OK && count)
+        st = hip_check(c, hipMemcpyAsync(count, c->slot_c, b, hipMemcpyDeviceToHost, c->stream), "slots");
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), "slots");
+}
+
+bh_status bh_counters(bh_ctx* c, uint64_t* pn, uint64_t* pp) {
+    if (!c) return BH_ERR_ARG;
+    bh_status st = hip_check(c, hipMemcpy(c->h_counters, c->counters, 2 * sizeof(unsigned long long),
+                                          hipMemcpyDeviceToHost), "counters");
+    if (st != BH_OK) return st;
+    if (pn) *pn = c->h_counters[0];
+    if (pp) *pp = c->h_counters[1];
+    return BH_OK;
+}
+
+bh_status bh_profile_enable(bh_ctx* c, int enable) {
+    if (!c) return BH_ERR_ARG;
+    c->profile = enable != 0;
+    return BH_OK;
+}
+
+bh_status bh_profile_read(bh_ctx* c, double* ms, uint64_t* launches, int reset) {
+    if (!c) return BH_ERR_ARG;
+    bh_status st = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
+    if (st != BH_OK) return st;
+    phase_collect(c);
+    for (int p = 0; p < BH_NUM_PHASES; ++p) {
+        if (ms) ms[p] = c->phases[p].ms;
+        if (launches) launches[p] = c->phases[p].launches;
+        if (reset) {
+            c->phases[p].ms = 0.0;
+            c->phases[p].launches = 0;
+        }
+    }
+    return BH_OK;
+}
+
+bh_status bh_synchronize(bh_ctx* c) {
+    if (!c) return BH_ERR_ARG;
+    return hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
+}
+
+}  // extern "C"
