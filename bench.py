@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the Barnes-Hut force step on one MI355X.
+
+Workload: two Plummer disks (scale radius 0.1, 524,288 particles each) centred at (+-0.5, 0) on
+counter-rotating circular orbits, N = 1,048,576 FP32 particles, G = 1, Plummer softening 1e-3,
+QuadTree tree (leaf capacity 16), theta = 0.5, kick-drift-kick leapfrog with dt = 1e-4. One
+step = bounds, Morton codes, radix sort, gather, tree build, upward pass, traversal and the
+leapfrog update, with the particle state resident in HBM. Metric: force evaluations per second
+(particles x steps / wall time).
+
+Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_INTERACTION = {"gravity": 14, "tsne": 15}
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, theta, eps, n):
+    """The reference CPU Barnes-Hut (NBodySolverBH + QuadTree, compiled from the reference
+    sources into oracle/_ref) timed on this host, one core: one updateTree + solveNbody step on
+    the benchmark's initial state. Falls back to the oracle's C restatement if the compiled
+    reference is absent."""
+    from oracle import oracle as O
+
+    x = p.x.astype(np.float64)
+    y = p.y.astype(np.float64)
+    if O.ref_available():
+        _, _, _, tb, ts = O.ref_bh_solve(x, y, O.KERNEL_GRAVITY, G=1.0, m=1.0 / n, eps2=eps * eps,
+                                         theta=theta, max_children=16)
+        kind = "reference"
+        secs = tb + ts
+    else:
+        t0 = time.perf_counter()
+        qt = O.QuadTree(x, y, 16)
+        qt.forces(O.KERNEL_GRAVITY, G=1.0, m=1.0 / n, eps2=eps * eps, theta=theta, nthreads=1)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    return {"value": n / secs, "unit": "force evals/s", "cores": 1, "kind": kind,
+            "seconds": secs,
+            "sample": f"one updateTree + solveNbody step (QuadTree, leaf capacity 16, "
+                      f"theta={theta}) over all {n} particles of the benchmark's initial state, "
+                      f"FP64, single thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--code-bits", type=int, default=32)
+    ap.add_argument("--error-sample", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs the Barnes-Hut step on one GPU only; --gpus {args.gpus} "
+              f"(WORLD_SIZE={world}) is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from masterthesislucas_amd import BHContext
+    from masterthesislucas_amd.workloads import sample_indices, two_plummer_disks
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    n, theta, eps, dt = args.n, args.theta, 1e-3, args.dt
+    log(f"[bench] generating two Plummer disks n={n}")
+    p = two_plummer_disks(n)
+    ctx = BHContext(n, kernel="gravity", tree_mode="quadtree", max_children=16,
+                    code_bits=args.code_bits, G=1.0, softening=eps)
+    ctx.set_particles(p.x, p.y, p.m, p.vx, p.vy)
+    log(f"[bench] warmup {args.warmup} steps")
+    ctx.leapfrog(args.warmup, dt, theta)
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    ctx.leapfrog(args.steps, dt, theta)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = ctx.profile_read(reset=True)
+    ctx.profile_enable(False)
+    wall = t1 - t0
+    pn, pp = ctx.counters()  # interactions of the last step's force evaluation
+    log(f"[bench] {wall:.4f} s for {args.steps} steps; pn={pn} pp={pp}")
+
+    trav_ms, trav_launches = phases["traversal"]
+    trav_avg_s = trav_ms / 1e3 / max(trav_launches, 1)
+    flop = FLOP_PER_INTERACTION["gravity"] * (pn + pp)
+    achieved = flop / trav_avg_s / 1e12
+    nodes, max_level, overfull = ctx.tree_info()
+
+    # Barnes-Hut error of the final state against the FP64 direct sum on sampled sinks
+    sample = sample_indices(n, args.error_sample).astype(np.uint32)
+    ax, ay, _ = ctx.forces(theta, want_total=False)
+    dx, dy, _ = ctx.direct_fp64(sample)
+    num = np.hypot(ax[sample].astype(np.float64) - dx, ay[sample].astype(np.float64) - dy)
+    rel = num / np.hypot(dx, dy)
+
+    result = {
+        "metric": "BH force evals/sec at N=1M, theta=0.5",
+        "value": n * args.steps / wall,
+        "unit": "force evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "two Plummer disks (a=0.1, 2 x 524,288) at (+-0.5,0), "
+                               "counter-rotating; G=1, eps=1e-3; QuadTree leaf capacity 16; "
+                               "leapfrog KDK dt=1e-4, full rebuild every step",
+                   "n": n, "theta": theta, "tree": "quadtree", "code_bits": args.code_bits,
+                   "parallelism": "single GPU"},
+        "interactions_per_s": (pn + pp) * args.steps / wall,
+        "particle_steps_per_s": n * args.steps / wall,
+        "interactions_per_particle": (pn + pp) / n,
+        "tree": {"nodes": nodes, "max_level": max_level, "overfull_leaves": overfull},
+        "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "roofline": {"bound": "mfma", "pipe": "FP32 VALU (traversal)", "kernel": "k_traverse",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_launch": flop, "avg_launch_ms": trav_avg_s * 1e3},
+        "bh_error_vs_fp64_direct": {"sample": int(sample.size),
+                                    "median_rel": float(np.median(rel)),
+                                    "p99_rel": float(np.quantile(rel, 0.99))},
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traverse_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            result["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (reference BH, one core)")
+        result["cpu_baseline"] = cpu_baseline(p, theta, eps, n)
+        result["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -107,6 +107,13 @@ bh_status bh_forces_device(bh_ctx* ctx, double theta, float* d_ax, float* d_ay, 
  * Positions and velocities stay on the device; read them with bh_get_particles. */
 bh_status bh_leapfrog(bh_ctx* ctx, int steps, float dt, double theta);
 
+/* FP64 direct sum for sampled sinks (NBodySolverNaive::solveNbody with the same kernel,
+ * nBodySolverNaive.h:53-140), over all current particles with the sink itself excluded by
+ * index. sinks: caller-order indices (host). Outputs (host, FP64, nsinks each): the exact
+ * interaction sums and per-sink totals. Used to report the Barnes-Hut error of a result. */
+bh_status bh_direct_fp64(bh_ctx* ctx, const uint32_t* sinks, uint32_t nsinks, double* ax,
+                         double* ay, double* total);
+
 /* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
 /* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */
 bh_status bh_get_bounds(bh_ctx* ctx, double* bounds, double* root_side);

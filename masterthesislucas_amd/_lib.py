@@ -51,6 +51,7 @@ SIGNATURES = {
     "bh_forces": (_i32, [_vp, _d, _vp, _vp, _vp]),
     "bh_forces_device": (_i32, [_vp, _d, _vp, _vp, _vp]),
     "bh_leapfrog": (_i32, [_vp, C.c_int, _f, _d]),
+    "bh_direct_fp64": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),
     "bh_get_bounds": (_i32, [_vp, _vp, _vp]),
     "bh_get_morton": (_i32, [_vp, _vp, _vp]),
     "bh_tree_info": (_i32, [_vp, _vp, _vp, _vp]),

@@ -76,6 +76,10 @@ struct bh_ctx {
     unsigned long long* counters = nullptr;
     uint32_t *slot_a = nullptr, *slot_b = nullptr, *slot_c = nullptr;
     uint64_t slot_cap = 0;
+    uint32_t* where = nullptr;           // inverse of the sorted-order ids
+    uint32_t* dsinks = nullptr;          // direct sum: sink indices (caller, sorted)
+    double* dpart = nullptr;             // direct sum: per-split partial sums + output
+    uint64_t dsink_cap = 0, dpart_cap = 0;
 
     BuildParams* h_prm = nullptr;  // pinned
     double* h_dsum = nullptr;
@@ -258,6 +262,8 @@ void bh_destroy(bh_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->dsinks) (void)hipFree(c->dsinks);
+    if (c->dpart) (void)hipFree(c->dpart);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -368,6 +374,7 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     TRY(dalloc(c, &c->red_counter, 1));
     TRY(dalloc(c, &c->dsum, 1));
     TRY(dalloc(c, &c->counters, 2));
+    TRY(dalloc(c, &c->where, N));
     if (!c->wide) {
         TRY(dalloc(c, &c->slot_a, c->slot_cap));
         TRY(dalloc(c, &c->slot_b, c->slot_cap));
@@ -524,6 +531,52 @@ bh_status bh_leapfrog(bh_ctx* c, int steps, float dt, double theta) {
     st = launch_status(c, "leapfrog launch");
     if (st != BH_OK) return st;
     return hip_check(c, hipStreamSynchronize(c->stream), "leapfrog");
+}
+
+bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* ax, double* ay,
+                         double* total) {
+    if (!c || (!sinks && ns)) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no particles in sorted state (build first)");
+    if (ns == 0) return BH_OK;
+    for (uint32_t k = 0; k < ns; ++k)
+        if (sinks[k] >= c->n) return fail(c, BH_ERR_ARG, "sink index %u out of range", sinks[k]);
+    const uint32_t S = direct_splits(c->n, ns);
+    const uint64_t need_s = 2ull * ns, need_p = (uint64_t)(3ull * S + 3ull) * ns;
+    if (need_s > c->dsink_cap) {
+        if (c->dsinks) (void)hipFree(c->dsinks);
+        c->dsinks = nullptr;
+        c->dsink_cap = 0;
+        bh_status st = hip_check(c, hipMalloc((void**)&c->dsinks, need_s * 4), "hipMalloc sinks");
+        if (st != BH_OK) return st;
+        c->dsink_cap = need_s;
+    }
+    if (need_p > c->dpart_cap) {
+        if (c->dpart) (void)hipFree(c->dpart);
+        c->dpart = nullptr;
+        c->dpart_cap = 0;
+        bh_status st = hip_check(c, hipMalloc((void**)&c->dpart, need_p * 8), "hipMalloc partials");
+        if (st != BH_OK) return st;
+        c->dpart_cap = need_p;
+    }
+    hipStream_t s = c->stream;
+    bh_status st = hip_check(c, hipMemcpyAsync(c->dsinks, sinks, (size_t)ns * 4, hipMemcpyHostToDevice, s),
+                             "upload sinks");
+    if (st != BH_OK) return st;
+    double* out = c->dpart + (uint64_t)3 * S * ns;
+    const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
+    const double eps = c->cfg.softening;
+    launch_direct_f64(s, grav ? kKernelGravity : kKernelTsne, c->pos4, c->id[c->cur], c->n,
+                      c->dsinks, ns, grav ? (double)c->cfg.G : 1.0, eps * eps, c->where,
+                      c->dsinks + ns, c->dpart, S, out);
+    st = launch_status(c, "direct launch");
+    if (st != BH_OK) return st;
+    const size_t b = (size_t)ns * 8;
+    if (ax) st = hip_check(c, hipMemcpyAsync(ax, out, b, hipMemcpyDeviceToHost, s), "ax");
+    if (st == BH_OK && ay) st = hip_check(c, hipMemcpyAsync(ay, out + ns, b, hipMemcpyDeviceToHost, s), "ay");
+    if (st == BH_OK && total)
+        st = hip_check(c, hipMemcpyAsync(total, out + 2 * (uint64_t)ns, b, hipMemcpyDeviceToHost, s), "tot");
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(s), "direct");
 }
 
 bh_status bh_get_bounds(bh_ctx* c, double* bounds, double* root_side) {

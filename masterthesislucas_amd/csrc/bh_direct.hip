@@ -1,0 +1,117 @@
+// FP64 direct sum on sampled sinks: the exact-sum oracle of the reference (NBodySolverNaive,
+// nBodySolverNaive.h:53-140) run on the GPU, used to report the Barnes-Hut error of every
+// result and to label training pairs. A block takes 64 sinks (one per lane); its four waves
+// split a chunk of sources, and the source chunks of a sink are summed by a second kernel in
+// a fixed order, so the result does not depend on scheduling.
+#include "bh_common.h"
+#include "bh_launch.h"
+
+namespace bh {
+
+template <int KERNEL>
+__global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict__ pos4, uint32_t n,
+                                                       const uint32_t* __restrict__ sinks,
+                                                       uint32_t ns, double G, double eps2,
+                                                       double* __restrict__ part, uint32_t S) {
+    __shared__ double red[3][kWavesPerBlock][kWave];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t s = blockIdx.x * kWave + lane;
+    const bool valid = s < ns;
+    const uint32_t si = valid ? sinks[s] : 0u;
+    const float4 me = pos4[si];
+    const double px = me.x, py = me.y;
+    const uint32_t split = blockIdx.y;
+    const uint64_t lo = (uint64_t)n * split / S, hi = (uint64_t)n * (split + 1) / S;
+    double sx = 0.0, sy = 0.0, st = 0.0;
+    for (uint64_t j = lo + (uint64_t)w; j < hi; j += kWavesPerBlock) {
+        const float4 q = pos4[j];
+        if (j == si) continue;
+        const double m = q.z;
+        if (KERNEL == kKernelGravity) {
+            const double dx = (double)q.x - px, dy = (double)q.y - py;
+            const double r2 = dx * dx + dy * dy + eps2;
+            const double inv = 1.0 / sqrt(r2);
+            const double f = G * m * inv * inv * inv;
+            sx += f * dx;
+            sy += f * dy;
+            st += -G * m * inv;
+        } else {
+            const double dx = px - (double)q.x, dy = py - (double)q.y;
+            const double fd = 1.0 / (1.0 + (dx * dx + dy * dy));
+            st += m * fd;
+            const double f = m * fd * fd;
+            sx += f * dx;
+            sy += f * dy;
+        }
+    }
+    red[0][w][lane] = sx;
+    red[1][w][lane] = sy;
+    red[2][w][lane] = st;
+    __syncthreads();
+    if (w == 0 && valid) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int k = 0; k < kWavesPerBlock; ++k) {
+            a += red[0][k][lane];
+            b += red[1][k][lane];
+            c += red[2][k][lane];
+        }
+        part[((uint64_t)split * 3 + 0) * ns + s] = a;
+        part[((uint64_t)split * 3 + 1) * ns + s] = b;
+        part[((uint64_t)split * 3 + 2) * ns + s] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_direct_sum(const double* __restrict__ part, uint32_t S,
+                                                       uint32_t ns, double* __restrict__ out) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= ns) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t k = 0; k < S; ++k) {
+        a += part[((uint64_t)k * 3 + 0) * ns + s];
+        b += part[((uint64_t)k * 3 + 1) * ns + s];
+        c += part[((uint64_t)k * 3 + 2) * ns + s];
+    }
+    out[s] = a;
+    out[ns + s] = b;
+    out[2 * (uint64_t)ns + s] = c;
+}
+
+// inverse of the sorted-order id array: where[id[i]] = i
+__global__ __launch_bounds__(kBlock) void k_invert(const uint32_t* __restrict__ id, uint32_t n,
+                                                   uint32_t* __restrict__ where) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) where[id[i]] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_take(const uint32_t* __restrict__ where,
+                                                 const uint32_t* __restrict__ sel, uint32_t ns,
+                                                 uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < ns) out[i] = where[sel[i]];
+}
+
+static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+uint32_t direct_splits(uint32_t n, uint32_t ns) {
+    const uint32_t sink_blocks = div_up(ns, kWave);
+    uint32_t S = div_up(4096, sink_blocks);            // aim for ~4096 blocks
+    S = S < 1 ? 1 : (S > 256 ? 256 : S);
+    const uint32_t maxS = div_up(n, 256);              // at least 256 sources per split
+    return S > maxS ? (maxS ? maxS : 1) : S;
+}
+
+void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
+                       uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
+                       uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
+                       double* out) {
+    k_invert<<<div_up(n, kBlock), kBlock, 0, s>>>(id, n, where);
+    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, sinks_caller, ns, sinks_sorted);
+    dim3 g(div_up(ns, kWave), S);
+    if (kernel == kKernelGravity)
+        k_direct_f64<kKernelGravity><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S);
+    else
+        k_direct_f64<kKernelTsne><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S);
+    k_direct_sum<<<div_up(ns, kBlock), kBlock, 0, s>>>(part, S, ns, out);
+}
+
+}  // namespace bh

@@ -55,4 +55,10 @@ void launch_kick_drift(hipStream_t s, float* x, float* y, float* vx, float* vy, 
 void launch_kick(hipStream_t s, float* vx, float* vy, const float* ax, const float* ay, uint32_t n,
                  float hdt);
 
+uint32_t direct_splits(uint32_t n, uint32_t ns);
+void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
+                       uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
+                       uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
+                       double* out);
+
 }  // namespace bh

@@ -121,6 +121,14 @@ class BHContext:
     def leapfrog(self, steps, dt, theta):
         self._c(_lib.load().bh_leapfrog(self._h, int(steps), float(dt), float(theta)))
 
+    def direct_fp64(self, sinks):
+        """FP64 direct sum on the GPU for caller-order sink indices -> (ax, ay, total)."""
+        sk = np.ascontiguousarray(sinks, dtype=np.uint32)
+        ax, ay, tot = (np.empty(sk.size, np.float64) for _ in range(3))
+        self._c(_lib.load().bh_direct_fp64(self._h, _ptr(sk), sk.size, _ptr(ax), _ptr(ay),
+                                           _ptr(tot)))
+        return ax, ay, tot
+
     def bounds(self):
         b = np.empty(4, np.float64)
         side = C.c_double(0)
