@@ -70,6 +70,10 @@ def main():
     ap.add_argument("--code-bits", type=int, default=32)
     ap.add_argument("--error-sample", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (PMC traffic calibration)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="traversal variant; -1 = autotune on the workload before timing")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -84,12 +88,31 @@ def main():
     from masterthesislucas_amd.workloads import sample_indices, two_plummer_disks
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.copy_mb} MiB done")
+        del src, dst
     n, theta, eps, dt = args.n, args.theta, 1e-3, args.dt
     log(f"[bench] generating two Plummer disks n={n}")
     p = two_plummer_disks(n)
     ctx = BHContext(n, kernel="gravity", tree_mode="quadtree", max_children=16,
                     code_bits=args.code_bits, G=1.0, softening=eps)
     ctx.set_particles(p.x, p.y, p.m, p.vx, p.vy)
+    tuning = None
+    if args.variant >= 0:
+        ctx.set_variant(args.variant)
+        variant = args.variant
+    else:
+        from masterthesislucas_amd.autotune import tune
+
+        ctx.build()
+        variant, times = tune(ctx, theta)
+        tuning = {str(k): round(v, 4) for k, v in times.items()}
+        log(f"[bench] autotune: variant {variant} ({tuning} ms)")
+        ctx.set_particles(p.x, p.y, p.m, p.vx, p.vy)
     log(f"[bench] warmup {args.warmup} steps")
     ctx.leapfrog(args.warmup, dt, theta)
     ctx.profile_enable(True)
@@ -143,6 +166,8 @@ def main():
         "particle_steps_per_s": n * args.steps / wall,
         "interactions_per_particle": (pn + pp) / n,
         "tree": {"nodes": nodes, "max_level": max_level, "overfull_leaves": overfull},
+        "traverse_variant": variant,
+        "autotune_ms": tuning,
         "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
         "roofline": {"bound": "mfma", "pipe": "FP32 VALU (traversal)", "kernel": "k_traverse",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

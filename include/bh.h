@@ -144,6 +144,16 @@ bh_status bh_profile_enable(bh_ctx* ctx, int enable);
 bh_status bh_profile_read(bh_ctx* ctx, double* ms, uint64_t* launches, int reset);
 bh_status bh_synchronize(bh_ctx* ctx);
 
+/* ---- autotuning: kernel variants of the traversal (no reference counterpart; the reference
+ * GPU traversal has one fixed shape, nBodySolverGpuBH.cu:260-348) ----
+ * variant v in [0, bh_traverse_variant_count()): v / 4 selects 1, 2 or 4 sink particles per
+ * lane (a wave serves 64, 128 or 256 Morton-consecutive particles); (v % 4) / 2 selects
+ * monopoles computed only when a lane accepts (0) or always, masked (1); v % 2 selects 1 or 4
+ * sources per scalar fetch in the particle-particle loop. Results are identical up to FP32
+ * summation order. */
+int bh_traverse_variant_count(void);
+bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
+
 #ifdef __cplusplus
 }
 #endif

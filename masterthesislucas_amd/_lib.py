@@ -61,6 +61,8 @@ SIGNATURES = {
     "bh_profile_enable": (_i32, [_vp, C.c_int]),
     "bh_profile_read": (_i32, [_vp, _vp, _vp, C.c_int]),
     "bh_synchronize": (_i32, [_vp]),
+    "bh_traverse_variant_count": (_i32, []),
+    "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
 }
 
 _lib = None

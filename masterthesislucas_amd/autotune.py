@@ -1,0 +1,62 @@
+"""Autotuner for the Barnes-Hut traversal kernel.
+
+The traversal has a few compile-time shapes (see bh_set_traverse_variant in include/bh.h):
+whether monopoles are computed only when a lane accepts or always (masked), and how many
+source particles each scalar fetch brings into the particle-particle loop. Which one is fastest
+depends on the particle distribution, N and theta, so the tuner times every variant on the
+caller's own context (already holding the workload) with the library's HIP-event phase timers
+and keeps the fastest. Choices are remembered per (kernel, tree mode, log2 N, theta) in a JSON
+table next to this file so later runs can skip the search.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuned_variants.json")
+
+
+def _key(ctx, theta):
+    return f"{ctx.kernel}|{ctx.tree_mode}|n{int(math.log2(max(ctx.n, 1)))}|t{theta:.3f}"
+
+
+def load_table(path=TABLE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(ctx, theta, path=TABLE):
+    return load_table(path).get(_key(ctx, theta))
+
+
+def time_variant(ctx, variant, theta, repeats=3):
+    """Average traversal milliseconds of `variant` on the current tree."""
+    ctx.set_variant(variant)
+    ctx.forces_device_only(theta)  # warm the code object and caches
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    for _ in range(repeats):
+        ctx.forces_device_only(theta)
+    ms, launches = ctx.profile_read(reset=True)["traversal"]
+    ctx.profile_enable(False)
+    return ms / max(launches, 1)
+
+
+def tune(ctx, theta, repeats=3, save=False, path=TABLE):
+    """Time all variants on ctx's current tree (ctx must have been built), select the fastest,
+    and return (best_variant, {variant: ms})."""
+    times = {}
+    for v in range(ctx.variant_count()):
+        times[v] = time_variant(ctx, v, theta, repeats)
+    best = min(times, key=times.get)
+    ctx.set_variant(best)
+    if save:
+        table = load_table(path)
+        table[_key(ctx, theta)] = {"variant": best, "ms": times}
+        with open(path, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    return best, times

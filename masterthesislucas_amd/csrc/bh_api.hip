@@ -65,13 +65,13 @@ struct bh_ctx {
     uint8_t *leaf_lvl = nullptr, *lcp1 = nullptr;
     uint32_t *cnt = nullptr, *noff = nullptr;
     Node* nodes = nullptr;
-    uint32_t *parent = nullptr, *arrived = nullptr;
+    uint32_t *parent = nullptr, *child4 = nullptr, *deferred = nullptr;
     float4* m2 = nullptr;
     BuildParams* prm = nullptr;
     float4* bounds_part = nullptr;
-    uint32_t* bounds_counter = nullptr;
     double* red_part = nullptr;
-    uint32_t* red_counter = nullptr;
+    uint2* stats_part = nullptr;
+    uint32_t max_blocks = 0;  // traversal blocks for n_max (per-block counter partials)
     double* dsum = nullptr;
     unsigned long long* counters = nullptr;
     uint32_t *slot_a = nullptr, *slot_b = nullptr, *slot_c = nullptr;
@@ -83,8 +83,11 @@ struct bh_ctx {
 
     BuildParams* h_prm = nullptr;  // pinned
     double* h_dsum = nullptr;
-    unsigned long long* h_counters = nullptr;
+    unsigned long long* h_counters = nullptr;  // 2 per traversal block
+    uint2* h_stats = nullptr;
 
+    int variant = 2;  // traversal kernel variant (see launch_traverse)
+    uint32_t trav_blocks = 0;  // blocks of the last traversal launch
     bool profile = false;
     PhaseTimer phases[BH_NUM_PHASES];
 
@@ -180,7 +183,7 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     }
 
     phase_begin(c, 0);
-    launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->bounds_counter, c->prm,
+    launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
                   quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
     launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a, c->wide);
     phase_end(c, 0);
@@ -205,11 +208,11 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 3);
     launch_tree(s, codes, c->wide, n, c->B, c->cfg.max_children, fixed, quadtree ? 1 : 0,
                 c->leaf_lvl, c->lcp1, c->cnt, c->noff, c->scan_part, c->prm, c->nodes, c->parent,
-                c->arrived, c->m2, c->cap_nodes);
+                c->child4, c->cap_nodes);
     phase_end(c, 3);
 
     phase_begin(c, 4);
-    launch_upward(s, c->pos4, c->nodes, c->parent, c->arrived, c->m2, c->prm, c->cap_nodes);
+    launch_upward(s, c->pos4, c->nodes, c->child4, c->m2, c->prm, c->deferred, c->cap_nodes);
     phase_end(c, 4);
     bh_status st = launch_status(c, "build launch");
     if (st != BH_OK) return st;
@@ -222,17 +225,18 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     if (!c->built) return fail(c, BH_ERR_STATE, "forces requested before a tree was built");
     if (!(theta >= 0.0)) return fail(c, BH_ERR_ARG, "theta must be >= 0");
     hipStream_t s = c->stream;
-    launch_fill_u32(s, (uint32_t*)c->counters, 4, 0u);
     const float inv_theta2 = (float)(1.0 / (theta * theta));
     const float eps = c->cfg.softening;
     const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
     phase_begin(c, 5);
-    launch_traverse(s, grav ? kKernelGravity : kKernelTsne, c->pos4, c->nodes, c->prm, c->n,
+    c->trav_blocks = launch_traverse(s, grav ? kKernelGravity : kKernelTsne, c->variant, c->pos4,
+                                     c->nodes, c->prm,
+                    c->n,
                     inv_theta2, eps * eps, grav ? c->cfg.G : 1.0f, c->ax, c->ay,
                     want_total ? c->tot : nullptr, c->counters);
     phase_end(c, 5);
     if (want_total)
-        launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->red_counter, c->dsum);
+        launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
     c->have_acc = true;
     return launch_status(c, "traverse launch");
 }
@@ -267,6 +271,7 @@ void bh_destroy(bh_ctx* c) {
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
     for (auto& t : c->phases) {
         for (auto& e : t.pending) {
             (void)hipEventDestroy(e.first);
@@ -337,7 +342,7 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
         TRY(dalloc(c, &c->vy[k], N));
         TRY(dalloc(c, &c->id[k], N));
     }
-    TRY(dalloc(c, &c->pos4, N));
+    TRY(dalloc(c, &c->pos4, (size_t)N + kPosPad));
     TRY(dalloc(c, &c->ax, N));
     TRY(dalloc(c, &c->ay, N));
     TRY(dalloc(c, &c->tot, N));
@@ -366,14 +371,15 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     TRY(dalloc(c, &c->noff, N));
     TRY(dalloc(c, &c->nodes, c->cap_nodes));
     TRY(dalloc(c, &c->parent, c->cap_nodes));
-    TRY(dalloc(c, &c->arrived, c->cap_nodes));
+    TRY(dalloc(c, &c->deferred, (size_t)up_chunks(c->cap_nodes) * kDeferStride));
+    TRY(dalloc(c, &c->stats_part, kStatsBlocks));
+    TRY(dalloc(c, &c->child4, 4ull * c->cap_nodes));
     TRY(dalloc(c, &c->prm, 1));
     TRY(dalloc(c, &c->bounds_part, kMaxBoundsBlocks));
-    TRY(dalloc(c, &c->bounds_counter, 1));
     TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
-    TRY(dalloc(c, &c->red_counter, 1));
     TRY(dalloc(c, &c->dsum, 1));
-    TRY(dalloc(c, &c->counters, 2));
+    c->max_blocks = traverse_blocks(N);
+    TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
     if (!c->wide) {
         TRY(dalloc(c, &c->slot_a, c->slot_cap));
@@ -382,10 +388,12 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     }
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_prm, sizeof(BuildParams), 0), "hipHostMalloc"));
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_dsum, sizeof(double), 0), "hipHostMalloc"));
-    TRY(hip_check(c, hipHostMalloc((void**)&c->h_counters, 2 * sizeof(unsigned long long), 0),
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_stats, kStatsBlocks * sizeof(uint2), 0),
                   "hipHostMalloc"));
-    launch_fill_u32(c->stream, c->bounds_counter, 1, 0u);
-    launch_fill_u32(c->stream, c->red_counter, 1, 0u);
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_counters,
+                                   2ull * c->max_blocks * sizeof(unsigned long long), 0),
+                  "hipHostMalloc"));
+    launch_fill_f32(c->stream, (float*)c->pos4, 4ull * ((uint64_t)N + kPosPad), 0.0f);
     launch_fill_u32(c->stream, (uint32_t*)c->prm, sizeof(BuildParams) / 4, 0u);
     TRY(hip_check(c, hipStreamSynchronize(c->stream), "init"));
 #undef TRY
@@ -627,12 +635,26 @@ bh_status bh_get_morton(bh_ctx* c, uint64_t* codes_sorted, uint32_t* perm) {
 bh_status bh_tree_info(bh_ctx* c, uint32_t* n_nodes, uint32_t* max_level, uint32_t* overfull) {
     if (!c) return BH_ERR_ARG;
     if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
-    bh_status st = hip_check(c, hipMemcpy(c->h_prm, c->prm, sizeof(BuildParams), hipMemcpyDeviceToHost),
-                             "params");
+    const int limit = c->wide ? c->B : c->depth;
+    launch_tree_stats(c->stream, c->nodes, c->prm, c->cap_nodes, limit,
+                      (uint32_t)c->cfg.max_children, c->stats_part, kStatsBlocks);
+    bh_status st = launch_status(c, "tree stats");
+    if (st == BH_OK)
+        st = hip_check(c, hipMemcpyAsync(c->h_stats, c->stats_part, kStatsBlocks * sizeof(uint2),
+                                         hipMemcpyDeviceToHost, c->stream), "stats");
+    if (st == BH_OK)
+        st = hip_check(c, hipMemcpyAsync(c->h_prm, c->prm, sizeof(BuildParams), hipMemcpyDeviceToHost,
+                                         c->stream), "params");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "tree info");
     if (st != BH_OK) return st;
+    uint32_t mx = 0, of = 0;
+    for (uint32_t b = 0; b < kStatsBlocks; ++b) {
+        mx = std::max(mx, c->h_stats[b].x);
+        of += c->h_stats[b].y;
+    }
     if (n_nodes) *n_nodes = c->h_prm->n_nodes;
-    if (max_level) *max_level = c->h_prm->max_level;
-    if (overfull) *overfull = c->h_prm->overfull_leaves;
+    if (max_level) *max_level = mx;
+    if (overfull) *overfull = of;
     return BH_OK;
 }
 
@@ -686,11 +708,20 @@ bh_status bh_get_tree_slots(bh_ctx* c, int32_t* depth, uint64_t* n_slots, uint32
 
 bh_status bh_counters(bh_ctx* c, uint64_t* pn, uint64_t* pp) {
     if (!c) return BH_ERR_ARG;
-    bh_status st = hip_check(c, hipMemcpy(c->h_counters, c->counters, 2 * sizeof(unsigned long long),
-                                          hipMemcpyDeviceToHost), "counters");
+    if (!c->have_acc) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    const uint32_t nb = c->trav_blocks;
+    bh_status st = hip_check(c, hipMemcpyAsync(c->h_counters, c->counters,
+                                               2ull * nb * sizeof(unsigned long long),
+                                               hipMemcpyDeviceToHost, c->stream), "counters");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "counters");
     if (st != BH_OK) return st;
-    if (pn) *pn = c->h_counters[0];
-    if (pp) *pp = c->h_counters[1];
+    uint64_t a = 0, b = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+        a += c->h_counters[2 * k];
+        b += c->h_counters[2 * k + 1];
+    }
+    if (pn) *pn = a;
+    if (pp) *pp = b;
     return BH_OK;
 }
 
@@ -715,6 +746,16 @@ bh_status bh_profile_read(bh_ctx* c, double* ms, uint64_t* launches, int reset) 
     }
     return BH_OK;
 }
+
+bh_status bh_set_traverse_variant(bh_ctx* c, int variant) {
+    if (!c) return BH_ERR_ARG;
+    if (variant < 0 || variant >= kTraverseVariants)
+        return fail(c, BH_ERR_ARG, "traversal variant %d outside [0, %d)", variant, kTraverseVariants);
+    c->variant = variant;
+    return BH_OK;
+}
+
+int bh_traverse_variant_count(void) { return kTraverseVariants; }
 
 bh_status bh_synchronize(bh_ctx* c) {
     if (!c) return BH_ERR_ARG;

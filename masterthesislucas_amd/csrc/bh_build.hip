@@ -6,69 +6,62 @@
 // contiguous particle range, its first child (if any) is the next node, and `skip` points past
 // its subtree. Nodes starting at sorted index i are the cells of levels
 // lcp(i-1, i)+1 .. leaf_level(i) whose first particle is i, so the layout follows from one scan.
+//
+// No kernel here uses device-scope fences or same-address device atomics: on a multi-XCD
+// part both are expensive (a device-scope fence writes back or invalidates the XCD's L2).
+// Results pass between kernels, and within a kernel only through LDS.
 #include "bh_common.h"
 #include "bh_launch.h"
 
 namespace bh {
 
 // ------------------------------------------------------------------------------------------
-// bounds: FP32 min/max, the last block derives the root square and the quantiser
+// bounds: FP32 min/max per block, then one block derives the root square and the quantiser
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ xs,
-                                                   const float* __restrict__ ys, uint32_t n,
-                                                   float4* __restrict__ partials,
-                                                   uint32_t* __restrict__ counter,
-                                                   BuildParams* __restrict__ prm, uint32_t tree_mode,
-                                                   int use_caller, double cminx, double cminy,
-                                                   double cmaxx, double cmaxy) {
+__device__ __forceinline__ float4 minmax4(float4 a, float4 b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+
+__device__ __forceinline__ float4 block_minmax4(float4 r) {
     __shared__ float4 red[kWavesPerBlock];
-    __shared__ int last;
-    float mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float x = xs[i], y = ys[i];
-        mnx = fminf(mnx, x);
-        mny = fminf(mny, y);
-        mxx = fmaxf(mxx, x);
-        mxy = fmaxf(mxy, y);
-    }
     for (int o = 32; o > 0; o >>= 1) {
-        mnx = fminf(mnx, __shfl_xor(mnx, o, kWave));
-        mny = fminf(mny, __shfl_xor(mny, o, kWave));
-        mxx = fmaxf(mxx, __shfl_xor(mxx, o, kWave));
-        mxy = fmaxf(mxy, __shfl_xor(mxy, o, kWave));
+        r.x = fminf(r.x, __shfl_xor(r.x, o, kWave));
+        r.y = fminf(r.y, __shfl_xor(r.y, o, kWave));
+        r.z = fmaxf(r.z, __shfl_xor(r.z, o, kWave));
+        r.w = fmaxf(r.w, __shfl_xor(r.w, o, kWave));
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = make_float4(mnx, mny, mxx, mxy);
+    if (lane == 0) red[w] = r;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float4 r = red[0];
-        for (int k = 1; k < kWavesPerBlock; ++k) {
-            r.x = fminf(r.x, red[k].x);
-            r.y = fminf(r.y, red[k].y);
-            r.z = fmaxf(r.z, red[k].z);
-            r.w = fmaxf(r.w, red[k].w);
-        }
-        partials[blockIdx.x] = r;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
+    r = red[0];
+    for (int k = 1; k < kWavesPerBlock; ++k) r = minmax4(r, red[k]);
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* __restrict__ xs,
+                                                           const float* __restrict__ ys,
+                                                           uint32_t n,
+                                                           float4* __restrict__ partials) {
     float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
-    for (uint32_t b = 0; b < gridDim.x; ++b) {
-        float4 p;
-        p.x = __hip_atomic_load(&partials[b].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        p.y = __hip_atomic_load(&partials[b].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        p.z = __hip_atomic_load(&partials[b].z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        p.w = __hip_atomic_load(&partials[b].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        r.x = fminf(r.x, p.x);
-        r.y = fminf(r.y, p.y);
-        r.z = fmaxf(r.z, p.z);
-        r.w = fmaxf(r.w, p.w);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float x = xs[i], y = ys[i];
+        r = minmax4(r, make_float4(x, y, x, y));
     }
-    *counter = 0u;  // ready for the next build (no memset node)
+    r = block_minmax4(r);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const float4* __restrict__ partials,
+                                                         uint32_t nb, uint32_t n,
+                                                         BuildParams* __restrict__ prm,
+                                                         uint32_t tree_mode, int use_caller,
+                                                         double cminx, double cminy,
+                                                         double cmaxx, double cmaxy) {
+    float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlock) r = minmax4(r, partials[b]);
+    r = block_minmax4(r);
+    if (threadIdx.x != 0) return;
     uint64_t mnx64, mny64, mxx64, mxy64;
     if (use_caller) {
         mnx64 = bh_sfp_f64_bits(cminx);
@@ -343,8 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
                                                         int B, int cap, int fixed_depth,
                                                         uint8_t* __restrict__ leaf_lvl,
                                                         uint8_t* __restrict__ lcp1,
-                                                        uint32_t* __restrict__ cnt,
-                                                        BuildParams* __restrict__ prm) {
+                                                        uint32_t* __restrict__ cnt) {
     __shared__ K s_code[kBlock + 2 * kMaxLeafCap];
     __shared__ int8_t s_w[kBlock + kMaxLeafCap];
     const int64_t b0 = (int64_t)blockIdx.x * kBlock;
@@ -356,6 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
             s_code[t] = (j >= 0 && j < (int64_t)n) ? codes[j] : (K)0;
         }
         __syncthreads();
+        // w(j) = common level of particles j and j+cap: the window [j, j+cap] shares a cell of
+        // that level, so a cell holding i splits while some window through i shares it
         for (int t = threadIdx.x; t < kBlock + cap; t += kBlock) {
             const int64_t j = b0 - cap + t;
             int8_t wv = -1;
@@ -367,27 +361,17 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
     const int64_t i = b0 + threadIdx.x;
     if (i >= (int64_t)n) return;
     int leaf;
-    bool overfull = false;
     if (adaptive) {
         int m = -1;
         for (int t = 0; t <= cap; ++t) m = max(m, (int)s_w[threadIdx.x + t]);
-        leaf = m + 1;
-        if (leaf > B) {  // more than `cap` particles share the finest cell
-            leaf = B;
-            overfull = true;
-        }
+        leaf = min(m + 1, B);  // more than `cap` particles in the finest cell: stop at depth B
     } else {
         leaf = fixed_depth;
     }
     const int lp = (i == 0) ? -1 : lcp_level(codes[i - 1], codes[i], B);
-    const uint32_t c = (leaf > lp) ? (uint32_t)(leaf - lp) : 0u;
     leaf_lvl[i] = (uint8_t)leaf;
     lcp1[i] = (uint8_t)(lp + 1);
-    cnt[i] = c;
-    if (c) {
-        atomicMax(&prm->max_level, (uint32_t)leaf);
-        if (overfull) atomicAdd(&prm->overfull_leaves, 1u);
-    }
+    cnt[i] = (leaf > lp) ? (uint32_t)(leaf - lp) : 0u;
 }
 
 // first j >= lo with codes[j] > hi_key (codes ascending), galloping then binary search
@@ -431,60 +415,70 @@ __device__ __forceinline__ uint32_t lower_back(const K* __restrict__ codes, K lo
     return lo;
 }
 
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_tree_emit(const K* __restrict__ codes, uint32_t n, int B,
-                                                      const uint8_t* __restrict__ leaf_lvl,
-                                                      const uint8_t* __restrict__ lcp1,
-                                                      const uint32_t* __restrict__ cnt,
-                                                      const uint32_t* __restrict__ noff,
-                                                      const BuildParams* __restrict__ prm,
-                                                      int single_child_pp, Node* __restrict__ nodes,
-                                                      uint32_t* __restrict__ parent,
-                                                      uint32_t* __restrict__ arrived,
-                                                      uint32_t cap_nodes) {
+// nodes of particle i: k = noff[i] + t at level lcp1[i] + t, t < cnt[i]
+__global__ __launch_bounds__(kBlock) void k_tree_emit_heads(uint32_t n,
+                                                            const uint8_t* __restrict__ lcp1,
+                                                            const uint32_t* __restrict__ cnt,
+                                                            const uint32_t* __restrict__ noff,
+                                                            Node* __restrict__ nodes,
+                                                            uint4* __restrict__ child4,
+                                                            uint32_t cap_nodes) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = cnt[i];
-    if (!c) return;
-    const uint32_t total = prm->n_nodes;
-    const float side = prm->side;
-    const int lp = (int)lcp1[i] - 1;
-    const int leaf = leaf_lvl[i];
     const uint32_t base = noff[i];
+    const uint32_t l0 = lcp1[i];
+    for (uint32_t t = 0; t < c; ++t) {
+        const uint32_t k = base + t;
+        if (k >= cap_nodes) break;  // cannot happen for the capacity the context allocates
+        nodes[k].first = i;
+        nodes[k].meta = (l0 + t) | (t == 0 ? kMetaHead : 0u) | (t + 1 == c ? kMetaLeaf : 0u);
+        child4[k] = make_uint4(kNone, kNone, kNone, kNone);
+    }
+}
+
+// one thread per node: subtree extent, flags, side, parent and the parent's child slot
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_tree_emit_nodes(const K* __restrict__ codes,
+                                                            uint32_t n, int B,
+                                                            const uint8_t* __restrict__ lcp1,
+                                                            const uint32_t* __restrict__ noff,
+                                                            const BuildParams* __restrict__ prm,
+                                                            int single_child_pp,
+                                                            Node* __restrict__ nodes,
+                                                            uint32_t* __restrict__ parent,
+                                                            uint32_t* __restrict__ child4,
+                                                            uint32_t cap_nodes) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t total = prm->n_nodes;
+    if (k >= total || k >= cap_nodes) return;
+    const uint32_t i = nodes[k].first;
+    uint32_t meta = nodes[k].meta;
+    const int l = (int)(meta & kMetaLevelMask);
     const K ci = codes[i];
-    uint32_t e = i + 1;
-    for (int t = (int)c - 1; t >= 0; --t) {
-        const int l = lp + 1 + t;
+    uint32_t e;
+    if (l == 0) {
+        e = n;
+    } else {
         const int sh = 2 * (B - l);
-        if (l == 0) {
-            e = n;
-        } else {
-            const K hk = ci | (K)(((K)1 << sh) - 1);
-            e = upper_from(codes, n, hk, e);
-        }
-        const uint32_t k = base + (uint32_t)t;
-        if (k >= cap_nodes) continue;  // cannot happen for the capacity the context allocates
-        uint32_t meta = (uint32_t)l;
-        if (l == leaf) {
-            meta |= kMetaLeaf | kMetaPP;
-        } else if (single_child_pp) {
-            const int shc = 2 * (B - l - 1);
-            const uint32_t d0 = (uint32_t)(ci >> shc) & 3u;
-            const uint32_t d1 = (uint32_t)(codes[e - 1] >> shc) & 3u;
-            if (d0 == d1) meta |= kMetaPP;
-        }
-        const float sl = ldexpf(side, -l);
-        Node* nd = nodes + k;
-        nd->l2 = sl * sl;
-        nd->skip = (e < n) ? noff[e] : total;
-        nd->first = i;
-        nd->count = e - i;
-        nd->meta = meta;
-        arrived[k] = 0u;
-        uint32_t par;
-        if (l == 0) {
-            par = 0xFFFFFFFFu;
-        } else if (t > 0) {
+        const K hk = ci | (K)(((K)1 << sh) - 1);
+        e = upper_from(codes, n, hk, i + 1);
+    }
+    if (meta & kMetaLeaf) {
+        meta |= kMetaPP;
+    } else if (single_child_pp) {
+        const int shc = 2 * (B - l - 1);
+        if (((uint32_t)(ci >> shc) & 3u) == ((uint32_t)(codes[e - 1] >> shc) & 3u)) meta |= kMetaPP;
+    }
+    const float sl = ldexpf(prm->side, -l);
+    Node* nd = nodes + k;
+    nd->l2 = sl * sl;
+    nd->skip = (e < n) ? noff[e] : total;
+    nd->count = e - i;
+    nd->meta = meta;
+    uint32_t par = kNone;
+    if (l > 0) {
+        if (!(meta & kMetaHead)) {
             par = k - 1;
         } else {
             const int pl = l - 1;
@@ -494,27 +488,28 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit(const K* __restrict__ code
                 const K lk = ci & ~(K)(((K)1 << psh) - 1);
                 ip = lower_back(codes, lk, i);
             }
-            par = noff[ip] + (uint32_t)(pl - ((int)lcp1[ip]));
+            par = noff[ip] + (uint32_t)(pl - (int)lcp1[ip]);
         }
-        parent[k] = par;
+        child4[4ull * par + ((uint32_t)(ci >> (2 * (B - l))) & 3u)] = k;
     }
+    parent[k] = par;
 }
 
 // ------------------------------------------------------------------------------------------
-// upward pass: leaves sum their particles; the last child to finish combines its parent
+// upward pass: mass, centre of mass and (optionally) second moments
+//
+// Pass 1: each block owns a chunk of kUpChunk consecutive nodes and settles them level by
+// level (deepest first) in LDS; a node whose subtree runs past the chunk end is left for pass
+// 2. Those nodes are ancestors of the chunk's end, so there are at most kMaxLevels + 1 of them
+// per chunk. Pass 2: one block settles all of them level by level.
+// Children are combined in digit order, so the result does not depend on scheduling.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_tree_upward(const float4* __restrict__ pos4,
-                                                        Node* __restrict__ nodes,
-                                                        const uint32_t* __restrict__ parent,
-                                                        uint32_t* __restrict__ arrived,
-                                                        float4* __restrict__ m2,
-                                                        const BuildParams* __restrict__ prm,
-                                                        uint32_t cap_nodes) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= cap_nodes || k >= prm->n_nodes) return;
-    Node* nd = nodes + k;
-    if (!(nd->meta & kMetaLeaf)) return;
-    const uint32_t f = nd->first, c = nd->count;
+struct Moments {
+    float m, x, y, xx, xy, yy;
+};
+
+__device__ __forceinline__ Moments leaf_moments(const float4* __restrict__ pos4, uint32_t f,
+                                                uint32_t c, bool with_m2) {
     const float4 p0 = pos4[f];
     float M = 0.f, sx = 0.f, sy = 0.f;
     for (uint32_t j = f; j < f + c; ++j) {
@@ -524,69 +519,270 @@ __global__ __launch_bounds__(kBlock) void k_tree_upward(const float4* __restrict
         sy = fmaf(p.z, p.y - p0.y, sy);
     }
     const float inv = (M != 0.f) ? 1.0f / M : 0.f;
-    float cx = p0.x + sx * inv, cy = p0.y + sy * inv;
-    nd->cx = cx;
-    nd->cy = cy;
-    nd->mass = M;
-    if (m2) {
-        float a = 0.f, b = 0.f, d = 0.f;
+    Moments r{M, p0.x + sx * inv, p0.y + sy * inv, 0.f, 0.f, 0.f};
+    if (with_m2) {
         for (uint32_t j = f; j < f + c; ++j) {
             const float4 p = pos4[j];
-            const float rx = p.x - cx, ry = p.y - cy;
-            a = fmaf(p.z * rx, rx, a);
-            b = fmaf(p.z * rx, ry, b);
-            d = fmaf(p.z * ry, ry, d);
+            const float rx = p.x - r.x, ry = p.y - r.y;
+            r.xx = fmaf(p.z * rx, rx, r.xx);
+            r.xy = fmaf(p.z * rx, ry, r.xy);
+            r.yy = fmaf(p.z * ry, ry, r.yy);
         }
-        m2[k] = make_float4(a, b, d, 0.f);
     }
-    uint32_t cur = k, cur_count = c;
-    while (true) {
-        const uint32_t p = parent[cur];
-        if (p == 0xFFFFFFFFu) break;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t old = atomicAdd(&arrived[p], cur_count);
-        const uint32_t pc = nodes[p].count;
-        if (old + cur_count != pc) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const uint32_t end = nodes[p].skip;
-        // children: p+1, then follow skip pointers until the parent's subtree ends
-        uint32_t ch = p + 1;
-        const float ox = __hip_atomic_load(&nodes[ch].cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float oy = __hip_atomic_load(&nodes[ch].cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float PM = 0.f, px = 0.f, py = 0.f;
-        while (ch < end) {
-            const float cm = __hip_atomic_load(&nodes[ch].mass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float ccx = __hip_atomic_load(&nodes[ch].cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float ccy = __hip_atomic_load(&nodes[ch].cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            PM += cm;
-            px = fmaf(cm, ccx - ox, px);
-            py = fmaf(cm, ccy - oy, py);
-            ch = nodes[ch].skip;
+    return r;
+}
+
+// combine up to four children (absent ones have m = 0 and are flagged by has[t] = false)
+__device__ __forceinline__ Moments combine(const Moments* ch, const bool* has, bool with_m2) {
+    int first = 0;
+    while (first < 3 && !has[first]) ++first;
+    const float ox = ch[first].x, oy = ch[first].y;
+    float M = 0.f, px = 0.f, py = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (!has[t]) continue;
+        M += ch[t].m;
+        px = fmaf(ch[t].m, ch[t].x - ox, px);
+        py = fmaf(ch[t].m, ch[t].y - oy, py);
+    }
+    const float inv = (M != 0.f) ? 1.0f / M : 0.f;
+    Moments r{M, ox + px * inv, oy + py * inv, 0.f, 0.f, 0.f};
+    if (with_m2) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (!has[t]) continue;
+            const float rx = ch[t].x - r.x, ry = ch[t].y - r.y;
+            r.xx += ch[t].xx + ch[t].m * rx * rx;
+            r.xy += ch[t].xy + ch[t].m * rx * ry;
+            r.yy += ch[t].yy + ch[t].m * ry * ry;
         }
-        const float pinv = (PM != 0.f) ? 1.0f / PM : 0.f;
-        const float pcx = ox + px * pinv, pcy = oy + py * pinv;
-        nodes[p].cx = pcx;
-        nodes[p].cy = pcy;
-        nodes[p].mass = PM;
-        if (m2) {
-            float a = 0.f, b = 0.f, d = 0.f;
-            ch = p + 1;
-            while (ch < end) {
-                const float cm = __hip_atomic_load(&nodes[ch].mass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const float rx = __hip_atomic_load(&nodes[ch].cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - pcx;
-                const float ry = __hip_atomic_load(&nodes[ch].cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - pcy;
-                const float ca = __hip_atomic_load(&m2[ch].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const float cb = __hip_atomic_load(&m2[ch].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const float cd = __hip_atomic_load(&m2[ch].z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                a += ca + cm * rx * rx;
-                b += cb + cm * rx * ry;
-                d += cd + cm * ry * ry;
-                ch = nodes[ch].skip;
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ pos4,
+                                                     Node* __restrict__ nodes,
+                                                     const uint4* __restrict__ child4,
+                                                     float4* __restrict__ m2,
+                                                     const BuildParams* __restrict__ prm,
+                                                     uint32_t* __restrict__ deferred,
+                                                     uint32_t cap_nodes) {
+    __shared__ float s_m[kUpChunk], s_x[kUpChunk], s_y[kUpChunk];
+    __shared__ float s_xx[kUpChunk], s_xy[kUpChunk], s_yy[kUpChunk];
+    __shared__ uint32_t s_maxl;
+    __shared__ uint32_t s_ndef;
+    constexpr int kPer = kUpChunk / kBlock;
+    const uint32_t total = min(prm->n_nodes, cap_nodes);
+    const uint32_t c0 = blockIdx.x * kUpChunk;
+    uint32_t* my_def = deferred + (size_t)blockIdx.x * kDeferStride;
+    if (c0 >= total) {
+        if (threadIdx.x == 0) my_def[0] = 0;
+        return;
+    }
+    const uint32_t c1 = min(c0 + kUpChunk, total);
+    const bool with_m2 = m2 != nullptr;
+    if (threadIdx.x == 0) {
+        s_maxl = 0;
+        s_ndef = 0;
+    }
+    __syncthreads();
+    Node nd[kPer];
+    uint32_t lmax = 0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        if (k < c1) {
+            nd[r] = nodes[k];
+            lmax = max(lmax, nd[r].meta & kMetaLevelMask);
+        }
+    }
+    atomicMax(&s_maxl, lmax);
+    // leaves first, all at once: their particle loads overlap instead of waiting level by level
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        if (k >= c1 || !(nd[r].meta & kMetaLeaf)) continue;
+        const uint32_t s = k - c0;
+        const Moments mo = leaf_moments(pos4, nd[r].first, nd[r].count, with_m2);
+        s_m[s] = mo.m; s_x[s] = mo.x; s_y[s] = mo.y;
+        s_xx[s] = mo.xx; s_xy[s] = mo.xy; s_yy[s] = mo.yy;
+    }
+    __syncthreads();
+    for (int L = (int)s_maxl; L >= 0; --L) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t k = c0 + threadIdx.x + r * kBlock;
+            if (k >= c1 || (int)(nd[r].meta & kMetaLevelMask) != L) continue;
+            const uint32_t s = k - c0;
+            if (nd[r].meta & kMetaLeaf) {
+                continue;
+            } else if (nd[r].skip <= c1) {
+                const uint4 cs = child4[k];
+                const uint32_t ci[4] = {cs.x, cs.y, cs.z, cs.w};
+                Moments ch[4];
+                bool has[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    has[t] = ci[t] != kNone;
+                    const uint32_t q = has[t] ? ci[t] - c0 : s;
+                    ch[t] = Moments{s_m[q], s_x[q], s_y[q], s_xx[q], s_xy[q], s_yy[q]};
+                }
+                const Moments mo = combine(ch, has, with_m2);
+                s_m[s] = mo.m; s_x[s] = mo.x; s_y[s] = mo.y;
+                s_xx[s] = mo.xx; s_xy[s] = mo.xy; s_yy[s] = mo.yy;
             }
-            m2[p] = make_float4(a, b, d, 0.f);
         }
-        cur = p;
-        cur_count = pc;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        if (k >= c1) continue;
+        const uint32_t s = k - c0;
+        if ((nd[r].meta & kMetaLeaf) || nd[r].skip <= c1) {
+            nodes[k].cx = s_x[s];
+            nodes[k].cy = s_y[s];
+            nodes[k].mass = s_m[s];
+            if (with_m2) m2[k] = make_float4(s_xx[s], s_xy[s], s_yy[s], 0.f);
+        } else {
+            // ancestors of the chunk end (at most one per level)
+            const uint32_t slot = atomicAdd(&s_ndef, 1u);
+            if (slot < kDeferStride - 1) my_def[1 + slot] = k;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) my_def[0] = min(s_ndef, (uint32_t)(kDeferStride - 1));
+}
+
+__device__ __forceinline__ Moments load_moments(const Node* __restrict__ nodes,
+                                                const float4* __restrict__ m2, uint32_t q,
+                                                bool with_m2) {
+    Moments r;
+    r.m = __hip_atomic_load(&nodes[q].mass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    r.x = __hip_atomic_load(&nodes[q].cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    r.y = __hip_atomic_load(&nodes[q].cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    r.xx = r.xy = r.yy = 0.f;
+    if (with_m2) {
+        r.xx = __hip_atomic_load(&m2[q].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        r.xy = __hip_atomic_load(&m2[q].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        r.yy = __hip_atomic_load(&m2[q].z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return r;
+}
+
+// single block: the nodes left over by k_up_chunk, deepest level first. Their (node, level)
+// pairs are first compacted into LDS; if there are more than kTopMax of them the lists are read
+// from global memory at every level instead (same result, slower).
+__global__ __launch_bounds__(kBlock) void k_up_top(Node* __restrict__ nodes,
+                                                   const uint4* __restrict__ child4,
+                                                   float4* __restrict__ m2,
+                                                   const BuildParams* __restrict__ prm,
+                                                   const uint32_t* __restrict__ deferred,
+                                                   uint32_t nchunks, uint32_t cap_nodes) {
+    __shared__ uint32_t s_k[kTopMax];
+    __shared__ uint8_t s_l[kTopMax];
+    __shared__ uint32_t s_count, s_maxl;
+    const uint32_t total = min(prm->n_nodes, cap_nodes);
+    const uint32_t nch = min(nchunks, (total + kUpChunk - 1) / kUpChunk);
+    const bool with_m2 = m2 != nullptr;
+    // compaction: chunk lists in order, one thread per chunk in rounds of kBlock chunks
+    if (threadIdx.x == 0) {
+        s_count = 0;
+        s_maxl = 0;
+    }
+    __syncthreads();
+    uint32_t lmax = 0;
+    for (uint32_t b0 = 0; b0 < nch; b0 += kBlock) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t c = (b < nch) ? deferred[(size_t)b * kDeferStride] : 0u;
+        uint32_t tot;
+        const uint32_t off = block_exclusive_scan(c, &tot);
+        const uint32_t base = s_count;
+        for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t k = deferred[(size_t)b * kDeferStride + 1 + j];
+            const uint32_t l = nodes[k].meta & kMetaLevelMask;
+            lmax = max(lmax, l);
+            const uint32_t at = base + off + j;
+            if (at < kTopMax) {
+                s_k[at] = k;
+                s_l[at] = (uint8_t)l;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_count = base + tot;
+        __syncthreads();
+    }
+    atomicMax(&s_maxl, lmax);
+    __syncthreads();
+    const uint32_t D = s_count;
+    const bool in_lds = D <= kTopMax;
+    const uint32_t entries = in_lds ? D : nch * (kDeferStride - 1);
+    for (int L = (int)s_maxl; L >= 0; --L) {
+        for (uint32_t e = threadIdx.x; e < entries; e += kBlock) {
+            uint32_t k;
+            if (in_lds) {
+                if ((int)s_l[e] != L) continue;
+                k = s_k[e];
+            } else {
+                const uint32_t b = e / (kDeferStride - 1), slot = e % (kDeferStride - 1);
+                const uint32_t* lst = deferred + (size_t)b * kDeferStride;
+                if (slot >= lst[0]) continue;
+                k = lst[1 + slot];
+                if ((int)(nodes[k].meta & kMetaLevelMask) != L) continue;
+            }
+            const uint4 cs = child4[k];
+            const uint32_t ci[4] = {cs.x, cs.y, cs.z, cs.w};
+            Moments ch[4];
+            bool has[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                has[t] = ci[t] != kNone;
+                ch[t] = load_moments(nodes, m2, has[t] ? ci[t] : k, with_m2);
+            }
+            const Moments mo = combine(ch, has, with_m2);
+            __hip_atomic_store(&nodes[k].cx, mo.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&nodes[k].cy, mo.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&nodes[k].mass, mo.m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (with_m2) {
+                __hip_atomic_store(&m2[k].x, mo.xx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&m2[k].y, mo.xy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&m2[k].z, mo.yy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// tree statistics on request: deepest leaf level and leaves at the depth limit holding more
+// than the leaf capacity (one partial per block, summed by the host)
+__global__ __launch_bounds__(kBlock) void k_tree_stats(const Node* __restrict__ nodes,
+                                                       const BuildParams* __restrict__ prm,
+                                                       uint32_t cap_nodes, int depth_limit,
+                                                       uint32_t leaf_cap,
+                                                       uint2* __restrict__ out) {
+    uint32_t mx = 0, over = 0;
+    const uint32_t total = min(prm->n_nodes, cap_nodes);
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < total; k += gridDim.x * kBlock) {
+        const uint32_t meta = nodes[k].meta;
+        if (!(meta & kMetaLeaf)) continue;
+        const uint32_t l = meta & kMetaLevelMask;
+        mx = max(mx, l);
+        if ((int)l == depth_limit && nodes[k].count > leaf_cap) ++over;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, kWave));
+        over += (uint32_t)__shfl_xor((int)over, o, kWave);
+    }
+    __shared__ uint2 red[kWavesPerBlock];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_uint2(mx, over);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint2 r = red[0];
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            r.x = max(r.x, red[w].x);
+            r.y += red[w].y;
+        }
+        out[blockIdx.x] = r;
     }
 }
 
@@ -636,11 +832,12 @@ uint32_t bounds_grid(uint32_t n) {
 }
 
 void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, float4* partials,
-                   uint32_t* counter, BuildParams* prm, uint32_t tree_mode, const double* caller) {
+                   BuildParams* prm, uint32_t tree_mode, const double* caller) {
     const uint32_t g = bounds_grid(n);
-    k_bounds<<<g, kBlock, 0, s>>>(xs, ys, n, partials, counter, prm, tree_mode, caller ? 1 : 0,
-                                  caller ? caller[0] : 0.0, caller ? caller[1] : 0.0,
-                                  caller ? caller[2] : 0.0, caller ? caller[3] : 0.0);
+    k_bounds_partial<<<g, kBlock, 0, s>>>(xs, ys, n, partials);
+    k_bounds_final<<<1, kBlock, 0, s>>>(partials, g, n, prm, tree_mode, caller ? 1 : 0,
+                                        caller ? caller[0] : 0.0, caller ? caller[1] : 0.0,
+                                        caller ? caller[2] : 0.0, caller ? caller[3] : 0.0);
 }
 
 void launch_morton(hipStream_t s, const float* xs, const float* ys, uint32_t n,
@@ -703,29 +900,41 @@ void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float*
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
-                 Node* nodes, uint32_t* parent, uint32_t* arrived, float4* m2, uint32_t cap_nodes) {
+                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes) {
     const uint32_t g = div_up(n, kBlock);
     if (wide)
         k_tree_levels<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)codes, n, B, cap,
-                                                     fixed_depth, leaf_lvl, lcp1, cnt, prm);
+                                                     fixed_depth, leaf_lvl, lcp1, cnt);
     else
         k_tree_levels<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)codes, n, B, cap,
-                                                     fixed_depth, leaf_lvl, lcp1, cnt, prm);
+                                                     fixed_depth, leaf_lvl, lcp1, cnt);
     launch_exclusive_scan(s, cnt, noff, n, scan_partials, &prm->n_nodes);
+    k_tree_emit_heads<<<g, kBlock, 0, s>>>(n, lcp1, cnt, noff, nodes, (uint4*)child4, cap_nodes);
+    const uint32_t gn = div_up(cap_nodes, kBlock);
     if (wide)
-        k_tree_emit<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)codes, n, B, leaf_lvl, lcp1,
-                                                   cnt, noff, prm, single_child_pp, nodes, parent,
-                                                   arrived, cap_nodes);
+        k_tree_emit_nodes<uint64_t><<<gn, kBlock, 0, s>>>((const uint64_t*)codes, n, B, lcp1, noff,
+                                                          prm, single_child_pp, nodes, parent,
+                                                          child4, cap_nodes);
     else
-        k_tree_emit<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)codes, n, B, leaf_lvl, lcp1,
-                                                   cnt, noff, prm, single_child_pp, nodes, parent,
-                                                   arrived, cap_nodes);
+        k_tree_emit_nodes<uint32_t><<<gn, kBlock, 0, s>>>((const uint32_t*)codes, n, B, lcp1, noff,
+                                                          prm, single_child_pp, nodes, parent,
+                                                          child4, cap_nodes);
 }
 
-void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* parent,
-                   uint32_t* arrived, float4* m2, const BuildParams* prm, uint32_t cap_nodes) {
-    k_tree_upward<<<div_up(cap_nodes, kBlock), kBlock, 0, s>>>(pos4, nodes, parent, arrived, m2,
-                                                                prm, cap_nodes);
+uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }
+
+void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
+                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes) {
+    const uint32_t nch = up_chunks(cap_nodes);
+    k_up_chunk<<<nch, kBlock, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
+                                      cap_nodes);
+    k_up_top<<<1, kBlock, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, nch, cap_nodes);
+}
+
+void launch_tree_stats(hipStream_t s, const Node* nodes, const BuildParams* prm,
+                       uint32_t cap_nodes, int depth_limit, uint32_t leaf_cap, uint2* out,
+                       uint32_t blocks) {
+    k_tree_stats<<<blocks, kBlock, 0, s>>>(nodes, prm, cap_nodes, depth_limit, leaf_cap, out);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {

@@ -13,12 +13,21 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxLevels = 32;  // deepest tree level (32 bits per axis)
 constexpr int kStackDepth = 40; // > kMaxLevels + 1 pushes per traversal
+constexpr int kFoldEvery = 8;   // node visits between folds of the monopole batch
+constexpr int kPPUnroll = 8;    // sources per particle-particle fetch
+constexpr int kPosPad = 8;      // float4 entries readable past the last particle (>= kPPUnroll)
 
 enum : uint32_t {
     kMetaLevelMask = 0xFFu,
     kMetaLeaf = 1u << 8,   // deepest cell on its path (holds <= leaf capacity particles)
     kMetaPP = 1u << 9,     // near field is particle-particle over the whole cell
+    kMetaHead = 1u << 10,  // first node starting at its particle (its parent starts earlier)
 };
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kUpChunk = 2048;                 // nodes per block in the upward pass
+constexpr int kDeferStride = kMaxLevels + 2;   // count + at most one deferred node per level
+constexpr int kTopMax = 6144;                  // deferred nodes the top pass keeps in LDS
 
 enum : uint32_t {
     kTreeQuadTree = 0,     // adaptive, leaf capacity, QuadTree root and near-field rules

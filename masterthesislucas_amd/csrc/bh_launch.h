@@ -11,12 +11,13 @@ namespace bh {
 constexpr int kMaxLeafCap = 256;       // largest leaf capacity of the adaptive build
 constexpr uint32_t kMaxBoundsBlocks = 1024;
 constexpr uint32_t kMaxReduceBlocks = 1024;
+constexpr uint32_t kStatsBlocks = 256;
 constexpr int kKernelGravity = 0;
 constexpr int kKernelTsne = 1;
 
 uint32_t bounds_grid(uint32_t n);
 void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, float4* partials,
-                   uint32_t* counter, BuildParams* prm, uint32_t tree_mode, const double* caller);
+                   BuildParams* prm, uint32_t tree_mode, const double* caller);
 void launch_morton(hipStream_t s, const float* xs, const float* ys, uint32_t n,
                    const BuildParams* prm, int B, void* keys, uint32_t* vals, bool wide);
 uint32_t scan_partials_size(uint64_t n);
@@ -32,9 +33,13 @@ void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float*
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
-                 Node* nodes, uint32_t* parent, uint32_t* arrived, float4* m2, uint32_t cap_nodes);
-void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* parent,
-                   uint32_t* arrived, float4* m2, const BuildParams* prm, uint32_t cap_nodes);
+                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes);
+uint32_t up_chunks(uint32_t cap_nodes);
+void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
+                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes);
+void launch_tree_stats(hipStream_t s, const Node* nodes, const BuildParams* prm,
+                       uint32_t cap_nodes, int depth_limit, uint32_t leaf_cap, uint2* out,
+                       uint32_t blocks);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 void launch_fill_f32(hipStream_t s, float* p, uint64_t n, float v);
 void launch_iota(hipStream_t s, uint32_t* p, uint32_t n);
@@ -42,12 +47,16 @@ void launch_tree_slots(hipStream_t s, const uint32_t* codes, const Node* nodes,
                        const BuildParams* prm, uint32_t cap_nodes, int depth, uint32_t* first_child,
                        uint32_t* first_particle, uint32_t* count, uint64_t slots);
 
-void launch_traverse(hipStream_t s, int kernel, const float4* pos4, const Node* nodes,
-                     const BuildParams* prm, uint32_t n, float inv_theta2, float eps2, float scale,
-                     float* ax, float* ay, float* tot, unsigned long long* counters);
+constexpr int kTraverseVariants = 12;
+// returns the number of blocks launched (one counter partial each)
+uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
+                     const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
+                     float eps2, float scale, float* ax, float* ay, float* tot,
+                     unsigned long long* counters);
 uint32_t reduce_grid(uint32_t n);
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
-                       uint32_t* counter, double* out);
+                       double* out);
+uint32_t traverse_blocks(uint32_t n);
 void launch_unsort2(hipStream_t s, const uint32_t* id, uint32_t n, const float* a, const float* b,
                     float* ao, float* bo);
 void launch_kick_drift(hipStream_t s, float* x, float* y, float* vx, float* vy, const float* ax,

@@ -7,7 +7,8 @@
 // the cell's subtree; the set of lanes still descending is kept as a 64-bit mask, and a small
 // per-wave stack in LDS holds (mask, subtree end) for every level where the mask narrowed.
 // Near-field cells are summed particle by particle with the self term excluded by index.
-// Forces accumulate in FP32 with Kahan compensation per node visit.
+// Forces accumulate in FP32 with Kahan compensation: each particle-particle cell sum and each
+// batch of at most kFoldEvery monopoles is one compensated addition.
 #include "bh_common.h"
 #include "bh_launch.h"
 
@@ -20,29 +21,38 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
+// One interaction, accumulated into (ax, ay[, at]) for lanes where `use` holds.
 // gravity: a += m (q - p) / (|q - p|^2 + eps^2)^(3/2); potential -= m / sqrt(...)
 // t-SNE:   w = 1 / (1 + |p - q|^2); total += m w; derivative += m w^2 (p - q)
-template <int KERNEL>
-__device__ __forceinline__ void pair_term(float dx, float dy, float d2, float m, float eps2,
-                                          float& tx, float& ty, float& tt) {
+// (dx, dy) = q - p points from the sink to the source.
+template <int KERNEL, bool TOT>
+__device__ __forceinline__ void accum(float dx, float dy, float d2, float m, float eps2, bool use,
+                                      float& ax, float& ay, float& at) {
     if (KERNEL == kKernelGravity) {
         const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
         const float mi = m * inv;
-        const float s = mi * inv * inv;
-        tx = s * dx;
-        ty = s * dy;
-        tt = -mi;
+        float s = mi * inv * inv;
+        s = use ? s : 0.f;
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        if (TOT) at -= use ? mi : 0.f;
     } else {
         const float w = __builtin_amdgcn_rcpf(1.0f + d2);
         const float mw = m * w;
-        const float s = mw * w;
-        tx = -s * dx;  // dx, dy point from the sink to the source; the t-SNE term uses p - q
-        ty = -s * dy;
-        tt = mw;
+        float s = mw * w;
+        s = use ? s : 0.f;
+        ax = fmaf(-s, dx, ax);
+        ay = fmaf(-s, dy, ay);
+        if (TOT) at += use ? mw : 0.f;
     }
 }
 
-template <int KERNEL>
+// Traversal variants (picked per workload by the autotuner):
+//   NS:      sinks per lane; a wave walks the tree once for 64 NS Morton-consecutive particles,
+//            so every node fetch feeds NS independent interactions per lane
+//   BRANCHY: monopoles only when some lane accepts (else computed masked, unconditionally)
+//   PPU:     sources fetched per scalar load in the particle-particle loop
+template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
                                                      const BuildParams* __restrict__ prm,
@@ -51,131 +61,205 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                                      float* __restrict__ ay_out,
                                                      float* __restrict__ tot_out,
                                                      unsigned long long* __restrict__ counters) {
-    __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth];
+    __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
     __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = (wb * kWavesPerBlock + w) * kWave + lane;
-    const bool valid = i < n;
-    const float4 me = valid ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    uint64_t active = __ballot(valid);
-    if (!active) return;
+    const uint32_t base = (wb * kWavesPerBlock + w) * (uint32_t)(kWave * NS);
+    uint32_t idx[NS];
+    bool valid[NS];
+    float px[NS], py[NS];
+    uint64_t active[NS];
+    uint64_t any = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        idx[s] = base + s * kWave + lane;
+        valid[s] = idx[s] < n;
+        const float4 me = valid[s] ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        px[s] = me.x;
+        py[s] = me.y;
+        active[s] = __ballot(valid[s]);
+        any |= active[s];
+    }
     const uint32_t nn = prm->n_nodes;
 
-    float ax = 0.f, ay = 0.f, at = 0.f, cax = 0.f, cay = 0.f, cat = 0.f;
-    uint32_t npn = 0, npp = 0;
+    // Compensated totals (ax, ay, at) and their Kahan terms; monopoles of consecutive visits are
+    // first gathered in a short plain batch (bx, by, bt) that is folded into the compensated
+    // totals at every particle-particle visit and at least every kFoldEvery visits.
+    float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
+    uint32_t npn[NS], npp[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] = by[s] = bt[s] = 0.f;
+        npn[s] = npp[s] = 0u;
+    }
     uint32_t k = 0, end_cur = nn;
-    int sp = 0;
-    while (true) {
+    int sp = 0, since_fold = 0;
+    Node cur = nodes[0];
+    while (any) {
         while (k == end_cur && sp > 0) {
             --sp;
-            active = st_mask[w][sp];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) active[s] = st_mask[w][sp][s];
             end_cur = st_end[w][sp];
         }
         if (k >= end_cur) break;
-        const Node nd = nodes[k];
-        const bool on = (active >> lane) & 1ull;
-        const float dx = nd.cx - me.x, dy = nd.cy - me.y;
-        const float d2 = fmaf(dx, dx, dy * dy);
-        const bool acc = on && (d2 > nd.l2 * inv_theta2);
-        if (acc) {
-            float tx, ty, tt;
-            pair_term<KERNEL>(dx, dy, d2, nd.mass, eps2, tx, ty, tt);
-            kahan_add(ax, cax, tx);
-            kahan_add(ay, cay, ty);
-            kahan_add(at, cat, tt);
-            ++npn;
-        }
-        const uint64_t open = __ballot(on && !acc);
         // skip > k for every node of a well-formed tree; max() keeps the walk finite regardless
-        const uint32_t next = max(nd.skip, k + 1);
-        if (!open) {
-            k = next;
-            continue;
+        const uint32_t next = max(cur.skip, k + 1);
+        // the next node is k+1 (descend) or `next` (skip): fetch both while this one is used
+        const Node nd_down = nodes[min(k + 1, nn - 1)];
+        const Node nd_skip = nodes[min(next, nn - 1)];
+        const float lim = cur.l2 * inv_theta2;
+        uint64_t open[NS];
+        uint64_t open_any = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const bool on = (active[s] >> lane) & 1ull;
+            const float dx = cur.cx - px[s], dy = cur.cy - py[s];
+            const float d2 = fmaf(dx, dx, dy * dy);
+            const bool acc = on && (d2 > lim);
+            if (!BRANCHY || __ballot(acc)) {
+                accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
+                npn[s] += acc ? 1u : 0u;
+            }
+            open[s] = __ballot(on && !acc);
+            open_any |= open[s];
         }
-        if (nd.meta & kMetaPP) {
-            if ((open >> lane) & 1ull) {
-                float lx = 0.f, ly = 0.f, lt = 0.f;
-                const uint32_t f = nd.first, e = f + nd.count;
-                for (uint32_t j = f; j < e; ++j) {
-                    const float4 q = pos4[j];
-                    const float qx = q.x - me.x, qy = q.y - me.y;
-                    const float q2 = fmaf(qx, qx, qy * qy);
-                    float tx, ty, tt;
-                    pair_term<KERNEL>(qx, qy, q2, q.z, eps2, tx, ty, tt);
-                    if (j != i) {
-                        lx += tx;
-                        ly += ty;
-                        lt += tt;
+        bool down = false;
+        if (open_any) {
+            if (cur.meta & kMetaPP) {
+                bool mine[NS];
+                float lx[NS], ly[NS], lt[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    mine[s] = (open[s] >> lane) & 1ull;
+                    lx[s] = ly[s] = lt[s] = 0.f;
+                }
+                const uint32_t f = cur.first, e = f + cur.count;
+                // PPU sources per scalar fetch; pos4 is padded past n, lanes mask the tail
+                for (uint32_t j = f; j < e; j += PPU) {
+                    float4 q[PPU];
+#pragma unroll
+                    for (int t = 0; t < PPU; ++t) q[t] = pos4[j + t];
+#pragma unroll
+                    for (int t = 0; t < PPU; ++t) {
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            const float qx = q[t].x - px[s], qy = q[t].y - py[s];
+                            const float q2 = fmaf(qx, qx, qy * qy);
+                            const bool use = mine[s] && (j + t < e) && (j + t != idx[s]);
+                            accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s], ly[s], lt[s]);
+                        }
                     }
                 }
-                kahan_add(ax, cax, lx);
-                kahan_add(ay, cay, ly);
-                kahan_add(at, cat, lt);
-                npp += nd.count - ((i >= f && i < e) ? 1u : 0u);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    npp[s] += mine[s] ? cur.count - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
+                    kahan_add(ax[s], cax[s], lx[s] + bx[s]);
+                    kahan_add(ay[s], cay[s], ly[s] + by[s]);
+                    if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
+                    bx[s] = by[s] = bt[s] = 0.f;
+                }
+                since_fold = 0;
+            } else {
+                bool changed = false;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) changed |= (open[s] != active[s]);
+                if (changed && sp < kStackDepth) {
+                    if (lane == 0) {
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) st_mask[w][sp][s] = active[s];
+                        st_end[w][sp] = end_cur;
+                    }
+                    ++sp;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) active[s] = open[s];
+                    end_cur = next;
+                }
+                down = true;
             }
-            k = next;
-            continue;
         }
-        if (open != active && sp < kStackDepth) {
-            if (lane == 0) {
-                st_mask[w][sp] = active;
-                st_end[w][sp] = end_cur;
+        if (++since_fold >= kFoldEvery) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                kahan_add(ax[s], cax[s], bx[s]);
+                kahan_add(ay[s], cay[s], by[s]);
+                if (TOT) kahan_add(at[s], cat[s], bt[s]);
+                bx[s] = by[s] = bt[s] = 0.f;
             }
-            ++sp;
-            active = open;
-            end_cur = next;
+            since_fold = 0;
         }
-        k = k + 1;
+        k = down ? k + 1 : next;
+        cur = down ? nd_down : nd_skip;
     }
-    if (valid) {
-        ax_out[i] = scale * (ax - cax);
-        ay_out[i] = scale * (ay - cay);
-        if (tot_out) tot_out[i] = scale * (at - cat);
+    unsigned long long s_pn = 0, s_pp = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        kahan_add(ax[s], cax[s], bx[s]);
+        kahan_add(ay[s], cay[s], by[s]);
+        if (TOT) kahan_add(at[s], cat[s], bt[s]);
+        if (valid[s]) {
+            ax_out[idx[s]] = scale * (ax[s] - cax[s]);
+            ay_out[idx[s]] = scale * (ay[s] - cay[s]);
+            if (TOT) tot_out[idx[s]] = scale * (at[s] - cat[s]);
+        }
+        s_pn += valid[s] ? npn[s] : 0u;
+        s_pp += valid[s] ? npp[s] : 0u;
     }
-    if (counters) {
-        const unsigned long long s_pn = wave_sum_u64(valid ? npn : 0u);
-        const unsigned long long s_pp = wave_sum_u64(valid ? npp : 0u);
+    if (counters) {  // one partial per block (no same-address device atomics)
+        __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
+        s_pn = wave_sum_u64(s_pn);
+        s_pp = wave_sum_u64(s_pp);
         if (lane == 0) {
-            atomicAdd(&counters[0], s_pn);
-            atomicAdd(&counters[1], s_pp);
+            s_cnt[w][0] = s_pn;
+            s_cnt[w][1] = s_pp;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long a = 0, b = 0;
+            for (int k2 = 0; k2 < kWavesPerBlock; ++k2) {
+                a += s_cnt[k2][0];
+                b += s_cnt[k2][1];
+            }
+            counters[2 * (size_t)wb] = a;
+            counters[2 * (size_t)wb + 1] = b;
         }
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// FP64 reduction of per-particle totals (t-SNE Z, potential energy): fixed partition, the
-// last block sums the block partials in order, so the result is deterministic.
+// FP64 reduction of per-particle totals (t-SNE Z, potential energy): fixed partition into
+// block partials, then one block sums them in order, so the result is deterministic.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_reduce_f64(const float* __restrict__ v, uint32_t n,
-                                                       const float* __restrict__ w,
-                                                       double* __restrict__ partials,
-                                                       uint32_t* __restrict__ counter,
-                                                       double* __restrict__ out) {
+__device__ __forceinline__ double block_sum_d(double s) {
     __shared__ double red[kWavesPerBlock];
-    __shared__ int last;
-    double s = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        s += w ? (double)v[i] * (double)w[i] : (double)v[i];
     s = wave_sum_d(s);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) red[wv] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = 0.0;
-        for (int k = 0; k < kWavesPerBlock; ++k) b += red[k];
-        partials[blockIdx.x] = b;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-    }
+    double b = 0.0;
+    for (int k = 0; k < kWavesPerBlock; ++k) b += red[k];
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double t = 0.0;
-    for (uint32_t b = 0; b < gridDim.x; ++b)
-        t += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *out = t;
-    *counter = 0u;
+    return b;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_f64_partial(const float* __restrict__ v,
+                                                               uint32_t n,
+                                                               const float* __restrict__ w,
+                                                               double* __restrict__ partials) {
+    double s = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        s += w ? (double)v[i] * (double)w[i] : (double)v[i];
+    s = block_sum_d(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_f64_final(const double* __restrict__ partials,
+                                                             uint32_t nb, double* __restrict__ out) {
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlock) s += partials[b];
+    s = block_sum_d(s);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // out[id[i]] = a[i] (results back to the caller's particle order)
@@ -220,16 +304,53 @@ __global__ __launch_bounds__(kBlock) void k_kick(float* __restrict__ vx, float* 
 // ------------------------------------------------------------------------------------------
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-void launch_traverse(hipStream_t s, int kernel, const float4* pos4, const Node* nodes,
-                     const BuildParams* prm, uint32_t n, float inv_theta2, float eps2, float scale,
-                     float* ax, float* ay, float* tot, unsigned long long* counters) {
-    const uint32_t g = div_up(n, kBlock);
-    if (kernel == kKernelGravity)
-        k_traverse<kKernelGravity><<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2, eps2, scale,
-                                                        ax, ay, tot, counters);
-    else
-        k_traverse<kKernelTsne><<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2, eps2, scale,
-                                                     ax, ay, tot, counters);
+// variant v: sinks per lane {1, 2, 4}[v / 4]; monopoles branchy (v % 4 < 2) or masked;
+// particle-particle fetch of 1 (v even) or 4 (v odd) sources
+template <int KERNEL, bool TOT>
+static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const float4* pos4,
+                                 const Node* nodes, const BuildParams* prm, float inv_theta2,
+                                 float eps2, float scale, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters) {
+    const int ns = variant < 4 ? 1 : (variant < 8 ? 2 : 4);
+    const uint32_t g = div_up(n, (uint64_t)kBlock * ns);
+#define BH_TRAV(NS, BR, U)                                                                        \
+    k_traverse<KERNEL, TOT, NS, BR, U><<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2,     \
+                                                             eps2, scale, ax, ay, tot, counters)
+    switch (variant) {
+        case 0: BH_TRAV(1, true, 1); break;
+        case 1: BH_TRAV(1, true, 4); break;
+        case 2: BH_TRAV(1, false, 1); break;
+        case 3: BH_TRAV(1, false, 4); break;
+        case 4: BH_TRAV(2, true, 1); break;
+        case 5: BH_TRAV(2, true, 4); break;
+        case 6: BH_TRAV(2, false, 1); break;
+        case 7: BH_TRAV(2, false, 4); break;
+        case 8: BH_TRAV(4, true, 1); break;
+        case 9: BH_TRAV(4, true, 4); break;
+        case 10: BH_TRAV(4, false, 1); break;
+        default: BH_TRAV(4, false, 4); break;
+    }
+#undef BH_TRAV
+    return g;
+}
+
+uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
+                         const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
+                         float eps2, float scale, float* ax, float* ay, float* tot,
+                         unsigned long long* counters) {
+    if (kernel == kKernelGravity) {
+        if (tot)
+            return traverse_variant<kKernelGravity, true>(s, variant, n, pos4, nodes, prm,
+                                                          inv_theta2, eps2, scale, ax, ay, tot,
+                                                          counters);
+        return traverse_variant<kKernelGravity, false>(s, variant, n, pos4, nodes, prm, inv_theta2,
+                                                       eps2, scale, ax, ay, tot, counters);
+    }
+    if (tot)
+        return traverse_variant<kKernelTsne, true>(s, variant, n, pos4, nodes, prm, inv_theta2,
+                                                   eps2, scale, ax, ay, tot, counters);
+    return traverse_variant<kKernelTsne, false>(s, variant, n, pos4, nodes, prm, inv_theta2, eps2,
+                                                scale, ax, ay, tot, counters);
 }
 
 uint32_t reduce_grid(uint32_t n) {
@@ -240,9 +361,13 @@ uint32_t reduce_grid(uint32_t n) {
 }
 
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
-                       uint32_t* counter, double* out) {
-    k_reduce_f64<<<reduce_grid(n), kBlock, 0, s>>>(v, n, w, partials, counter, out);
+                       double* out) {
+    const uint32_t g = reduce_grid(n);
+    k_reduce_f64_partial<<<g, kBlock, 0, s>>>(v, n, w, partials);
+    k_reduce_f64_final<<<1, kBlock, 0, s>>>(partials, g, out);
 }
+
+uint32_t traverse_blocks(uint32_t n) { return div_up(n, kBlock); }
 
 void launch_unsort2(hipStream_t s, const uint32_t* id, uint32_t n, const float* a, const float* b,
                     float* ao, float* bo) {

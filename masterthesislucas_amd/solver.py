@@ -118,6 +118,10 @@ class BHContext:
                                              C.byref(tot) if want_total else None))
         return tot.value
 
+    def forces_device_only(self, theta):
+        """Traversal only: results stay on the device (used for timing)."""
+        self._c(_lib.load().bh_forces_device(self._h, float(theta), None, None, None))
+
     def leapfrog(self, steps, dt, theta):
         self._c(_lib.load().bh_leapfrog(self._h, int(steps), float(dt), float(theta)))
 
@@ -177,6 +181,13 @@ class BHContext:
         la = np.zeros(_lib.NUM_PHASES, np.uint64)
         self._c(_lib.load().bh_profile_read(self._h, _ptr(ms), _ptr(la), 1 if reset else 0))
         return {name: (float(ms[i]), int(la[i])) for i, name in enumerate(_lib.PHASE_NAMES)}
+
+    def set_variant(self, variant):
+        self._c(_lib.load().bh_set_traverse_variant(self._h, int(variant)))
+
+    @staticmethod
+    def variant_count():
+        return int(_lib.load().bh_traverse_variant_count())
 
     def synchronize(self):
         self._c(_lib.load().bh_synchronize(self._h))

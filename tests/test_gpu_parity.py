@@ -309,3 +309,25 @@ def test_rebuild_reuses_context(gpu):
         fresh.build()
         fx, fy, _ = fresh.forces(0.5)
         assert np.array_equal(ax, fx) and np.array_equal(ay, fy)
+
+
+@pytest.mark.parametrize("variant", range(12))
+@pytest.mark.parametrize("kernel", ["gravity", "tsne"])
+def test_every_traversal_variant(gpu, variant, kernel):
+    """All autotuner choices give the oracle's forces (they differ only in summation order)."""
+    p = two_plummer_disks(40000, seed=13)
+    ctx = _ctx(p.n, kernel=kernel, tree_mode="quadtree", softening=EPS)
+    m = p.m if kernel == "gravity" else None
+    ctx.set_particles(p.x, p.y, m)
+    ctx.set_variant(variant)
+    ctx.build()
+    ax, ay, tot = ctx.forces(0.5)
+    okern = O.KERNEL_GRAVITY if kernel == "gravity" else O.KERNEL_TSNE
+    t = oracle_tree(p.x, p.y, m, mode="quadtree")
+    bx, by, bt, cnt = oracle_forces(t, okern, 0.5, eps=EPS if kernel == "gravity" else 0.0)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= MEDIAN_TOL
+    assert np.quantile(r, 0.99) <= P99_TOL
+    assert abs(tot - bt.sum()) <= 1e-5 * abs(bt.sum())
+    pn, pp = ctx.counters()
+    assert abs(int(pn) - int(cnt[0])) <= 1e-3 * cnt[0]
