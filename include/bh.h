@@ -114,6 +114,44 @@ bh_status bh_leapfrog(bh_ctx* ctx, int steps, float dt, double theta);
 bh_status bh_direct_fp64(bh_ctx* ctx, const uint32_t* sinks, uint32_t nsinks, double* ax,
                          double* ay, double* total);
 
+/* ---- t-SNE (the reference TSNE driver, tsne.h:42-563) ---- */
+typedef struct bh_tsne_params {
+    double learning_rate;    /* tsne.h:90: N / 15 */
+    double momentum_early;   /* tsne.h:186: 0.5 before momentum_switch */
+    double momentum_late;    /* 0.8 after */
+    double exag_early;       /* tsne.h:358: 16 (4 x early exaggeration 4) before exag_switch */
+    double exag_late;        /* 4 */
+    double theta_min;        /* thetaFunction (tsne.h:208-233) */
+    double theta_max;
+    int32_t momentum_switch; /* 250 */
+    int32_t exag_switch;     /* 250 */
+    int32_t schedule_iters;  /* 999: theta(t) = min + (max-min)(1/2 + cos(pi min(t/999, 1))/2) */
+    int32_t pad;
+} bh_tsne_params;
+
+/* reference constants for n points, constant theta (theta_min = theta_max = theta) */
+bh_status bh_tsne_params_default(bh_tsne_params* p, uint32_t n, double theta);
+/* theta used at iteration t: theta_max at t = 0, the cosine schedule at t - 1 afterwards */
+double bh_tsne_theta(const bh_tsne_params* p, int32_t iteration);
+/* P matrix in CSR, caller order, rows sorted by (row, col) as setupEfficientPmatrix builds it
+ * (tsne.h:462-520); values FP32. The context must use BH_KERNEL_TSNE. */
+bh_status bh_tsne_set_p(bh_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                        const float* val, uint32_t nnz);
+/* embedding (host, caller order); the previous positions start equal to it (tsne.h:553-555) */
+bh_status bh_tsne_set_embedding(bh_ctx* ctx, const float* yx, const float* yy);
+bh_status bh_tsne_get_embedding(bh_ctx* ctx, float* yx, float* yy);
+/* one iteration t = updateDerivative + updatePoints (tsne.h:179-206, 272-301, 356-381):
+ * theta(t) is theta_max at t = 0 and the schedule at t-1 afterwards, like the reference, whose
+ * updatePoints sets the theta the next iteration uses. dx/dy (host, may be NULL) receive the
+ * derivative d = (4/Z) repulsion + attraction at the pre-step embedding; z the Barnes-Hut Z. */
+bh_status bh_tsne_step(bh_ctx* ctx, const bh_tsne_params* p, int32_t iteration, float* dx,
+                       float* dy, double* z);
+/* iterations first .. first+count-1 without host synchronisation in between */
+bh_status bh_tsne_run(bh_ctx* ctx, const bh_tsne_params* p, int32_t first, int32_t count);
+/* KL divergence of the current embedding with the exact FP64 Z (costFunction,
+ * tsne.h:383-460; O(N^2) on the GPU in FP64) */
+bh_status bh_tsne_kl(bh_ctx* ctx, double* kl, double* z_exact);
+
 /* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
 /* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */
 bh_status bh_get_bounds(bh_ctx* ctx, double* bounds, double* root_side);

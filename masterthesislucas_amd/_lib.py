@@ -29,6 +29,14 @@ class BHConfig(C.Structure):
                 ("theta", C.c_double)]
 
 
+class TsneParams(C.Structure):
+    _fields_ = [("learning_rate", C.c_double), ("momentum_early", C.c_double),
+                ("momentum_late", C.c_double), ("exag_early", C.c_double),
+                ("exag_late", C.c_double), ("theta_min", C.c_double), ("theta_max", C.c_double),
+                ("momentum_switch", C.c_int32), ("exag_switch", C.c_int32),
+                ("schedule_iters", C.c_int32), ("pad", C.c_int32)]
+
+
 class BHError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -52,6 +60,14 @@ SIGNATURES = {
     "bh_forces_device": (_i32, [_vp, _d, _vp, _vp, _vp]),
     "bh_leapfrog": (_i32, [_vp, C.c_int, _f, _d]),
     "bh_direct_fp64": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),
+    "bh_tsne_params_default": (_i32, [C.POINTER(TsneParams), _u32, _d]),
+    "bh_tsne_theta": (_d, [C.POINTER(TsneParams), _i32]),
+    "bh_tsne_set_p": (_i32, [_vp, _u32, _vp, _vp, _vp, _u32]),
+    "bh_tsne_set_embedding": (_i32, [_vp, _vp, _vp]),
+    "bh_tsne_get_embedding": (_i32, [_vp, _vp, _vp]),
+    "bh_tsne_step": (_i32, [_vp, C.POINTER(TsneParams), _i32, _vp, _vp, _vp]),
+    "bh_tsne_run": (_i32, [_vp, C.POINTER(TsneParams), _i32, _i32]),
+    "bh_tsne_kl": (_i32, [_vp, _vp, _vp]),
     "bh_get_bounds": (_i32, [_vp, _vp, _vp]),
     "bh_get_morton": (_i32, [_vp, _vp, _vp]),
     "bh_tree_info": (_i32, [_vp, _vp, _vp, _vp]),

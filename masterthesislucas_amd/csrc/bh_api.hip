@@ -86,7 +86,20 @@ struct bh_ctx {
     unsigned long long* h_counters = nullptr;  // 2 per traversal block
     uint2* h_stats = nullptr;
 
-    int variant = 2;  // traversal kernel variant (see launch_traverse)
+    // t-SNE state (caller order): P in CSR, embedding triple buffer, repulsion, derivative
+    uint32_t *p_row = nullptr, *p_col = nullptr;
+    float* p_val = nullptr;
+    uint32_t p_n = 0, p_nnz = 0;
+    float *ty_x[3]{}, *ty_y[3]{};
+    int ty_cur = 0;  // ty[ty_cur] current, ty[(ty_cur+2)%3] previous
+    bool ty_set = false;
+    float *rep_x = nullptr, *rep_y = nullptr, *tder_x = nullptr, *tder_y = nullptr;
+    double* tz = nullptr;
+    double* kl_part = nullptr;
+    double* kl_out = nullptr;
+    uint32_t* iota_dev = nullptr;
+
+    int variant = 4;  // traversal kernel variant (see launch_traverse)
     uint32_t trav_blocks = 0;  // blocks of the last traversal launch
     bool profile = false;
     PhaseTimer phases[BH_NUM_PHASES];
@@ -267,6 +280,9 @@ void bh_destroy(bh_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->dsinks) (void)hipFree(c->dsinks);
+    if (c->p_row) (void)hipFree(c->p_row);
+    if (c->p_col) (void)hipFree(c->p_col);
+    if (c->p_val) (void)hipFree(c->p_val);
     if (c->dpart) (void)hipFree(c->dpart);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
@@ -381,6 +397,21 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     c->max_blocks = traverse_blocks(N);
     TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
+    if (cfg->kernel == BH_KERNEL_TSNE) {
+        for (int k = 0; k < 3; ++k) {
+            TRY(dalloc(c, &c->ty_x[k], N));
+            TRY(dalloc(c, &c->ty_y[k], N));
+        }
+        TRY(dalloc(c, &c->rep_x, N));
+        TRY(dalloc(c, &c->rep_y, N));
+        TRY(dalloc(c, &c->tder_x, N));
+        TRY(dalloc(c, &c->tder_y, N));
+        TRY(dalloc(c, &c->tz, 2));
+        TRY(dalloc(c, &c->kl_part, kMaxReduceBlocks));
+        TRY(dalloc(c, &c->kl_out, 2));
+        TRY(dalloc(c, &c->iota_dev, N));
+        launch_iota(c->stream, c->iota_dev, N);
+    }
     if (!c->wide) {
         TRY(dalloc(c, &c->slot_a, c->slot_cap));
         TRY(dalloc(c, &c->slot_b, c->slot_cap));
@@ -552,6 +583,9 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
     const uint64_t need_s = 2ull * ns, need_p = (uint64_t)(3ull * S + 3ull) * ns;
     if (need_s > c->dsink_cap) {
         if (c->dsinks) (void)hipFree(c->dsinks);
+    if (c->p_row) (void)hipFree(c->p_row);
+    if (c->p_col) (void)hipFree(c->p_col);
+    if (c->p_val) (void)hipFree(c->p_val);
         c->dsinks = nullptr;
         c->dsink_cap = 0;
         bh_status st = hip_check(c, hipMalloc((void**)&c->dsinks, need_s * 4), "hipMalloc sinks");
@@ -585,6 +619,197 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
         st = hip_check(c, hipMemcpyAsync(total, out + 2 * (uint64_t)ns, b, hipMemcpyDeviceToHost, s), "tot");
     if (st != BH_OK) return st;
     return hip_check(c, hipStreamSynchronize(s), "direct");
+}
+
+// ---------------------------------------------------------------------------------------------
+// t-SNE
+// ---------------------------------------------------------------------------------------------
+bh_status bh_tsne_params_default(bh_tsne_params* p, uint32_t n, double theta) {
+    if (!p) return BH_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    p->learning_rate = (double)n / 15.0;
+    p->momentum_early = 0.5;
+    p->momentum_late = 0.8;
+    p->exag_early = 16.0;
+    p->exag_late = 4.0;
+    p->theta_min = theta;
+    p->theta_max = theta;
+    p->momentum_switch = 250;
+    p->exag_switch = 250;
+    p->schedule_iters = 999;
+    return BH_OK;
+}
+
+// thetaFunction (tsne.h:208-233) runs inside updatePoints before the counter increments, so
+// iteration t traverses with the value computed at t - 1; the constructor sets theta_max
+// (tsne.h:105-108) for the first iteration.
+static double tsne_theta(const bh_tsne_params* p, int32_t t) {
+    if (t <= 0) return p->theta_max;
+    double r = (double)(t - 1) / (double)(p->schedule_iters > 0 ? p->schedule_iters : 999);
+    if (r > 1.0) r = 1.0;
+    return p->theta_min + (0.5 + 0.5 * std::cos(M_PI * r)) * (p->theta_max - p->theta_min);
+}
+
+double bh_tsne_theta(const bh_tsne_params* p, int32_t iteration) {
+    return p ? tsne_theta(p, iteration) : 0.0;
+}
+
+bh_status bh_tsne_set_p(bh_ctx* c, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                        const float* val, uint32_t nnz) {
+    if (!c || !row_ptr || (nnz && (!col || !val))) return BH_ERR_ARG;
+    if (c->cfg.kernel != BH_KERNEL_TSNE) return fail(c, BH_ERR_STATE, "t-SNE needs BH_KERNEL_TSNE");
+    if (n == 0 || n > c->cfg.n_max) return fail(c, BH_ERR_ARG, "n=%u outside (0, n_max]", n);
+    if (row_ptr[n] != nnz) return fail(c, BH_ERR_ARG, "row_ptr[n]=%u differs from nnz=%u", row_ptr[n], nnz);
+    for (uint32_t r = 0; r < n; ++r)
+        if (row_ptr[r] > row_ptr[r + 1]) return fail(c, BH_ERR_ARG, "row_ptr decreases at %u", r);
+    for (uint32_t e = 0; e < nnz; ++e)
+        if (col[e] >= n) return fail(c, BH_ERR_ARG, "column %u out of range at %u", col[e], e);
+    if (c->p_row) (void)hipFree(c->p_row);
+    if (c->p_col) (void)hipFree(c->p_col);
+    if (c->p_val) (void)hipFree(c->p_val);
+    c->p_row = nullptr;
+    c->p_col = nullptr;
+    c->p_val = nullptr;
+    bh_status st = hip_check(c, hipMalloc((void**)&c->p_row, (size_t)(n + 1) * 4), "hipMalloc P");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&c->p_col, (size_t)(nnz ? nnz : 1) * 4), "hipMalloc P");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&c->p_val, (size_t)(nnz ? nnz : 1) * 4), "hipMalloc P");
+    if (st == BH_OK) st = hip_check(c, hipMemcpy(c->p_row, row_ptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice), "P");
+    if (st == BH_OK && nnz) st = hip_check(c, hipMemcpy(c->p_col, col, (size_t)nnz * 4, hipMemcpyHostToDevice), "P");
+    if (st == BH_OK && nnz) st = hip_check(c, hipMemcpy(c->p_val, val, (size_t)nnz * 4, hipMemcpyHostToDevice), "P");
+    if (st != BH_OK) return st;
+    c->p_n = n;
+    c->p_nnz = nnz;
+    return BH_OK;
+}
+
+bh_status bh_tsne_set_embedding(bh_ctx* c, const float* yx, const float* yy) {
+    if (!c || !yx || !yy) return BH_ERR_ARG;
+    if (c->cfg.kernel != BH_KERNEL_TSNE) return fail(c, BH_ERR_STATE, "t-SNE needs BH_KERNEL_TSNE");
+    if (!c->p_n) return fail(c, BH_ERR_STATE, "set the P matrix first");
+    const size_t b = (size_t)c->p_n * 4;
+    bh_status st = BH_OK;
+    for (int k = 0; k < 3 && st == BH_OK; ++k) {
+        st = hip_check(c, hipMemcpy(c->ty_x[k], yx, b, hipMemcpyHostToDevice), "embedding");
+        if (st == BH_OK) st = hip_check(c, hipMemcpy(c->ty_y[k], yy, b, hipMemcpyHostToDevice), "embedding");
+    }
+    if (st != BH_OK) return st;
+    c->ty_cur = 0;
+    c->ty_set = true;
+    return BH_OK;
+}
+
+bh_status bh_tsne_get_embedding(bh_ctx* c, float* yx, float* yy) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->ty_set) return fail(c, BH_ERR_STATE, "no embedding");
+    const size_t b = (size_t)c->p_n * 4;
+    bh_status st = BH_OK;
+    if (yx) st = hip_check(c, hipMemcpyAsync(yx, c->ty_x[c->ty_cur], b, hipMemcpyDeviceToHost, c->stream), "y");
+    if (st == BH_OK && yy) st = hip_check(c, hipMemcpyAsync(yy, c->ty_y[c->ty_cur], b, hipMemcpyDeviceToHost, c->stream), "y");
+    if (st != BH_OK) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), "get embedding");
+}
+
+// stage the current embedding as the solver's particles and build the tree
+static bh_status tsne_stage_build(bh_ctx* c) {
+    const uint32_t n = c->p_n;
+    c->n = n;
+    c->cur = 0;
+    launch_tsne_stage(c->stream, c->ty_x[c->ty_cur], c->ty_y[c->ty_cur], n, c->x[0], c->y[0],
+                      c->m[0], c->id[0]);
+    return do_build(c, nullptr);
+}
+
+static bh_status tsne_iteration(bh_ctx* c, const bh_tsne_params* p, int32_t t, bool keep_deriv) {
+    bh_status st = tsne_stage_build(c);
+    if (st != BH_OK) return st;
+    st = do_forces(c, tsne_theta(p, t), true);  // repulsion + per-point Z terms, Z -> dsum
+    if (st != BH_OK) return st;
+    hipStream_t s = c->stream;
+    const uint32_t n = c->p_n;
+    launch_unsort2(s, c->id[c->cur], n, c->ax, c->ay, c->rep_x, c->rep_y);
+    const int cur = c->ty_cur, prev = (cur + 2) % 3, next = (cur + 1) % 3;
+    const float exag = (float)(t < p->exag_switch ? p->exag_early : p->exag_late);
+    const float mu = (float)(t < p->momentum_switch ? p->momentum_early : p->momentum_late);
+    launch_tsne_update(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[cur], c->ty_y[cur],
+                       c->ty_x[prev], c->ty_y[prev], c->rep_x, c->rep_y, c->dsum, exag,
+                       (float)p->learning_rate, mu, c->ty_x[next], c->ty_y[next],
+                       keep_deriv ? c->tder_x : nullptr, keep_deriv ? c->tder_y : nullptr);
+    // rotate: next becomes current, current becomes previous
+    c->ty_cur = next;
+    return launch_status(c, "t-SNE step");
+}
+
+bh_status bh_tsne_step(bh_ctx* c, const bh_tsne_params* p, int32_t iteration, float* dx, float* dy,
+                       double* z) {
+    if (!c || !p || iteration < 0) return BH_ERR_ARG;
+    if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
+    bh_status st = tsne_iteration(c, p, iteration, dx || dy);
+    if (st != BH_OK) return st;
+    hipStream_t s = c->stream;
+    const size_t b = (size_t)c->p_n * 4;
+    if (dx) st = hip_check(c, hipMemcpyAsync(dx, c->tder_x, b, hipMemcpyDeviceToHost, s), "dx");
+    if (st == BH_OK && dy) st = hip_check(c, hipMemcpyAsync(dy, c->tder_y, b, hipMemcpyDeviceToHost, s), "dy");
+    if (st == BH_OK && z) st = hip_check(c, hipMemcpyAsync(c->h_dsum, c->dsum, 8, hipMemcpyDeviceToHost, s), "z");
+    if (st != BH_OK) return st;
+    st = hip_check(c, hipStreamSynchronize(s), "t-SNE step");
+    if (st == BH_OK && z) *z = *c->h_dsum;
+    return st;
+}
+
+bh_status bh_tsne_run(bh_ctx* c, const bh_tsne_params* p, int32_t first, int32_t count) {
+    if (!c || !p || first < 0 || count < 0) return BH_ERR_ARG;
+    if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
+    for (int32_t t = first; t < first + count; ++t) {
+        bh_status st = tsne_iteration(c, p, t, false);
+        if (st != BH_OK) return st;
+    }
+    return hip_check(c, hipStreamSynchronize(c->stream), "t-SNE run");
+}
+
+bh_status bh_tsne_kl(bh_ctx* c, double* kl, double* z_exact) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
+    bh_status st = tsne_stage_build(c);
+    if (st != BH_OK) return st;
+    const uint32_t n = c->p_n;
+    // exact Z: FP64 direct sum over all sinks (their totals), then the KL sum over P
+    const uint32_t S = direct_splits(n, n);
+    const uint64_t need_s = 2ull * n, need_p = (uint64_t)(3ull * S + 3ull) * n;
+    if (need_s > c->dsink_cap) {
+        if (c->dsinks) (void)hipFree(c->dsinks);
+        c->dsinks = nullptr;
+        c->dsink_cap = 0;
+        st = hip_check(c, hipMalloc((void**)&c->dsinks, need_s * 4), "hipMalloc sinks");
+        if (st != BH_OK) return st;
+        c->dsink_cap = need_s;
+    }
+    if (need_p > c->dpart_cap) {
+        if (c->dpart) (void)hipFree(c->dpart);
+        c->dpart = nullptr;
+        c->dpart_cap = 0;
+        st = hip_check(c, hipMalloc((void**)&c->dpart, need_p * 8), "hipMalloc partials");
+        if (st != BH_OK) return st;
+        c->dpart_cap = need_p;
+    }
+    hipStream_t s = c->stream;
+    st = hip_check(c, hipMemcpyAsync(c->dsinks, c->iota_dev, (size_t)n * 4, hipMemcpyDeviceToDevice, s),
+                   "sinks");
+    if (st != BH_OK) return st;
+    double* out = c->dpart + (uint64_t)3 * S * n;
+    launch_direct_f64(s, kKernelTsne, c->pos4, c->id[c->cur], n, c->dsinks, n, 1.0, 0.0, c->where,
+                      c->dsinks + n, c->dpart, S, out);
+    launch_sum_f64(s, out + 2ull * n, n, c->tz);
+    launch_tsne_kl(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[c->ty_cur], c->ty_y[c->ty_cur],
+                   c->tz, c->kl_part, c->kl_out);
+    st = launch_status(c, "KL");
+    double h[2] = {0.0, 0.0};
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&h[0], c->kl_out, 8, hipMemcpyDeviceToHost, s), "kl");
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&h[1], c->tz, 8, hipMemcpyDeviceToHost, s), "z");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "KL");
+    if (st != BH_OK) return st;
+    if (kl) *kl = h[0];
+    if (z_exact) *z_exact = h[1];
+    return BH_OK;
 }
 
 bh_status bh_get_bounds(bh_ctx* c, double* bounds, double* root_side) {

@@ -560,7 +560,7 @@ __device__ __forceinline__ Moments combine(const Moments* ch, const bool* has, b
     return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ pos4,
+__global__ __launch_bounds__(kUpThreads) void k_up_chunk(const float4* __restrict__ pos4,
                                                      Node* __restrict__ nodes,
                                                      const uint4* __restrict__ child4,
                                                      float4* __restrict__ m2,
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ 
     __shared__ float s_xx[kUpChunk], s_xy[kUpChunk], s_yy[kUpChunk];
     __shared__ uint32_t s_maxl;
     __shared__ uint32_t s_ndef;
-    constexpr int kPer = kUpChunk / kBlock;
+    constexpr int kPer = kUpChunk / kUpThreads;
     const uint32_t total = min(prm->n_nodes, cap_nodes);
     const uint32_t c0 = blockIdx.x * kUpChunk;
     uint32_t* my_def = deferred + (size_t)blockIdx.x * kDeferStride;
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ 
     uint32_t lmax = 0;
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
         if (k < c1) {
             nd[r] = nodes[k];
             lmax = max(lmax, nd[r].meta & kMetaLevelMask);
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ 
     // leaves first, all at once: their particle loads overlap instead of waiting level by level
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
         if (k >= c1 || !(nd[r].meta & kMetaLeaf)) continue;
         const uint32_t s = k - c0;
         const Moments mo = leaf_moments(pos4, nd[r].first, nd[r].count, with_m2);
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ 
     for (int L = (int)s_maxl; L >= 0; --L) {
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t k = c0 + threadIdx.x + r * kBlock;
+            const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
             if (k >= c1 || (int)(nd[r].meta & kMetaLevelMask) != L) continue;
             const uint32_t s = k - c0;
             if (nd[r].meta & kMetaLeaf) {
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chunk(const float4* __restrict__ 
     }
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-        const uint32_t k = c0 + threadIdx.x + r * kBlock;
+        const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
         if (k >= c1) continue;
         const uint32_t s = k - c0;
         if ((nd[r].meta & kMetaLeaf) || nd[r].skip <= c1) {
@@ -681,6 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_up_top(Node* __restrict__ nodes,
                                                    uint32_t nchunks, uint32_t cap_nodes) {
     __shared__ uint32_t s_k[kTopMax];
     __shared__ uint8_t s_l[kTopMax];
+    __shared__ uint4 s_c4[kTopMax];  // child indices of each deferred node
     __shared__ uint32_t s_count, s_maxl;
     const uint32_t total = min(prm->n_nodes, cap_nodes);
     const uint32_t nch = min(nchunks, (total + kUpChunk - 1) / kUpChunk);
@@ -706,6 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_up_top(Node* __restrict__ nodes,
             if (at < kTopMax) {
                 s_k[at] = k;
                 s_l[at] = (uint8_t)l;
+                s_c4[at] = child4[k];
             }
         }
         __syncthreads();
@@ -720,17 +722,19 @@ __global__ __launch_bounds__(kBlock) void k_up_top(Node* __restrict__ nodes,
     for (int L = (int)s_maxl; L >= 0; --L) {
         for (uint32_t e = threadIdx.x; e < entries; e += kBlock) {
             uint32_t k;
+            uint4 cs;
             if (in_lds) {
                 if ((int)s_l[e] != L) continue;
                 k = s_k[e];
+                cs = s_c4[e];
             } else {
                 const uint32_t b = e / (kDeferStride - 1), slot = e % (kDeferStride - 1);
                 const uint32_t* lst = deferred + (size_t)b * kDeferStride;
                 if (slot >= lst[0]) continue;
                 k = lst[1 + slot];
                 if ((int)(nodes[k].meta & kMetaLevelMask) != L) continue;
+                cs = child4[k];
             }
-            const uint4 cs = child4[k];
             const uint32_t ci[4] = {cs.x, cs.y, cs.z, cs.w};
             Moments ch[4];
             bool has[4];
@@ -926,7 +930,7 @@ uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
                    float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes) {
     const uint32_t nch = up_chunks(cap_nodes);
-    k_up_chunk<<<nch, kBlock, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
+    k_up_chunk<<<nch, kUpThreads, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
                                       cap_nodes);
     k_up_top<<<1, kBlock, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, nch, cap_nodes);
 }

@@ -26,8 +26,9 @@ enum : uint32_t {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kUpChunk = 2048;                 // nodes per block in the upward pass
+constexpr int kUpThreads = 1024;               // threads per block in the upward pass
 constexpr int kDeferStride = kMaxLevels + 2;   // count + at most one deferred node per level
-constexpr int kTopMax = 6144;                  // deferred nodes the top pass keeps in LDS
+constexpr int kTopMax = 2560;                  // deferred nodes the top pass keeps in LDS
 
 enum : uint32_t {
     kTreeQuadTree = 0,     // adaptive, leaf capacity, QuadTree root and near-field rules

@@ -70,4 +70,17 @@ void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
                        double* out);
 
+void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
+                       float* y, float* m, uint32_t* id);
+void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
+                        const float* val, uint32_t n, const float* yx, const float* yy,
+                        const float* pyx, const float* pyy, const float* rep_x,
+                        const float* rep_y, const double* z, float exag, float lr, float mu,
+                        float* nyx, float* nyy, float* dx, float* dy);
+uint32_t tsne_kl_blocks(uint32_t n);
+void launch_tsne_kl(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col, const float* val,
+                    uint32_t n, const float* yx, const float* yy, const double* z,
+                    double* partials, double* out);
+void launch_sum_f64(hipStream_t s, const double* v, uint32_t nv, double* out);
+
 }  // namespace bh

@@ -1,0 +1,213 @@
+"""t-SNE on the Barnes-Hut solver: input affinities and the iteration driver.
+
+`TSNE` mirrors the reference driver `TSNE` (src/nBodyInstances/tsne.h:42-563): the same
+learning rate N/15 (tsne.h:90), momentum 0.5/0.8 switching at iteration 250 (tsne.h:186),
+exaggeration with the factor 4 of the gradient folded in (16 then 4, tsne.h:358), the cosine
+theta schedule (tsne.h:208-233), the derivative (4/Z) * BH repulsion + sparse attraction
+(tsne.h:272-301, 356-381), the momentum update (tsne.h:179-206) and the KL cost with log2
+(tsne.h:383-460). All of an iteration runs on the GPU inside the library (include/bh.h,
+bh_tsne_*); this class only holds parameters and the context.
+
+The reference loads its P matrix from a Matrix Market file prepared offline (tsne.h:94-98,
+Eigen loadMarket). Here the affinities come from `joint_probabilities`: exact k-nearest
+neighbours, a per-point Gaussian bandwidth found by bisection to the requested perplexity, and
+the symmetrisation P = (P_cond + P_cond^T) / (2N), returned in CSR sorted by (row, col) as
+setupEfficientPmatrix builds it (tsne.h:462-520).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .solver import BHContext, _ptr
+
+
+# ---------------------------------------------------------------------------------------------
+# input data and affinities
+# ---------------------------------------------------------------------------------------------
+def gaussian_mixture(n, dim=784, components=10, seed=4523, mean_std=2.0, sigma=1.0):
+    """n points of an equal-weight isotropic Gaussian mixture: component means ~ N(0, mean_std^2
+    I), points = mean + sigma N(0, I). Returns (X float32 [n, dim], labels uint8)."""
+    rng = np.random.default_rng(seed)
+    means = rng.normal(0.0, mean_std, size=(components, dim))
+    labels = rng.integers(0, components, size=n).astype(np.uint8)
+    X = means[labels] + rng.normal(0.0, sigma, size=(n, dim))
+    return X.astype(np.float32), labels
+
+
+def knn_exact(X, k, device="cpu", chunk=4096):
+    """Exact k nearest neighbours (self excluded) by squared Euclidean distance.
+    Returns (idx int64 [n, k], d2 float64 [n, k]) with neighbours in ascending distance."""
+    import torch
+
+    Xt = torch.as_tensor(np.asarray(X), dtype=torch.float32, device=device)
+    n = Xt.shape[0]
+    if k >= n:
+        raise ValueError(f"k={k} needs more than {n} points")
+    sq = (Xt * Xt).sum(1)
+    idx_out = torch.empty((n, k), dtype=torch.int64, device=device)
+    d_out = torch.empty((n, k), dtype=torch.float64, device=device)
+    ar = torch.arange(n, device=device)
+    for s in range(0, n, chunk):
+        e = min(s + chunk, n)
+        d = sq[s:e, None] + sq[None, :] - 2.0 * (Xt[s:e] @ Xt.T)
+        d[torch.arange(e - s, device=device), ar[s:e]] = float("inf")  # exclude self
+        _, idx = torch.topk(d, k, dim=1, largest=False, sorted=True)
+        idx_out[s:e] = idx
+        # recompute the selected distances in FP64 from the differences (no cancellation)
+        for a in range(s, e, 128):
+            b = min(a + 128, e)
+            diff = Xt[a:b, None, :].double() - Xt[idx[a - s:b - s]].double()
+            d_out[a:b] = (diff * diff).sum(-1)
+    order = torch.argsort(d_out, dim=1, stable=True)
+    return (torch.gather(idx_out, 1, order).cpu().numpy(),
+            torch.gather(d_out, 1, order).cpu().numpy())
+
+
+def conditional_probabilities(d2, perplexity, tol=1e-5, max_iter=200):
+    """Row-wise p_{j|i} = exp(-beta_i d2_ij) / sum_j, with beta_i bisected so that the entropy
+    (natural log) equals log(perplexity). d2: float64 [n, k]. Returns (P [n, k], beta [n])."""
+    d2 = np.asarray(d2, dtype=np.float64)
+    n = d2.shape[0]
+    target = math.log(perplexity)
+    dr = d2 - d2[:, :1]  # shift by the row minimum: same P, no underflow
+    beta = np.ones(n)
+    lo = np.full(n, -np.inf)
+    hi = np.full(n, np.inf)
+    done = np.zeros(n, bool)
+    P = None
+    for _ in range(max_iter):
+        E = np.exp(-beta[:, None] * dr)
+        S = E.sum(1)
+        P = E / S[:, None]
+        H = np.log(S) + beta * (dr * P).sum(1)
+        diff = H - target
+        done |= np.abs(diff) < tol
+        if done.all():
+            break
+        up = (diff > 0) & ~done  # entropy too high: sharpen
+        dn = (diff <= 0) & ~done
+        lo = np.where(up, beta, lo)
+        hi = np.where(dn, beta, hi)
+        beta = np.where(up, np.where(np.isinf(hi), beta * 2.0, 0.5 * (beta + hi)), beta)
+        beta = np.where(dn, np.where(np.isinf(lo), beta * 0.5, 0.5 * (beta + lo)), beta)
+    return P, beta
+
+
+def joint_probabilities(X, perplexity=30.0, k=90, device="cpu"):
+    """Symmetrised sparse affinities from exact kNN. Returns CSR (row_ptr uint32 [n+1],
+    col uint32 [nnz], val float32 [nnz]) sorted by (row, col), summing to 1."""
+    idx, d2 = knn_exact(X, k, device=device)
+    Pc, _ = conditional_probabilities(d2, perplexity)
+    return symmetrize(idx, Pc)
+
+
+def symmetrize(idx, Pc):
+    """P = (P_cond + P_cond^T) / (2 n) from neighbour lists, as CSR sorted by (row, col)."""
+    n, k = idx.shape
+    rows = np.repeat(np.arange(n, dtype=np.int64), k)
+    cols = idx.reshape(-1).astype(np.int64)
+    vals = Pc.reshape(-1)
+    r = np.concatenate([rows, cols])
+    c = np.concatenate([cols, rows])
+    v = np.concatenate([vals, vals]) / (2.0 * n)
+    key = r * n + c
+    order = np.argsort(key, kind="stable")
+    key, v = key[order], v[order]
+    uniq, start = np.unique(key, return_index=True)
+    vsum = np.add.reduceat(v, start)
+    ur, uc = uniq // n, uniq % n
+    row_ptr = np.zeros(n + 1, np.uint32)
+    np.cumsum(np.bincount(ur, minlength=n), out=row_ptr[1:])
+    return row_ptr, uc.astype(np.uint32), vsum.astype(np.float32)
+
+
+def initial_embedding(n, seed=4523, std=1e-2):
+    """2-D start points ~ N(0, std^2 I) (the reference draws N(0, 0.01^2), tsne.h:530-548)."""
+    rng = np.random.default_rng(seed + 1)
+    Y = rng.normal(0.0, std, size=(n, 2))
+    return Y[:, 0].astype(np.float32), Y[:, 1].astype(np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+# driver
+# ---------------------------------------------------------------------------------------------
+class TSNE:
+    """Reference t-SNE driver over the GPU Barnes-Hut solver (tsne.h:42-563).
+
+    `timeStep()` is one updateDerivative + updatePoints; `costFunction()` the KL divergence
+    (FP64, exact Z); `embeddedPoints` the current positions (caller order, shape [N, 2]).
+    """
+
+    def __init__(self, row_ptr, col, val, y0, min_theta=0.5, max_theta=0.5,
+                 exaggeration=(16.0, 4.0), momentum=(0.5, 0.8), switch=250, max_children=16,
+                 learn_rate=None):
+        row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
+        col = np.ascontiguousarray(col, np.uint32)
+        val = np.ascontiguousarray(val, np.float32)
+        yx = np.ascontiguousarray(y0[0], np.float32)
+        yy = np.ascontiguousarray(y0[1], np.float32)
+        n = yx.size
+        if row_ptr.size != n + 1:
+            raise ValueError("row_ptr must hold n + 1 entries")
+        self.n = n
+        self.ctx = BHContext(n, kernel="tsne", max_children=max_children, theta=max_theta)
+        L = _lib.load()
+        p = _lib.TsneParams()
+        self.ctx._c(L.bh_tsne_params_default(C.byref(p), n, float(max_theta)))
+        p.theta_min, p.theta_max = float(min_theta), float(max_theta)
+        p.exag_early, p.exag_late = map(float, exaggeration)
+        p.momentum_early, p.momentum_late = map(float, momentum)
+        p.momentum_switch = p.exag_switch = int(switch)
+        if learn_rate is not None:
+            p.learning_rate = float(learn_rate)
+        self.params = p
+        self.iteration_counter = 0
+        self.ctx._c(L.bh_tsne_set_p(self.ctx.handle, n, _ptr(row_ptr), _ptr(col), _ptr(val),
+                                    int(col.size)))
+        self.ctx._c(L.bh_tsne_set_embedding(self.ctx.handle, _ptr(yx), _ptr(yy)))
+
+    @property
+    def learnRate(self):
+        return self.params.learning_rate
+
+    def theta(self, iteration=None):
+        it = self.iteration_counter if iteration is None else iteration
+        return float(_lib.load().bh_tsne_theta(C.byref(self.params), int(it)))
+
+    def timeStep(self, return_derivative=False):
+        """One iteration. Returns (dx, dy, Z) when return_derivative, else Z."""
+        L = _lib.load()
+        z = C.c_double(0.0)
+        dx = dy = None
+        if return_derivative:
+            dx, dy = np.empty(self.n, np.float32), np.empty(self.n, np.float32)
+        self.ctx._c(L.bh_tsne_step(self.ctx.handle, C.byref(self.params), self.iteration_counter,
+                                   _ptr(dx), _ptr(dy), C.byref(z)))
+        self.iteration_counter += 1
+        return (dx, dy, z.value) if return_derivative else z.value
+
+    def run(self, iterations):
+        """`iterations` steps without host synchronisation in between."""
+        L = _lib.load()
+        self.ctx._c(L.bh_tsne_run(self.ctx.handle, C.byref(self.params), self.iteration_counter,
+                                  int(iterations)))
+        self.iteration_counter += int(iterations)
+
+    def costFunction(self):
+        """(KL divergence, exact Z), both FP64."""
+        kl, z = C.c_double(0.0), C.c_double(0.0)
+        self.ctx._c(_lib.load().bh_tsne_kl(self.ctx.handle, C.byref(kl), C.byref(z)))
+        return kl.value, z.value
+
+    @property
+    def embeddedPoints(self):
+        yx, yy = np.empty(self.n, np.float32), np.empty(self.n, np.float32)
+        self.ctx._c(_lib.load().bh_tsne_get_embedding(self.ctx.handle, _ptr(yx), _ptr(yy)))
+        return np.stack([yx, yy], 1)
+
+    def close(self):
+        self.ctx.close()

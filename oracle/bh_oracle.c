@@ -1023,6 +1023,37 @@ double orc_tsne_kl(const double* x, const double* y, int64_t n, const uint32_t* 
     return cost;
 }
 
+/* One iteration at iteration counter t: updateDerivative (tsne.h:272-301, 356-381) with the
+ * reference BH solver (QuadTree, leaf capacity 16, t-SNE kernels, nBodySolverBH.h:33-116), then
+ * updatePoints (tsne.h:179-206): y_next = y + lr d + mu (y - y_prev). Writes d (may be NULL)
+ * and y_next; returns the Barnes-Hut Z. */
+double orc_tsne_step(const double* x, const double* y, const double* px, const double* py,
+                     int64_t n, const uint32_t* row_ptr, const uint32_t* col, const double* val,
+                     double theta, double exag, double lr, double mu, double* dx, double* dy,
+                     double* nx, double* ny, int nthreads) {
+    double* rx = (double*)calloc((size_t)n, sizeof(double));
+    double* ry = (double*)calloc((size_t)n, sizeof(double));
+    orc_quadtree* qt = orc_quadtree_build(x, y, n, 16);
+    const double z = orc_quadtree_forces(qt, x, y, 1, 1.0, 1.0, 0.0, theta, rx, ry, NULL,
+                                         nthreads);
+    orc_quadtree_free(qt);
+    const double s = 4.0 / z;
+    for (int64_t i = 0; i < n; ++i) {
+        rx[i] *= s;
+        ry[i] *= s;
+    }
+    orc_tsne_attraction(x, y, n, row_ptr, col, val, exag, rx, ry);
+    for (int64_t i = 0; i < n; ++i) {
+        if (dx) dx[i] = rx[i];
+        if (dy) dy[i] = ry[i];
+        nx[i] = x[i] + lr * rx[i] + mu * (x[i] - px[i]);
+        ny[i] = y[i] + lr * ry[i] + mu * (y[i] - py[i]);
+    }
+    free(rx);
+    free(ry);
+    return z;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* MLP (din -> h -> h -> dout, SiLU), FP64 reference for the learned far-field correction       */
 /* ------------------------------------------------------------------------------------------ */

@@ -127,6 +127,12 @@ void orc_tsne_attraction(const double* x, const double* y, int64_t n, const uint
 double orc_tsne_kl(const double* x, const double* y, int64_t n, const uint32_t* row_ptr,
                    const uint32_t* col, const double* val, int nthreads);
 double orc_tsne_z(const double* x, const double* y, int64_t n, int nthreads);
+/* one iteration: derivative (4/Z) BH repulsion (QuadTree, leaf capacity 16) + attraction, then
+ * y_next = y + lr d + mu (y - y_prev); returns the Barnes-Hut Z */
+double orc_tsne_step(const double* x, const double* y, const double* px, const double* py,
+                     int64_t n, const uint32_t* row_ptr, const uint32_t* col, const double* val,
+                     double theta, double exag, double lr, double mu, double* dx, double* dy,
+                     double* nx, double* ny, int nthreads);
 
 /* ---- MLP (10 -> H -> H -> 2, SiLU), FP64 ---- */
 void orc_mlp_forward(const double* X, int64_t B, int din, int h, int dout, const double* W1,

@@ -68,6 +68,8 @@ def lib():
             "orc_tsne_attraction": (None, [_vp, _vp, I64, _vp, _vp, _vp, D, _vp, _vp]),
             "orc_tsne_kl": (D, [_vp, _vp, I64, _vp, _vp, _vp, I32]),
             "orc_tsne_z": (D, [_vp, _vp, I64, I32]),
+            "orc_tsne_step": (D, [_vp, _vp, _vp, _vp, I64, _vp, _vp, _vp, D, D, D, D, _vp, _vp,
+                                  _vp, _vp, I32]),
             "orc_mlp_forward": (None, [_vp, I64, I32, I32, I32] + [_vp] * 7),
             "orc_mlp_loss_grad": (D, [_vp, _vp, I64, I32, I32, I32] + [_vp] * 12),
             "orc_adam": (None, [_vp, _vp, _vp, _vp, I64, D, D, D, D, I64]),
@@ -308,6 +310,19 @@ def tsne_kl(x, y, row_ptr, col, val, nthreads=8):
     cl = np.ascontiguousarray(col, np.uint32)
     vl = _f64(val)
     return float(lib().orc_tsne_kl(_p(x), _p(y), x.size, _p(rp), _p(cl), _p(vl), int(nthreads)))
+
+
+def tsne_step(x, y, px, py, row_ptr, col, val, theta, exag, lr, mu, nthreads=8):
+    """One reference t-SNE iteration -> (dx, dy, x_next, y_next, Z)."""
+    x, y, px, py = _f64(x), _f64(y), _f64(px), _f64(py)
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    cl = np.ascontiguousarray(col, np.uint32)
+    vl = _f64(val)
+    dx, dy, nx, ny = (np.empty(x.size) for _ in range(4))
+    z = lib().orc_tsne_step(_p(x), _p(y), _p(px), _p(py), x.size, _p(rp), _p(cl), _p(vl),
+                            float(theta), float(exag), float(lr), float(mu), _p(dx), _p(dy),
+                            _p(nx), _p(ny), int(nthreads))
+    return dx, dy, nx, ny, float(z)
 
 
 def mlp_forward(X, params):

@@ -1,0 +1,148 @@
+"""t-SNE iteration on the GPU (include/bh.h, bh_tsne_*) against the oracle's restatement of the
+reference driver (tsne.h:179-233, 272-301, 356-460).
+
+Tolerances: the GPU computes in FP32 with the Barnes-Hut repulsion on the same tree topology as
+the reference QuadTree, the oracle in FP64. Per point, errors are measured against the scale
+|4/Z rep_i| + |att_i| of the two terms that make up the derivative (their sum can cancel), with
+the FP32 bar of the force tests: median 1e-5, p99 1e-4.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from masterthesislucas_amd import _lib, tsne
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+MEDIAN_TOL, P99_TOL = 1e-5, 1e-4
+
+
+def _problem(n, dim=40, seed=11, perplexity=20.0, k=60):
+    X, _ = tsne.gaussian_mixture(n, dim=dim, seed=seed)
+    rp, col, val = tsne.joint_probabilities(X, perplexity=perplexity, k=k)
+    yx, yy = tsne.initial_embedding(n, seed=seed, std=1.0)
+    return rp, col, val, yx, yy
+
+
+def _oracle_terms(x, y, rp, col, val, theta, exag):
+    qt = orc.QuadTree(x, y, 16)
+    rx, ry, z = qt.forces(orc.KERNEL_TSNE, 1.0, 1.0, 0.0, theta, nthreads=8)[:3]
+    ax, ay = orc.tsne_attraction(x, y, rp, col, val, exag)
+    return 4.0 / z * rx, 4.0 / z * ry, ax, ay, z
+
+
+def _scaled_err(got, want, scale):
+    e = np.abs(got - want) / scale
+    return float(np.median(e)), float(np.quantile(e, 0.99))
+
+
+@pytest.mark.parametrize("n,theta", [(3000, 0.5), (6000, 0.8)])
+def test_tsne_step_matches_oracle(gpu, n, theta):
+    rp, col, val, yx, yy = _problem(n)
+    t = tsne.TSNE(rp, col, val, (yx, yy), min_theta=theta, max_theta=theta)
+    try:
+        x0, y0 = yx.astype(np.float64), yy.astype(np.float64)
+        # iteration 0: previous positions equal the current ones (tsne.h:553-555)
+        dx, dy, z = t.timeStep(return_derivative=True)
+        odx, ody, onx, ony, oz = orc.tsne_step(x0, y0, x0, y0, rp, col, val, theta, 16.0,
+                                               n / 15.0, 0.5)
+        rx, ry, ax, ay, _ = _oracle_terms(x0, y0, rp, col, val, theta, 16.0)
+        scale = np.hypot(np.abs(rx) + np.abs(ax), np.abs(ry) + np.abs(ay))
+        med, p99 = _scaled_err(np.hypot(dx - odx, dy - ody), 0.0, scale)
+        assert med < MEDIAN_TOL and p99 < P99_TOL, (med, p99)
+        assert abs(z - oz) < 1e-6 * oz
+        y1 = t.embeddedPoints.astype(np.float64)
+        step = (n / 15.0) * scale + 1e-30
+        med, p99 = _scaled_err(np.hypot(y1[:, 0] - onx, y1[:, 1] - ony), 0.0,
+                               step + np.hypot(x0, y0) * 1e-6)
+        assert med < MEDIAN_TOL and p99 < P99_TOL, (med, p99)
+
+        # iteration 1 from the GPU's own state: exercises the momentum term mu (y - y_prev)
+        dx, dy, z = t.timeStep(return_derivative=True)
+        x1, y1_ = y1[:, 0], y1[:, 1]
+        odx, ody, onx, ony, oz = orc.tsne_step(x1, y1_, x0, y0, rp, col, val, theta, 16.0,
+                                               n / 15.0, 0.5)
+        rx, ry, ax, ay, _ = _oracle_terms(x1, y1_, rp, col, val, theta, 16.0)
+        scale = np.hypot(np.abs(rx) + np.abs(ax), np.abs(ry) + np.abs(ay))
+        med, p99 = _scaled_err(np.hypot(dx - odx, dy - ody), 0.0, scale)
+        assert med < MEDIAN_TOL and p99 < P99_TOL, (med, p99)
+        y2 = t.embeddedPoints.astype(np.float64)
+        mom = 0.5 * np.hypot(x1 - x0, y1_ - y0)
+        med, p99 = _scaled_err(np.hypot(y2[:, 0] - onx, y2[:, 1] - ony), 0.0,
+                               (n / 15.0) * scale + mom + np.hypot(x1, y1_) * 1e-6)
+        assert med < MEDIAN_TOL and p99 < P99_TOL, (med, p99)
+    finally:
+        t.close()
+
+
+def test_tsne_late_phase_constants(gpu):
+    # iteration >= 250: exaggeration 4, momentum 0.8 (tsne.h:186, 358)
+    n, theta = 3000, 0.5
+    rp, col, val, yx, yy = _problem(n, seed=12)
+    t = tsne.TSNE(rp, col, val, (yx, yy), min_theta=theta, max_theta=theta)
+    try:
+        t.iteration_counter = 300
+        dx, dy, _ = t.timeStep(return_derivative=True)
+        x0, y0 = yx.astype(np.float64), yy.astype(np.float64)
+        odx, ody, onx, ony, _ = orc.tsne_step(x0, y0, x0, y0, rp, col, val, theta, 4.0,
+                                              n / 15.0, 0.8)
+        rx, ry, ax, ay, _ = _oracle_terms(x0, y0, rp, col, val, theta, 4.0)
+        scale = np.hypot(np.abs(rx) + np.abs(ax), np.abs(ry) + np.abs(ay))
+        med, p99 = _scaled_err(np.hypot(dx - odx, dy - ody), 0.0, scale)
+        assert med < MEDIAN_TOL and p99 < P99_TOL, (med, p99)
+    finally:
+        t.close()
+
+
+def test_tsne_kl_matches_oracle(gpu):
+    n = 4000
+    rp, col, val, yx, yy = _problem(n, seed=13)
+    t = tsne.TSNE(rp, col, val, (yx, yy))
+    try:
+        kl, z = t.costFunction()
+        x, y = yx.astype(np.float64), yy.astype(np.float64)
+        oz = orc.tsne_z(x, y)
+        okl = orc.tsne_kl(x, y, rp, col, val.astype(np.float64))
+        assert abs(z - oz) <= 1e-11 * oz
+        assert abs(kl - okl) <= 1e-9 * abs(okl)
+        assert kl > 0
+    finally:
+        t.close()
+
+
+def test_tsne_run_descends_and_tracks_oracle(gpu):
+    # 40 iterations, early exaggeration: the KL divergence falls; the GPU trajectory and the
+    # oracle's stay close (a sanity bound only: FP32 and FP64 trajectories drift apart)
+    n, theta = 2000, 0.5
+    rp, col, val, yx, yy = _problem(n, seed=14)
+    t = tsne.TSNE(rp, col, val, (yx, yy), min_theta=theta, max_theta=theta)
+    try:
+        kl0, _ = t.costFunction()
+        t.run(40)
+        kl1, _ = t.costFunction()
+        assert kl1 < kl0
+        x, y = yx.astype(np.float64), yy.astype(np.float64)
+        px, py = x.copy(), y.copy()
+        for it in range(40):
+            _, _, nx, ny, _ = orc.tsne_step(x, y, px, py, rp, col, val, theta, 16.0, n / 15.0,
+                                            0.5)
+            px, py, x, y = x, y, nx, ny
+        okl = orc.tsne_kl(x, y, rp, col, val.astype(np.float64))
+        assert abs(kl1 - okl) < 1e-2 * okl, (kl1, okl)
+    finally:
+        t.close()
+
+
+def test_tsne_rejects_bad_input(gpu):
+    n = 100
+    rp, col, val, yx, yy = _problem(n, seed=15, k=10, perplexity=5.0)
+    bad = col.copy()
+    bad[3] = n
+    with pytest.raises(_lib.BHError):
+        tsne.TSNE(rp, bad, val, (yx, yy))
+    rp2 = rp.copy()
+    rp2[-1] += 1
+    with pytest.raises(_lib.BHError):
+        tsne.TSNE(rp2, col, val, (yx, yy))
