@@ -1,0 +1,458 @@
+"""Pins of the CPU oracle (oracle/bh_oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something that does not restate its own formula:
+the compiled reference (oracle/_ref, built from the reference's own QuadTree/NBodySolverBH
+sources when /root/reference is present), brute force, closed forms, invariants and finite
+differences. Mistakes these catch: a swapped bit plane or axis in the Morton interleave, a
+wrong boundary side in the quantiser, unstable tie order, miscounted or misplaced tree slots,
+a wrong sign, a dropped factor 4 or Z, w instead of w^2 in the t-SNE kernel, a wrong leapfrog
+ordering, and gradient slips in the MLP.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import oracle_forces, oracle_tree, rel_err
+
+
+def _interleave_brute(cx, cy, bits):
+    code = 0
+    for b in range(bits):
+        code |= ((cx >> b) & 1) << (2 * b)
+        code |= ((cy >> b) & 1) << (2 * b + 1)
+    return code
+
+
+# ---------------------------------------------------------------------------------------------
+# Morton codes (nBodySolverFMM_SYM_MORTON.h:538-561)
+# ---------------------------------------------------------------------------------------------
+def test_morton32_exhaustive_axis_interleave():
+    # dyadic positions t = k / 65536 quantise exactly to k: the code is the bit interleave of
+    # (kx, ky) with y on the odd bits, for every 16-bit value on each axis
+    k = np.arange(65536, dtype=np.float64)
+    codes_x = O.morton32(k / 65536.0, np.zeros_like(k), 0.0, 0.0, 1.0)
+    codes_y = O.morton32(np.zeros_like(k), k / 65536.0, 0.0, 0.0, 1.0)
+    ki = k.astype(np.int64)
+    exp_x = np.zeros(65536, np.int64)
+    for b in range(16):
+        exp_x |= ((ki >> b) & 1) << (2 * b)
+    assert np.array_equal(codes_x.astype(np.int64), exp_x)
+    assert np.array_equal(codes_y.astype(np.int64), exp_x << 1)
+
+
+def test_morton32_random_against_brute_force():
+    rng = np.random.default_rng(0)
+    x, y = rng.uniform(-3, 5, 500), rng.uniform(-1, 2, 500)
+    minx, miny = x.min(), y.min()
+    L = max(x.max() - minx, y.max() - miny)
+    codes = O.morton32(x, y, minx, miny, L)
+    for i in range(500):
+        cx = min(max(int(math.floor((x[i] - minx) / L * 65536.0)), 0), 65535)
+        cy = min(max(int(math.floor((y[i] - miny) / L * 65536.0)), 0), 65535)
+        assert int(codes[i]) == _interleave_brute(cx, cy, 16)
+
+
+def test_morton_boundaries():
+    # p = min -> 0; p = max along the longest axis -> clamped to 65535; just below max -> 65535
+    c = O.morton32([0.0, 2.0, 2.0 - 1e-12, 1.0], [0.0, 2.0, 0.0, 1.0], 0.0, 0.0, 2.0)
+    assert c[0] == 0
+    assert c[1] == 0xFFFFFFFF
+    assert c[2] == _interleave_brute(65535, 0, 16)
+    assert c[3] == _interleave_brute(32768, 32768, 16)
+    c64 = O.morton64([0.0, 2.0, 1.0], [0.0, 2.0, 1.0], 0.0, 0.0, 2.0, bits=32)
+    assert c64[0] == 0 and c64[1] == (1 << 64) - 1
+    assert int(c64[2]) == _interleave_brute(1 << 31, 1 << 31, 32)
+
+
+def test_morton64_matches_quantize_and_brute_force():
+    rng = np.random.default_rng(1)
+    x, y = rng.normal(size=300), rng.normal(size=300)
+    minx, miny = x.min(), y.min()
+    side = O.quadtree_side(minx, miny, x.max(), y.max())
+    for bits in (8, 21, 32):
+        c = O.morton64(x, y, minx, miny, side, bits)
+        qx = O.quantize_axis(x, minx, side, bits)
+        qy = O.quantize_axis(y, miny, side, bits)
+        for i in range(0, 300, 7):
+            assert int(c[i]) == _interleave_brute(int(qx[i]), int(qy[i]), bits)
+        # 16-bit prefix of the 32-bit code is the 32-bit code of the same quantiser
+        if bits == 32:
+            c16 = O.morton64(x, y, minx, miny, side, 16)
+            assert np.array_equal(c >> np.uint64(32), c16)
+
+
+def test_quadtree_side_closed_form():
+    # quadtree.h:45-47: side = ((min + largest) + 1e-4) - min, evaluated left to right
+    assert O.quadtree_side(0.0, 0.0, 1.0, 0.5) == (1.0 + 0.0001) - 0.0
+    assert O.quadtree_side(-2.0, 1.0, 3.0, 2.0) == ((-2.0 + 5.0) + 0.0001) + 2.0
+
+
+# ---------------------------------------------------------------------------------------------
+# sort, depth, fixed tree
+# ---------------------------------------------------------------------------------------------
+def test_stable_sort_against_numpy():
+    rng = np.random.default_rng(2)
+    for n in (0, 1, 5, 1000, 70001):
+        keys = rng.integers(0, 50, size=n).astype(np.uint64) << np.uint64(40)
+        keys |= rng.integers(0, 3, size=n).astype(np.uint64)  # many ties
+        perm = O.stable_sort(keys)
+        assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def test_get_depth_table():
+    # D = ceil(ln((4/pi) (N/16)) / ln 4) with integer N/16 (nBodySolverFMM_SYM_MORTON.h:106-115)
+    assert O.get_depth(16, 65536) == 7
+    assert O.get_depth(16, 70000) == 7
+    assert O.get_depth(16, 1 << 20) == 9
+    assert O.get_depth(16, 1 << 22) == 10
+    assert O.get_depth(16, 15) == 0
+    for n in (17, 100, 5000, 123457):
+        q = n // 16
+        assert O.get_depth(16, n) == max(0, math.ceil(math.log(q / (math.pi / 4)) / math.log(4)))
+
+
+def test_fixed_tree_brute_force_recount():
+    rng = np.random.default_rng(3)
+    n, D = 700, 4
+    x, y = rng.uniform(-1, 1, n), rng.uniform(0, 3, n)
+    x[:50] = x[0]  # a crowded cell
+    y[:50] = y[0]
+    minx, miny, maxx, maxy = x.min(), y.min(), x.max(), y.max()
+    L = max(maxx - minx, maxy - miny)
+    codes = O.morton32(x, y, minx, miny, L)
+    perm = O.stable_sort(codes.astype(np.uint64))
+    xs, ys, cs = x[perm], y[perm], codes[perm].astype(np.int64)
+    t = O.fixed_tree(xs, ys, D, minx, miny, maxx, maxy)
+    level_index = [((1 << (2 * l)) - 1) // 3 for l in range(D + 2)]
+    for l in range(D + 1):
+        cell = cs >> (2 * (16 - l))  # level-l cell = top 2l bits of the code
+        for i in range(1 << (2 * l)):
+            s = level_index[l] + i
+            members = np.nonzero(cell == i)[0]
+            assert t["count"][s] == members.size
+            if members.size == 0:
+                continue
+            assert t["first_particle"][s] == members.min()
+            assert np.array_equal(members, np.arange(members.min(), members.min() + members.size))
+            assert t["m0"][s] == members.size
+            assert abs(t["com_x"][s] - xs[members].mean()) < 1e-12
+            assert abs(t["com_y"][s] - ys[members].mean()) < 1e-12
+            size = L / (1 << l)
+            assert t["bb_len"][s] == pytest.approx(size, rel=1e-14)
+            if l < D:
+                assert t["first_child"][s] == level_index[l + 1] + 4 * i
+            else:  # leaf second moment = direct sum about the leaf COM
+                rx, ry = xs[members] - xs[members].mean(), ys[members] - ys[members].mean()
+                assert t["m2"][s] == pytest.approx([(rx * rx).sum(), (rx * ry).sum(),
+                                                     (ry * ry).sum()], abs=1e-12)
+    assert t["count"][0] == n
+
+
+def test_fixed_preorder_matches_slot_tree():
+    rng = np.random.default_rng(4)
+    n, D = 400, 3
+    x, y = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    minx, miny = x.min(), y.min()
+    L = max(x.max() - minx, y.max() - miny)
+    codes = O.morton32(x, y, minx, miny, L)
+    perm = O.stable_sort(codes.astype(np.uint64))
+    cs = codes[perm]
+    pre = O.fixed_preorder(cs, D)
+    slots = O.fixed_tree(x[perm], y[perm], D, minx, miny, x.max(), y.max())
+    # the pre-order tree holds exactly the non-empty slots, with their counts and first indices
+    nonempty = np.nonzero(slots["count"])[0]
+    assert pre["first"].size == nonempty.size
+    assert sorted(zip(pre["first"], pre["count"], pre["level"])) == sorted(
+        (slots["first_particle"][s], slots["count"][s],
+         next(l for l in range(D + 1) if s < ((1 << (2 * (l + 1))) - 1) // 3)) for s in nonempty)
+    # skip pointers: pre-order successor after the subtree
+    for k in range(pre["first"].size):
+        e = pre["skip"][k]
+        assert e == pre["first"].size or pre["level"][e] <= pre["level"][k]
+
+
+# ---------------------------------------------------------------------------------------------
+# adaptive tree and the QuadTree restatement against the compiled reference
+# ---------------------------------------------------------------------------------------------
+def _ref_or_skip():
+    if not O.ref_available():
+        pytest.skip("compiled reference (oracle/_ref) not built: /root/reference absent")
+
+
+@pytest.mark.parametrize("n,seed", [(3000, 5), (12000, 6)])
+def test_quadtree_restatement_bitexact_against_reference(n, seed):
+    _ref_or_skip()
+    rng = np.random.default_rng(seed)
+    r = 0.1 * np.sqrt(rng.uniform(size=n) / (1 - rng.uniform(size=n) * 0.999))
+    a = rng.uniform(0, 2 * np.pi, n)
+    x = (r * np.cos(a)).astype(np.float32).astype(np.float64)
+    y = (r * np.sin(a)).astype(np.float32).astype(np.float64)
+    ours = O.QuadTree(x, y, 16).export()
+    ref = O.ref_quadtree(x, y, 16)
+    for k in ("depth", "count", "nchild", "mass", "comx", "comy", "lowx", "lowy", "highx",
+              "highy"):
+        assert np.array_equal(ours[k], ref[k]), k
+    for kernel in (O.KERNEL_GRAVITY, O.KERNEL_TSNE):
+        # gravity: particle mass 1/n; the reference t-SNE kernels take unit masses
+        eps2, m = (1e-6, 1.0 / n) if kernel == O.KERNEL_GRAVITY else (0.0, 1.0)
+        qx, qy, qt, _ = O.QuadTree(x, y, 16).forces(kernel, 1.0, m, eps2, 0.5, nthreads=1)
+        rx, ry, rt, _, _ = O.ref_bh_solve(x, y, kernel, 1.0, m, eps2, 0.5)
+        assert np.array_equal(qx, rx) and np.array_equal(qy, ry)
+        assert qt == rt
+
+
+def test_adaptive_tree_topology_equals_quadtree_restatement():
+    t = None
+    for seed, n in ((7, 2500), (8, 9000)):
+        rng = np.random.default_rng(seed)
+        x = rng.normal(0, 0.3, n).astype(np.float32)
+        y = rng.normal(0, 0.1, n).astype(np.float32)
+        t = oracle_tree(x, y)
+        qt = O.QuadTree(t["x"], t["y"], 16)
+        q = qt.export()
+        nf, occ = qt.occupants()
+        # the QuadTree lists children HH, HL, LH, LL; the code-ordered tree LL, HL, LH, HH, so
+        # nodes are matched by (level, occupant set) rather than by pre-order position
+        def keys(levels, starts, counts, nch, members):
+            out = []
+            for k in range(levels.size):
+                s = members[starts[k]:starts[k] + counts[k]].astype(np.int64)
+                out.append((int(levels[k]), int(counts[k]), int(nch[k]), int(s.min()),
+                            int(s.max()), int(s.sum()), int((s * s).sum())))
+            return sorted(out)
+        tr = t["tree"]
+        assert keys(tr["level"], tr["first"], tr["count"], tr["nchild"], t["perm"]) == \
+            keys(q["depth"], nf, q["count"], q["nchild"], occ)
+
+
+# ---------------------------------------------------------------------------------------------
+# traversal and direct sum
+# ---------------------------------------------------------------------------------------------
+def test_direct_two_body_closed_form():
+    G, eps2 = 2.0, 0.01
+    ax, ay, tot = O.direct([0.0, 3.0], [0.0, 4.0], [1.5, 0.5], O.KERNEL_GRAVITY, G, eps2)
+    s = (25.0 + eps2) ** 1.5
+    assert ax[0] == pytest.approx(G * 0.5 * 3.0 / s, rel=1e-15)
+    assert ay[0] == pytest.approx(G * 0.5 * 4.0 / s, rel=1e-15)
+    assert ax[1] == pytest.approx(-G * 1.5 * 3.0 / s, rel=1e-15)
+    assert tot[0] == pytest.approx(-G * 0.5 / math.sqrt(25.0 + eps2), rel=1e-15)
+    # t-SNE kernel: w = 1/(1+d^2), total = m w, derivative = m w^2 (p - q)
+    ax, ay, tot = O.direct([0.0, 3.0], [0.0, 4.0], None, O.KERNEL_TSNE)
+    assert tot[0] == pytest.approx(1.0 / 26.0, rel=1e-15)
+    assert ax[0] == pytest.approx(-3.0 / 26.0 ** 2, rel=1e-15)
+    assert ay[1] == pytest.approx(4.0 / 26.0 ** 2, rel=1e-15)
+
+
+def test_direct_invariants():
+    rng = np.random.default_rng(9)
+    n = 500
+    x, y, m = rng.normal(size=n), rng.normal(size=n), rng.uniform(0.5, 2, n)
+    ax, ay, _ = O.direct(x, y, m, O.KERNEL_GRAVITY, 1.0, 1e-4)
+    # Newton's third law: total momentum change vanishes
+    assert abs((m * ax).sum()) < 1e-10 * np.abs(m * ax).sum()
+    assert abs((m * ay).sum()) < 1e-10 * np.abs(m * ay).sum()
+    # ring of equal masses: zero net force at the centre
+    k = np.arange(12)
+    xr = np.concatenate([[0.0], np.cos(2 * np.pi * k / 12)])
+    yr = np.concatenate([[0.0], np.sin(2 * np.pi * k / 12)])
+    ax, ay, _ = O.direct(xr, yr, None, O.KERNEL_GRAVITY, 1.0, 0.0, sinks=[0])
+    assert abs(ax[0]) < 1e-14 and abs(ay[0]) < 1e-14
+
+
+@pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
+def test_theta_zero_traversal_equals_direct(mode):
+    rng = np.random.default_rng(10)
+    x = rng.normal(0, 1, 3000).astype(np.float32)
+    y = rng.normal(0, 1, 3000).astype(np.float32)
+    t = oracle_tree(x, y, mode=mode)
+    # node moments: root mass N, root centre of mass = mean
+    assert t["mom"]["mass"][0] == 3000
+    assert abs(t["mom"]["cx"][0] - t["x"].mean()) < 1e-12
+    for kernel in (O.KERNEL_GRAVITY, O.KERNEL_TSNE):
+        ax, ay, tot, cnt = oracle_forces(t, kernel, 0.0, mode=mode, eps=1e-3)
+        dx, dy, dt = O.direct(t["x"], t["y"], None, kernel, 1.0, 1e-6)
+        assert rel_err(ax, ay, dx, dy).max() < 1e-12
+        assert np.allclose(tot, dt, rtol=1e-12)
+        assert cnt[0] == 0  # nothing accepted
+
+
+def test_far_cluster_is_one_monopole():
+    # a compact cluster far from the sink is accepted as a single particle-node interaction
+    # (theta 0.7 accepts the level-1 cell of side ~50 at distance ~100; at 0.5 that cell, with
+    # one child, would be opened into particle-particle over all its occupants)
+    rng = np.random.default_rng(11)
+    cx, cy = rng.normal(0, 0.01, 40), rng.normal(0, 0.01, 40)  # > 16: the root splits
+    x = np.concatenate([[100.0], cx]).astype(np.float32)
+    y = np.concatenate([[0.0], cy]).astype(np.float32)
+    t = oracle_tree(x, y)
+    ax, ay, tot, cnt = O.bh_traverse(t["xs"], t["ys"], t["ms"], t["tree"], t["mom"], t["side"],
+                                     O.SEM_QUADTREE, O.KERNEL_GRAVITY, theta=0.7,
+                                     sinks=[int(np.nonzero(t["perm"] == 0)[0][0])], pp=t["pp"])
+    assert cnt[0] == 1 and cnt[1] == 0
+    X, Y = t["x"][1:].mean(), t["y"][1:].mean()
+    d = np.hypot(X - t["x"][0], Y - t["y"][0])
+    assert ax[0] == pytest.approx(40 * (X - t["x"][0]) / d ** 3, rel=1e-12)
+    assert ay[0] == pytest.approx(40 * (Y - t["y"][0]) / d ** 3, rel=1e-12)
+
+
+def test_traversal_matches_quadtree_restatement():
+    rng = np.random.default_rng(12)
+    x = rng.normal(0, 1, 5000).astype(np.float32)
+    y = rng.normal(0, 0.5, 5000).astype(np.float32)
+    t = oracle_tree(x, y)
+    for kernel in (O.KERNEL_GRAVITY, O.KERNEL_TSNE):
+        ax, ay, tot, cnt = oracle_forces(t, kernel, 0.6)
+        qx, qy, qt, qc = O.QuadTree(t["x"], t["y"], 16).forces(kernel, 1.0, 1.0, 0.0, 0.6, 4)
+        # same acceptances; node COMs differ only by the QuadTree's FP32-rounded child returns
+        assert np.array_equal(cnt, qc)
+        assert np.median(rel_err(ax, ay, qx, qy)) < 1e-6
+        assert abs(tot.sum() - qt) < 1e-6 * abs(qt)
+
+
+# ---------------------------------------------------------------------------------------------
+# leapfrog
+# ---------------------------------------------------------------------------------------------
+def test_leapfrog_free_particle_and_reversibility():
+    x, y, vx, vy = O.leapfrog([0.5], [0.25], [2.0], [-1.0], None, 10, 0.125, 1.0, 0.0, 0.5)
+    assert (x[0], y[0], vx[0], vy[0]) == (0.5 + 2.5, 0.25 - 1.25, 2.0, -1.0)
+    rng = np.random.default_rng(13)
+    n = 200
+    x0, y0 = rng.normal(size=n), rng.normal(size=n)
+    vx0, vy0 = rng.normal(0, 0.1, n), rng.normal(0, 0.1, n)
+    m = np.full(n, 1.0 / n)
+    x1, y1, vx1, vy1 = O.leapfrog(x0, y0, vx0, vy0, m, 20, 1e-3, 1.0, 1e-2, 0.0)
+    # KDK with exact forces (theta = 0) is time-reversible
+    x2, y2, vx2, vy2 = O.leapfrog(x1, y1, -vx1, -vy1, m, 20, 1e-3, 1.0, 1e-2, 0.0)
+    assert np.abs(x2 - x0).max() < 1e-12 and np.abs(vx2 + vx0).max() < 1e-12
+
+
+def test_leapfrog_two_body_energy():
+    # circular binary: energy is conserved to O(dt^2) over an orbit, momentum exactly
+    m = np.array([0.5, 0.5])
+    v = math.sqrt(1.0 * 0.5 / 1.0) / 2 * math.sqrt(2) / math.sqrt(2)
+    x0, y0 = np.array([-0.5, 0.5]), np.zeros(2)
+    vx0, vy0 = np.zeros(2), np.array([-v, v])
+
+    def energy(x, y, vx, vy):
+        r = math.hypot(x[1] - x[0], y[1] - y[0])
+        return 0.5 * (m * (vx ** 2 + vy ** 2)).sum() - m[0] * m[1] / r
+
+    x, y, vx, vy = O.leapfrog(x0, y0, vx0, vy0, m, 1000, 1e-3, 1.0, 0.0, 0.0)
+    e0, e1 = energy(x0, y0, vx0, vy0), energy(x, y, vx, vy)
+    assert abs(e1 - e0) < 1e-6 * abs(e0)
+    assert abs((m * vx).sum()) < 1e-15 and abs((m * vy).sum()) < 1e-15
+
+
+# ---------------------------------------------------------------------------------------------
+# t-SNE: gradient against finite differences of the KL cost, Z invariant
+# ---------------------------------------------------------------------------------------------
+def _tiny_p(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(size=(n, n)) * (rng.uniform(size=(n, n)) < 0.5)
+    A = A + A.T
+    np.fill_diagonal(A, 0)
+    A /= A.sum()
+    rows, cols = np.nonzero(A)
+    rp = np.zeros(n + 1, np.uint32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    return rp, cols.astype(np.uint32), A[rows, cols]
+
+
+def test_tsne_derivative_is_minus_ln2_kl_gradient():
+    n = 25
+    rp, col, val = _tiny_p(n, 14)
+    rng = np.random.default_rng(15)
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    # theta = 0: exact repulsion; exaggeration 4 = the folded factor 4 with no exaggeration
+    dx, dy, _, _, z = O.tsne_step(x, y, x, y, rp, col, val, 0.0, 4.0, 1.0, 0.0)
+    assert z == pytest.approx(O.tsne_z(x, y), rel=1e-13)
+    h = 1e-6
+    for i in range(0, n, 3):
+        for arr, d in ((x, dx), (y, dy)):
+            old = arr[i]
+            arr[i] = old + h
+            kp = O.tsne_kl(x, y, rp, col, val)
+            arr[i] = old - h
+            km = O.tsne_kl(x, y, rp, col, val)
+            arr[i] = old
+            g = (kp - km) / (2 * h)
+            assert d[i] == pytest.approx(-math.log(2.0) * g, rel=1e-6, abs=1e-9)
+
+
+def test_tsne_z_invariant_and_kl_nonnegative():
+    n = 300
+    rng = np.random.default_rng(16)
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    _, _, tot = O.direct(x, y, None, O.KERNEL_TSNE)
+    assert O.tsne_z(x, y) == pytest.approx(tot.sum(), rel=1e-13)
+    rp, col, val = _tiny_p(n, 17)
+    assert O.tsne_kl(x, y, rp, col, val) > 0
+    # BH Z at theta = 0 equals the exact Z
+    _, _, _, _, z = O.tsne_step(x, y, x, y, rp, col, val, 0.0, 4.0, 1.0, 0.0)
+    assert z == pytest.approx(tot.sum(), rel=1e-13)
+
+
+def test_tsne_step_update_rule():
+    # y' = y + lr d + mu (y - y_prev): with P = 0 rows and a far-apart pair the derivative is
+    # the pair's repulsion only, known in closed form
+    x, y = np.array([0.0, 3.0]), np.array([0.0, 4.0])
+    px, py = np.array([0.1, 3.0]), np.array([0.0, 4.2])
+    rp = np.zeros(3, np.uint32)
+    dx, dy, nx, ny, z = O.tsne_step(x, y, px, py, rp, np.zeros(0, np.uint32), np.zeros(0),
+                                    0.5, 16.0, 2.0, 0.5)
+    assert z == pytest.approx(2.0 / 26.0)
+    w2 = 1.0 / 26.0 ** 2
+    assert dx[0] == pytest.approx(4.0 / z * (-3.0 * w2))
+    assert nx[0] == pytest.approx(0.0 + 2.0 * dx[0] + 0.5 * (0.0 - 0.1))
+    assert ny[1] == pytest.approx(4.0 + 2.0 * dy[1] + 0.5 * (4.0 - 4.2))
+
+
+# ---------------------------------------------------------------------------------------------
+# MLP and Adam
+# ---------------------------------------------------------------------------------------------
+def _mlp_params(rng, din=10, h=8, dout=2):
+    return dict(W1=rng.normal(0, 0.5, (din, h)), b1=rng.normal(0, 0.1, h),
+                W2=rng.normal(0, 0.5, (h, h)), b2=rng.normal(0, 0.1, h),
+                W3=rng.normal(0, 0.5, (h, dout)), b3=rng.normal(0, 0.1, dout))
+
+
+def test_mlp_forward_closed_form():
+    rng = np.random.default_rng(18)
+    p = _mlp_params(rng)
+    X = rng.normal(size=(7, 10))
+    silu = lambda v: v / (1 + np.exp(-v))  # noqa: E731
+    want = silu(silu(X @ p["W1"] + p["b1"]) @ p["W2"] + p["b2"]) @ p["W3"] + p["b3"]
+    assert np.allclose(O.mlp_forward(X, p), want, rtol=1e-13, atol=1e-14)
+
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(19)
+    p = _mlp_params(rng)
+    X, T = rng.normal(size=(9, 10)), rng.normal(size=(9, 2))
+    loss, g = O.mlp_loss_grad(X, T, p)
+    pred = O.mlp_forward(X, p)
+    assert loss == pytest.approx(((pred - T) ** 2).mean(), rel=1e-13)
+    h = 1e-6
+    for k in p:
+        flat = p[k].reshape(-1)
+        for j in range(0, flat.size, max(1, flat.size // 6)):
+            old = flat[j]
+            flat[j] = old + h
+            lp, _ = O.mlp_loss_grad(X, T, p)
+            flat[j] = old - h
+            lm, _ = O.mlp_loss_grad(X, T, p)
+            flat[j] = old
+            assert g[k].reshape(-1)[j] == pytest.approx((lp - lm) / (2 * h), rel=1e-5, abs=1e-9)
+
+
+def test_adam_closed_form():
+    # first step from zero moments: m_hat = g, v_hat = g^2, p -= lr g / (|g| + eps)
+    p = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -4.0, 0.0])
+    m1, m2 = np.zeros(3), np.zeros(3)
+    O.adam(p, g, m1, m2, 0.01, 0.9, 0.999, 1e-8, 1)
+    assert p == pytest.approx([1.0 - 0.01 * 0.3 / (0.3 + 1e-8), -2.0 + 0.01 * 4.0 / (4.0 + 1e-8),
+                               0.5], rel=1e-12)
+    assert m1 == pytest.approx(0.1 * g) and m2 == pytest.approx(0.001 * g * g)
