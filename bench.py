@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC traffic calibration)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the theta sweep (traversal time and error at 0.3/0.5/0.7/1.0)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="traversal variant; -1 = autotune on the workload before timing")
     args = ap.parse_args()
@@ -138,11 +140,31 @@ def main():
     nodes, max_level, overfull = ctx.tree_info()
 
     # Barnes-Hut error of the final state against the FP64 direct sum on sampled sinks
-    sample = sample_indices(n, args.error_sample).astype(np.uint32)
-    ax, ay, _ = ctx.forces(theta, want_total=False)
-    dx, dy, _ = ctx.direct_fp64(sample)
-    num = np.hypot(ax[sample].astype(np.float64) - dx, ay[sample].astype(np.float64) - dy)
-    rel = num / np.hypot(dx, dy)
+    err = None
+    sweep = []
+    if args.error_sample > 0:
+        sample = sample_indices(n, args.error_sample).astype(np.uint32)
+        dx, dy, _ = ctx.direct_fp64(sample)
+        dnorm = np.hypot(dx, dy)
+
+        def bh_error(th):
+            ax, ay, _ = ctx.forces(th, want_total=False)
+            num = np.hypot(ax[sample].astype(np.float64) - dx, ay[sample].astype(np.float64) - dy)
+            rel = num / dnorm
+            return {"median_rel": float(np.median(rel)), "p99_rel": float(np.quantile(rel, 0.99))}
+
+        err = {"sample": int(sample.size), **bh_error(theta)}
+        if not args.no_sweep:
+            from masterthesislucas_amd.autotune import time_variant
+
+            for th in (0.3, 0.5, 0.7, 1.0):
+                ms = time_variant(ctx, variant, th, repeats=3)
+                cpn, cpp = ctx.counters()
+                sweep.append({"theta": th, "traversal_ms": ms,
+                              "interactions_per_particle": (cpn + cpp) / n,
+                              "force_evals_per_s_traversal_only": n / (ms / 1e3),
+                              **bh_error(th)})
+            ctx.set_variant(variant)
 
     result = {
         "metric": "BH force evals/sec at N=1M, theta=0.5",
@@ -173,9 +195,8 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "flop_per_launch": flop, "avg_launch_ms": trav_avg_s * 1e3},
-        "bh_error_vs_fp64_direct": {"sample": int(sample.size),
-                                    "median_rel": float(np.median(rel)),
-                                    "p99_rel": float(np.quantile(rel, 0.99))},
+        "bh_error_vs_fp64_direct": err,
+        "theta_sweep": sweep,
     }
     traffic_file = os.path.join(ROOT, "profiles", "traverse_traffic.json")
     if os.path.exists(traffic_file):
