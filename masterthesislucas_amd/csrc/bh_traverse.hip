@@ -47,11 +47,27 @@ __device__ __forceinline__ void accum(float dx, float dy, float d2, float m, flo
     }
 }
 
+// Wave-uniform 64-bit value moved to scalar registers (the compiler then keeps lane masks and
+// node indices in SGPRs and the per-lane booleans derived from them cost no vector work).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane(v);
+}
+
 // Traversal variants (picked per workload by the autotuner):
 //   NS:      sinks per lane; a wave walks the tree once for 64 NS Morton-consecutive particles,
 //            so every node fetch feeds NS independent interactions per lane
 //   BRANCHY: monopoles only when some lane accepts (else computed masked, unconditionally)
 //   PPU:     sources fetched per scalar load in the particle-particle loop
+//
+// Per-lane state that is a function of lane masks (walking / accepting / opening) is held as
+// lane booleans, which live in scalar register pairs: the acceptance test is the only vector
+// work per node and sink besides the interaction itself. Interaction counts are wave totals
+// (popcounts of the acceptance masks).
 template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
@@ -67,74 +83,70 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t base = (wb * kWavesPerBlock + w) * (uint32_t)(kWave * NS);
     uint32_t idx[NS];
-    bool valid[NS];
+    uint64_t onm[NS];  // lanes still walking the current subtree for sink s
     float px[NS], py[NS];
-    uint64_t active[NS];
-    uint64_t any = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         idx[s] = base + s * kWave + lane;
-        valid[s] = idx[s] < n;
-        const float4 me = valid[s] ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool valid = idx[s] < n;
+        onm[s] = uniform_u64(__ballot(valid));
+        const float4 me = valid ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
         px[s] = me.x;
         py[s] = me.y;
-        active[s] = __ballot(valid[s]);
-        any |= active[s];
     }
-    const uint32_t nn = prm->n_nodes;
+    const uint32_t nn = uniform_u32(prm->n_nodes);
 
     // Compensated totals (ax, ay, at) and their Kahan terms; monopoles of consecutive visits are
     // first gathered in a short plain batch (bx, by, bt) that is folded into the compensated
     // totals at every particle-particle visit and at least every kFoldEvery visits.
     float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
-    uint32_t npn[NS], npp[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] = by[s] = bt[s] = 0.f;
-        npn[s] = npp[s] = 0u;
-    }
+    for (int s = 0; s < NS; ++s) ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] =
+        by[s] = bt[s] = 0.f;
+    uint64_t cnt_pn = 0, cnt_pp = 0;  // wave totals
     uint32_t k = 0, end_cur = nn;
     int sp = 0, since_fold = 0;
-    Node cur = nodes[0];
-    while (any) {
-        while (k == end_cur && sp > 0) {
+    for (;;) {
+        while (k >= end_cur && sp > 0) {
             --sp;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) active[s] = st_mask[w][sp][s];
-            end_cur = st_end[w][sp];
+            for (int s = 0; s < NS; ++s) onm[s] = uniform_u64(st_mask[w][sp][s]);
+            end_cur = uniform_u32(st_end[w][sp]);
         }
         if (k >= end_cur) break;
+        uint64_t walking = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) walking |= onm[s];
+        if (!walking) {  // nobody left in this subtree frame
+            k = end_cur;
+            continue;
+        }
+        const Node cur = nodes[k];
         // skip > k for every node of a well-formed tree; max() keeps the walk finite regardless
         const uint32_t next = max(cur.skip, k + 1);
-        // the next node is k+1 (descend) or `next` (skip): fetch both while this one is used
-        const Node nd_down = nodes[min(k + 1, nn - 1)];
-        const Node nd_skip = nodes[min(next, nn - 1)];
         const float lim = cur.l2 * inv_theta2;
-        uint64_t open[NS];
-        uint64_t open_any = 0;
+        uint64_t openm[NS];
+        uint64_t open_any = 0, acc_any = 0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const bool on = (active[s] >> lane) & 1ull;
             const float dx = cur.cx - px[s], dy = cur.cy - py[s];
             const float d2 = fmaf(dx, dx, dy * dy);
-            const bool acc = on && (d2 > lim);
-            if (!BRANCHY || __ballot(acc)) {
+            const uint64_t accm = __ballot(d2 > lim) & onm[s];
+            if (!BRANCHY || accm) {
+                const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
                 accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
-                npn[s] += acc ? 1u : 0u;
+                cnt_pn += __popcll(accm);
             }
-            open[s] = __ballot(on && !acc);
-            open_any |= open[s];
+            openm[s] = onm[s] & ~accm;
+            open_any |= openm[s];
+            acc_any |= accm;
         }
         bool down = false;
         if (open_any) {
             if (cur.meta & kMetaPP) {
-                bool mine[NS];
                 float lx[NS], ly[NS], lt[NS];
 #pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    mine[s] = (open[s] >> lane) & 1ull;
-                    lx[s] = ly[s] = lt[s] = 0.f;
-                }
+                for (int s = 0; s < NS; ++s) lx[s] = ly[s] = lt[s] = 0.f;
                 const uint32_t f = cur.first, e = f + cur.count;
                 // PPU sources per scalar fetch; pos4 is padded past n, lanes mask the tail
                 for (uint32_t j = f; j < e; j += PPU) {
@@ -147,14 +159,16 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         for (int s = 0; s < NS; ++s) {
                             const float qx = q[t].x - px[s], qy = q[t].y - py[s];
                             const float q2 = fmaf(qx, qx, qy * qy);
-                            const bool use = mine[s] && (j + t < e) && (j + t != idx[s]);
+                            const bool use = __builtin_amdgcn_inverse_ballot_w64(openm[s]) &&
+                                             (j + t < e) && (j + t != idx[s]);
                             accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s], ly[s], lt[s]);
                         }
                     }
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    npp[s] += mine[s] ? cur.count - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
+                    const uint64_t self = __ballot(idx[s] >= f && idx[s] < e) & openm[s];
+                    cnt_pp += (uint64_t)__popcll(openm[s]) * cur.count - (uint64_t)__popcll(self);
                     kahan_add(ax[s], cax[s], lx[s] + bx[s]);
                     kahan_add(ay[s], cay[s], ly[s] + by[s]);
                     if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
@@ -162,18 +176,15 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                 }
                 since_fold = 0;
             } else {
-                bool changed = false;
+                // lanes that accepted sit out the subtree: remember the frame, narrow the masks
+                if (acc_any && sp < kStackDepth) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) changed |= (open[s] != active[s]);
-                if (changed && sp < kStackDepth) {
-                    if (lane == 0) {
-#pragma unroll
-                        for (int s = 0; s < NS; ++s) st_mask[w][sp][s] = active[s];
-                        st_end[w][sp] = end_cur;
+                    for (int s = 0; s < NS; ++s) {
+                        if (lane == 0) st_mask[w][sp][s] = onm[s];
+                        onm[s] = openm[s];
                     }
+                    if (lane == 0) st_end[w][sp] = end_cur;
                     ++sp;
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) active[s] = open[s];
                     end_cur = next;
                 }
                 down = true;
@@ -190,29 +201,23 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             since_fold = 0;
         }
         k = down ? k + 1 : next;
-        cur = down ? nd_down : nd_skip;
     }
-    unsigned long long s_pn = 0, s_pp = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         kahan_add(ax[s], cax[s], bx[s]);
         kahan_add(ay[s], cay[s], by[s]);
         if (TOT) kahan_add(at[s], cat[s], bt[s]);
-        if (valid[s]) {
+        if (idx[s] < n) {
             ax_out[idx[s]] = scale * (ax[s] - cax[s]);
             ay_out[idx[s]] = scale * (ay[s] - cay[s]);
             if (TOT) tot_out[idx[s]] = scale * (at[s] - cat[s]);
         }
-        s_pn += valid[s] ? npn[s] : 0u;
-        s_pp += valid[s] ? npp[s] : 0u;
     }
     if (counters) {  // one partial per block (no same-address device atomics)
         __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
-        s_pn = wave_sum_u64(s_pn);
-        s_pp = wave_sum_u64(s_pp);
         if (lane == 0) {
-            s_cnt[w][0] = s_pn;
-            s_cnt[w][1] = s_pp;
+            s_cnt[w][0] = cnt_pn;
+            s_cnt[w][1] = cnt_pp;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
