@@ -59,6 +59,134 @@ def cpu_baseline(p, theta, eps, n):
                       f"FP64, single thread"}
 
 
+def learned_comparison(args, eps):
+    """Learned far-field correction at theta_l vs plain Barnes-Hut: a 2D Plummer disk (a = 1),
+    N = args.learned_n. The network is trained on pairs labelled from a second realisation of
+    the distribution (different seed) and evaluated on the first. Errors are medians of the
+    relative force error against the FP64 direct sum on 4,096 seeded sinks; force evals/s are
+    N / (build + traversal) from the library's HIP-event phase timers."""
+    from masterthesislucas_amd import BHContext
+    from masterthesislucas_amd.autotune import time_variant
+    from masterthesislucas_amd.learned import FarFieldMLP
+    from masterthesislucas_amd.workloads import SEED_PLUMMER, plummer, sample_indices
+
+    n, th_l = args.learned_n, args.learned_theta
+    ctx = BHContext(n, kernel="gravity", tree_mode="quadtree", softening=eps)
+    ctx.learned_enable(True)
+    t0 = time.perf_counter()
+    log(f"[bench] learned: training system n={n}")
+    tr = plummer(n, a=1.0, seed=SEED_PLUMMER + 1)
+    ctx.set_particles(tr.x, tr.y, tr.m)
+    ctx.build()
+    net = FarFieldMLP(max_samples=args.learned_pairs + 4096 * 1024, seed=SEED_PLUMMER)
+    added = 0
+    for rnd in range(64):  # sinks in rounds until the pair budget is reached
+        if added >= args.learned_pairs:
+            break
+        sinks = sample_indices(n, 4096, seed=1000 + rnd).astype(np.uint32)
+        added += net.collect(ctx, th_l, sinks, band=args.learned_band, max_pairs_per_sink=1024)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = net.train(epochs=args.learned_epochs, batch=4096, lr=2e-3, seed=7)
+    t_train = time.perf_counter() - t0
+    log(f"[bench] learned: {added} pairs ({t_collect:.1f} s), trained {args.learned_epochs} "
+        f"epochs in {t_train:.1f} s, loss {loss:.3e}")
+    ev = plummer(n, a=1.0, seed=SEED_PLUMMER)
+    ctx.set_particles(ev.x, ev.y, ev.m)
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    for _ in range(3):
+        ctx.build()
+    ph = ctx.profile_read(reset=True)
+    build_ms = sum(ph[k][0] for k in ("bounds+morton", "sort", "gather", "tree", "upward")) / 3
+    sample = sample_indices(n, 4096).astype(np.uint32)
+    dx, dy, _ = ctx.direct_fp64(sample)
+    dn = np.hypot(dx, dy)
+
+    def med(ax, ay):
+        return float(np.median(np.hypot(ax[sample] - dx, ay[sample] - dy) / dn))
+
+    plain = []
+    for th in (0.3, 0.4, 0.5, 0.6, 0.7, 0.9):
+        ms = time_variant(ctx, ctx.variant, th, repeats=3)
+        e = med(*ctx.forces(th, want_total=False)[:2])
+        plain.append({"theta": th, "traversal_ms": ms, "median_rel": e,
+                      "force_evals_per_s": n / ((ms + build_ms) / 1e3)})
+    ctx.forces_learned(net, th_l, band=args.learned_band)  # warm
+    ctx.profile_enable(True)
+    ctx.profile_read(reset=True)
+    for _ in range(3):
+        ctx.forces_learned(net, th_l, band=args.learned_band)
+    lms = ctx.profile_read(reset=True)["traversal"][0] / 3
+    ctx.profile_enable(False)
+    le = med(*ctx.forces_learned(net, th_l, band=args.learned_band)[:2])
+    learned_rate = n / ((lms + build_ms) / 1e3)
+    p05 = next(r for r in plain if r["theta"] == 0.5)
+    # plain theta with the same median error (linear interpolation in the sweep)
+    eq = None
+    for a, b in zip(plain, plain[1:]):
+        if a["median_rel"] <= le <= b["median_rel"]:
+            f = (le - a["median_rel"]) / max(b["median_rel"] - a["median_rel"], 1e-30)
+            eq = {"theta": a["theta"] + f * (b["theta"] - a["theta"]),
+                  "force_evals_per_s": a["force_evals_per_s"]
+                  + f * (b["force_evals_per_s"] - a["force_evals_per_s"])}
+    out = {"workload": f"2D Plummer disk a=1.0, N={n}, eps={eps}",
+           "theta_learned": th_l, "band": args.learned_band, "training_pairs": added,
+           "training_epochs": args.learned_epochs, "train_loss": loss,
+           "collect_s": t_collect, "train_s": t_train, "build_ms": build_ms,
+           "learned_traversal_ms": lms, "learned_median_rel": le,
+           "learned_force_evals_per_s": learned_rate, "plain": plain,
+           "plain_at_equal_error": eq,
+           "speedup_vs_plain_theta_0.5": learned_rate / p05["force_evals_per_s"],
+           "error_ratio_vs_plain_theta_0.5": le / p05["median_rel"]}
+    net.close()
+    ctx.close()
+    return out
+
+
+def batched_comparison(args, eps, theta, dt):
+    """Batched systems: `args.batched_systems` independent systems with N log-uniform in
+    [1,024, 262,144] (even: uniform disks, odd: Plummer a = 0.1 truncated at R = 3), all in one
+    context (one set of
+    build launches, one traversal launch per step). Leapfrog steps timed eagerly and replayed
+    from captured HIP graphs; the traversal variant is autotuned on the batched tree."""
+    from masterthesislucas_amd import BHContext
+    from masterthesislucas_amd.autotune import tune
+    from masterthesislucas_amd.workloads import batched_systems
+
+    ps = batched_systems(args.batched_systems)
+    sizes = [len(p.x) for p in ps]
+    cat = [np.concatenate([getattr(p, f) for p in ps]) for f in ("x", "y", "m", "vx", "vy")]
+    n = int(sum(sizes))
+    log(f"[bench] batched: {len(sizes)} systems, {n} particles")
+    ctx = BHContext(n, kernel="gravity", softening=eps, sizes=sizes)
+    ctx.set_particles(*cat)
+    ctx.build()
+    variant, times = tune(ctx, theta)
+    ctx.set_particles(*cat)
+    K = args.batched_steps
+    ctx.leapfrog(2, dt, theta)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.leapfrog(K, dt, theta)
+    ctx.synchronize()
+    eager = (time.perf_counter() - t0) / K
+    ctx.leapfrog_graph(2, dt, theta)  # capture + warm
+    t0 = time.perf_counter()
+    ctx.leapfrog_graph(K, dt, theta)
+    graph = (time.perf_counter() - t0) / K
+    pn, pp = ctx.counters()
+    ctx.close()
+    return {"workload": f"{len(sizes)} systems, N log-uniform in [1024, 262144] (seeded), "
+                        f"even uniform disk / odd Plummer a=0.1 truncated at R=3, eps={eps}, "
+                        f"theta={theta}",
+            "systems": len(sizes), "particles": n, "n_min": int(min(sizes)),
+            "n_max": int(max(sizes)), "steps": K, "traverse_variant": variant,
+            "autotune_ms": {str(k): round(v, 4) for k, v in times.items()},
+            "eager_ms_per_step": eager * 1e3, "graph_ms_per_step": graph * 1e3,
+            "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +202,17 @@ def main():
                     help="run a plain device copy of this many MiB first (PMC traffic calibration)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the theta sweep (traversal time and error at 0.3/0.5/0.7/1.0)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-correction comparison (Plummer a=1, --learned-n)")
+    ap.add_argument("--learned-n", type=int, default=1 << 22)
+    ap.add_argument("--learned-theta", type=float, default=0.9)
+    ap.add_argument("--learned-band", type=float, default=0.0)
+    ap.add_argument("--learned-pairs", type=int, default=2_000_000)
+    ap.add_argument("--learned-epochs", type=int, default=8)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-systems run (256 systems, graph-captured steps)")
+    ap.add_argument("--batched-systems", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--variant", type=int, default=-1,
                     help="traversal variant; -1 = autotune on the workload before timing")
     args = ap.parse_args()
@@ -202,6 +341,11 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             result["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
+    ctx.close()
+    if not args.no_batched:
+        result["batched"] = batched_comparison(args, eps, theta, dt)
+    if not args.no_learned:
+        result["learned"] = learned_comparison(args, eps)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (reference BH, one core)")
         result["cpu_baseline"] = cpu_baseline(p, theta, eps, n)

@@ -76,6 +76,20 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg);
 void bh_destroy(bh_ctx* ctx);
 const char* bh_last_error(const bh_ctx* ctx);
 
+/* Many independent systems in one context (no counterpart in the reference, whose instances
+ * hold one system each): `systems` (<= 256) systems of sizes[s] > 0 particles, stored one after
+ * the other in the order given. Each system gets its own QuadTree (root square, leaf capacity,
+ * near-field rule and acceptance test as in BH_TREE_QUADTREE, with 24 code bits per axis: trees
+ * deeper than 24 levels stop there); all systems are built by the same launches and traversed
+ * by one launch whose waves each serve one system. Interactions never cross systems. Every
+ * entry point below works on a batched context; bh_direct_fp64 sums over the sink's own
+ * system. cfg->n_max is ignored (the total size is used). */
+bh_status bh_create_batched(bh_ctx** out, const bh_config* cfg, const uint32_t* sizes,
+                            uint32_t systems);
+/* number of systems (0 for a one-system context) and, if offsets is not NULL, the
+ * systems + 1 particle offsets */
+bh_status bh_batch_info(bh_ctx* ctx, uint32_t* systems, uint32_t* offsets);
+
 /* Upload particles in the caller's order (host pointers). m, vx, vy may be NULL:
  * masses default to 1 (the reference trees count particles, quadtree.h:163), velocities to 0.
  * The copy of a TsnePoint2D vector's positions that updateTree sees (tsne.h:202). */
@@ -106,6 +120,10 @@ bh_status bh_forces_device(bh_ctx* ctx, double theta, float* d_ax, float* d_ay, 
  * the reference's time step: update points, rebuild, solve; tsne_buffers.h:125-168).
  * Positions and velocities stay on the device; read them with bh_get_particles. */
 bh_status bh_leapfrog(bh_ctx* ctx, int steps, float dt, double theta);
+/* bh_leapfrog with each step (kick-drift, build, traversal, kick) captured once in a HIP graph
+ * and replayed: same kernels and results, without per-launch host overhead. The graphs are
+ * re-captured when dt, theta or the traversal variant change. */
+bh_status bh_leapfrog_graph(bh_ctx* ctx, int steps, float dt, double theta);
 
 /* FP64 direct sum for sampled sinks (NBodySolverNaive::solveNbody with the same kernel,
  * nBodySolverNaive.h:53-140), over all current particles with the sink itself excluded by
@@ -151,6 +169,46 @@ bh_status bh_tsne_run(bh_ctx* ctx, const bh_tsne_params* p, int32_t first, int32
 /* KL divergence of the current embedding with the exact FP64 Z (costFunction,
  * tsne.h:383-460; O(N^2) on the GPU in FP64) */
 bh_status bh_tsne_kl(bh_ctx* ctx, double* kl, double* z_exact);
+
+/* ---- learned far-field correction ----------------------------------------------------------
+ * No counterpart in the reference. A 10 -> 64 -> 64 -> 2 SiLU MLP predicts, for a cell the
+ * opening-angle test accepts, the residual between the exact sum over the cell's particles and
+ * its monopole (inputs and units in masterthesislucas_amd/csrc/bh_mlp.h). Training data come
+ * from the tree itself, labelled by FP64 direct sums over cell contents; training runs on the
+ * GPU (FP32 forward/backward kernels, Adam); inference inside the traversal uses BF16 MFMA with
+ * FP32 accumulation. Gravity kernel only. */
+typedef struct bh_mlp bh_mlp;
+uint32_t bh_mlp_param_count(void); /* 4994: W1[10x64] b1 W2[64x64] b2 W3[64x2] b3, W row-major */
+/* seeded uniform(+-1/sqrt(fan_in)) weights, zero biases; room for max_samples training pairs */
+bh_status bh_mlp_create(bh_mlp** out, uint32_t max_samples, uint64_t seed);
+void bh_mlp_destroy(bh_mlp* mlp);
+const char* bh_mlp_last_error(const bh_mlp* mlp);
+bh_status bh_mlp_set_params(bh_mlp* mlp, const float* params, uint32_t count); /* resets Adam */
+bh_status bh_mlp_get_params(bh_mlp* mlp, float* params, uint32_t count);
+/* Y[n x 2] for host X[n x 10]: FP32 kernels (bf16_mfma = 0) or the BF16 MFMA inference path */
+bh_status bh_mlp_forward(bh_mlp* mlp, const float* X, uint32_t n, float* Y, int bf16_mfma);
+/* mean squared error over the n x 2 outputs and its gradient (FP32 kernels) */
+bh_status bh_mlp_loss_grad(bh_mlp* mlp, const float* X, const float* T, uint32_t n, float* grads,
+                           double* loss);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with a given gradient */
+bh_status bh_mlp_adam_step(bh_mlp* mlp, const float* grads, float lr);
+bh_status bh_mlp_dataset_clear(bh_mlp* mlp);
+bh_status bh_mlp_dataset_size(bh_mlp* mlp, uint32_t* n);
+bh_status bh_mlp_dataset_get(bh_mlp* mlp, float* X, float* T, uint32_t n);
+bh_status bh_mlp_dataset_set(bh_mlp* mlp, const float* X, const float* T, uint32_t n);
+/* epochs over the data set in shuffled full batches; final_loss = MSE over the data set */
+bh_status bh_mlp_train(bh_mlp* mlp, int epochs, uint32_t batch, float lr, uint64_t seed,
+                       double* final_loss);
+/* keep cell second moments in the builds of ctx (needed by the two calls below) */
+bh_status bh_learned_enable(bh_ctx* ctx, int enable);
+/* append the (sink, cell) pairs the walk at theta accepts with l/r >= band (band 0: all) for
+ * the given caller-order sinks, labelled by FP64 sums over the cells, to mlp's data set */
+bh_status bh_learned_collect(bh_ctx* ctx, bh_mlp* mlp, double theta, double band,
+                             const uint32_t* sinks, uint32_t nsinks, uint32_t max_pairs_per_sink,
+                             uint32_t* added);
+/* bh_forces with the learned correction on accepted cells with l/r >= band */
+bh_status bh_forces_learned(bh_ctx* ctx, const bh_mlp* mlp, double theta, double band, float* ax,
+                            float* ay, double* total);
 
 /* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
 /* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */

@@ -68,6 +68,26 @@ SIGNATURES = {
     "bh_tsne_step": (_i32, [_vp, C.POINTER(TsneParams), _i32, _vp, _vp, _vp]),
     "bh_tsne_run": (_i32, [_vp, C.POINTER(TsneParams), _i32, _i32]),
     "bh_tsne_kl": (_i32, [_vp, _vp, _vp]),
+    "bh_mlp_param_count": (_u32, []),
+    "bh_mlp_create": (_i32, [_vp, _u32, _u64]),
+    "bh_mlp_destroy": (None, [_vp]),
+    "bh_mlp_last_error": (C.c_char_p, [_vp]),
+    "bh_mlp_set_params": (_i32, [_vp, _vp, _u32]),
+    "bh_mlp_get_params": (_i32, [_vp, _vp, _u32]),
+    "bh_mlp_forward": (_i32, [_vp, _vp, _u32, _vp, _i32]),
+    "bh_mlp_loss_grad": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "bh_mlp_adam_step": (_i32, [_vp, _vp, _f]),
+    "bh_mlp_dataset_clear": (_i32, [_vp]),
+    "bh_mlp_dataset_size": (_i32, [_vp, _vp]),
+    "bh_mlp_dataset_get": (_i32, [_vp, _vp, _vp, _u32]),
+    "bh_mlp_dataset_set": (_i32, [_vp, _vp, _vp, _u32]),
+    "bh_mlp_train": (_i32, [_vp, _i32, _u32, _f, _u64, _vp]),
+    "bh_learned_enable": (_i32, [_vp, _i32]),
+    "bh_learned_collect": (_i32, [_vp, _vp, _d, _d, _vp, _u32, _u32, _vp]),
+    "bh_forces_learned": (_i32, [_vp, _vp, _d, _d, _vp, _vp, _vp]),
+    "bh_create_batched": (_i32, [_vp, _vp, _vp, _u32]),
+    "bh_batch_info": (_i32, [_vp, _vp, _vp]),
+    "bh_leapfrog_graph": (_i32, [_vp, _i32, _f, _d]),
     "bh_get_bounds": (_i32, [_vp, _vp, _vp]),
     "bh_get_morton": (_i32, [_vp, _vp, _vp]),
     "bh_tree_info": (_i32, [_vp, _vp, _vp, _vp]),

@@ -12,6 +12,7 @@
 #include "../../include/bh.h"
 #include "bh_common.h"
 #include "bh_launch.h"
+#include "bh_mlp.h"
 
 using namespace bh;
 
@@ -98,6 +99,31 @@ struct bh_ctx {
     double* kl_part = nullptr;
     double* kl_out = nullptr;
     uint32_t* iota_dev = nullptr;
+
+    bool learned = false;  // second moments in builds, learned correction available
+
+    // batched systems (bh_create_batched): particles grouped by system, one tree per system
+    uint32_t nsys = 0;             // 0: one system
+    int R = 0;                     // level of the system roots in the tree (system id above)
+    int qbits = 0;                 // quantisation bits per axis of batched codes
+    std::vector<uint32_t> h_sys_off;
+    uint32_t* sys_off = nullptr;   // nsys + 1 offsets
+    BuildParams* sys_prm = nullptr;
+    uint32_t* roots = nullptr;
+    uint2* bblk = nullptr;         // bounds blocks: (first, count)
+    uint32_t nbblk = 0;
+    uint32_t* bblk_off = nullptr;  // first bounds block of each system, nsys + 1
+    float4* bpart = nullptr;
+    uint4* work[3]{};              // traversal work tables for 1, 2, 4 sinks per lane
+    uint32_t n_work[3]{};
+
+    // captured leapfrog steps (bh_leapfrog_graph), one per ping-pong parity
+    hipGraph_t graph[2]{};
+    hipGraphExec_t gexec[2]{};
+    bool g_valid = false;
+    float g_dt = 0.f;
+    double g_theta = 0.0;
+    int g_variant = -1;
 
     int variant = 4;  // traversal kernel variant (see launch_traverse)
     uint32_t trav_blocks = 0;  // blocks of the last traversal launch
@@ -196,14 +222,22 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     }
 
     phase_begin(c, 0);
-    launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
-                  quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
-    launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a, c->wide);
+    if (c->nsys) {
+        launch_bounds_batched(s, c->x[cur], c->y[cur], c->bblk, c->nbblk, c->bblk_off, c->sys_off,
+                              c->nsys, c->bpart, c->sys_prm, c->prm);
+        launch_morton_batched(s, c->x[cur], c->y[cur], n, c->sys_off, c->nsys, c->sys_prm,
+                              c->qbits, (uint64_t*)c->keys_a, c->vals_a);
+    } else {
+        launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
+                      quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
+        launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a, c->wide);
+    }
     phase_end(c, 0);
 
     phase_begin(c, 1);
     bool in_b = false;
-    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, 2 * c->B, c->wide, c->hist,
+    const int key_bits = c->nsys ? 2 * c->qbits + 2 * c->R : 2 * c->B;
+    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, key_bits, c->wide, c->hist,
                       c->scan_part, &in_b);
     phase_end(c, 1);
     c->keys_in_b = in_b;
@@ -221,17 +255,29 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 3);
     launch_tree(s, codes, c->wide, n, c->B, c->cfg.max_children, fixed, quadtree ? 1 : 0,
                 c->leaf_lvl, c->lcp1, c->cnt, c->noff, c->scan_part, c->prm, c->nodes, c->parent,
-                c->child4, c->cap_nodes);
+                c->child4, c->cap_nodes, c->R, c->nsys ? c->sys_prm : nullptr);
+    if (c->nsys) launch_batch_roots(s, c->sys_off, c->nsys, c->noff, c->roots);
     phase_end(c, 3);
 
     phase_begin(c, 4);
-    launch_upward(s, c->pos4, c->nodes, c->child4, c->m2, c->prm, c->deferred, c->cap_nodes);
+    launch_upward(s, c->pos4, c->nodes, c->child4, c->learned ? c->m2 : nullptr, c->prm, c->deferred,
+                  c->cap_nodes);
     phase_end(c, 4);
     bh_status st = launch_status(c, "build launch");
     if (st != BH_OK) return st;
     c->built = true;
     c->have_acc = false;
     return BH_OK;
+}
+
+TraverseBatch batch_of(const bh_ctx* c, int ns_index) {
+    TraverseBatch b;
+    if (c->nsys) {
+        b.work = c->work[ns_index];
+        b.n_work = c->n_work[ns_index];
+        b.roots = c->roots;
+    }
+    return b;
 }
 
 bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
@@ -243,10 +289,10 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
     phase_begin(c, 5);
     c->trav_blocks = launch_traverse(s, grav ? kKernelGravity : kKernelTsne, c->variant, c->pos4,
-                                     c->nodes, c->prm,
-                    c->n,
-                    inv_theta2, eps * eps, grav ? c->cfg.G : 1.0f, c->ax, c->ay,
-                    want_total ? c->tot : nullptr, c->counters);
+                                     c->nodes, c->prm, c->n, inv_theta2, eps * eps,
+                                     grav ? c->cfg.G : 1.0f, c->ax, c->ay,
+                                     want_total ? c->tot : nullptr, c->counters,
+                                     batch_of(c, c->variant < 4 ? 0 : (c->variant < 8 ? 1 : 2)));
     phase_end(c, 5);
     if (want_total)
         launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
@@ -275,9 +321,12 @@ bh_status bh_config_default(bh_config* cfg) {
 
 const char* bh_last_error(const bh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+static void graph_reset(bh_ctx* c);
+
 void bh_destroy(bh_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    graph_reset(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->dsinks) (void)hipFree(c->dsinks);
     if (c->p_row) (void)hipFree(c->p_row);
@@ -302,7 +351,8 @@ void bh_destroy(bh_ctx* c) {
     delete c;
 }
 
-bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
+static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_nodes,
+                             uint32_t extra_blocks) {
     if (!out || !cfg) return BH_ERR_ARG;
     *out = nullptr;
     if (cfg->n_max == 0 || cfg->n_max > (1u << 30)) return BH_ERR_ARG;
@@ -336,7 +386,8 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     if (c->wide) {
         // cells of one level are disjoint and an internal cell holds > max_children particles:
         // at most N leaves plus B * N / (max_children + 1) internal cells
-        const uint64_t capn = (uint64_t)N + (uint64_t)c->B * N / (uint64_t)(cfg->max_children + 1) + 64;
+        const uint64_t capn = (uint64_t)N + (uint64_t)c->B * N / (uint64_t)(cfg->max_children + 1) +
+                              64 + extra_nodes;
         if (capn > 0xFFFFFFF0ull) {
             *out = c;
             return fail(c, BH_ERR_CAPACITY, "node capacity overflows 32 bits");
@@ -394,7 +445,7 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     TRY(dalloc(c, &c->bounds_part, kMaxBoundsBlocks));
     TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
     TRY(dalloc(c, &c->dsum, 1));
-    c->max_blocks = traverse_blocks(N);
+    c->max_blocks = traverse_blocks(N) + extra_blocks;
     TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
     if (cfg->kernel == BH_KERNEL_TSNE) {
@@ -432,10 +483,92 @@ bh_status bh_create(bh_ctx** out, const bh_config* cfg) {
     return BH_OK;
 }
 
+bh_status bh_create(bh_ctx** out, const bh_config* cfg) { return create_impl(out, cfg, 0, 0); }
+
+// batched systems: codes of kBatchBits per axis under an 8-bit system id (4 tree levels)
+constexpr int kBatchBits = 24;
+constexpr int kBatchRootLevel = 4;
+constexpr uint32_t kBatchMaxSystems = 256;
+constexpr uint32_t kBatchBoundsChunk = 8192;
+
+bh_status bh_create_batched(bh_ctx** out, const bh_config* cfg, const uint32_t* sizes,
+                            uint32_t systems) {
+    if (!out || !cfg || !sizes) return BH_ERR_ARG;
+    *out = nullptr;
+    if (systems == 0 || systems > kBatchMaxSystems) return BH_ERR_ARG;
+    if (cfg->tree_mode != BH_TREE_QUADTREE) return BH_ERR_ARG;
+    uint64_t N = 0;
+    for (uint32_t s = 0; s < systems; ++s) {
+        if (sizes[s] == 0) return BH_ERR_ARG;
+        N += sizes[s];
+    }
+    if (N > (1u << 30)) return BH_ERR_ARG;
+    bh_config c2 = *cfg;
+    c2.n_max = (uint32_t)N;
+    c2.code_bits = kBatchBits + kBatchRootLevel;  // tree levels counted from the system id
+    bh_status st = create_impl(out, &c2, 64u * systems, systems + 1);
+    if (st != BH_OK) return st;
+    bh_ctx* c = *out;
+    c->nsys = systems;
+    c->R = kBatchRootLevel;
+    c->qbits = kBatchBits;
+    c->h_sys_off.assign(systems + 1, 0);
+    for (uint32_t s = 0; s < systems; ++s) c->h_sys_off[s + 1] = c->h_sys_off[s] + sizes[s];
+    std::vector<uint2> blk;
+    std::vector<uint32_t> blk_off(systems + 1, 0);
+    for (uint32_t s = 0; s < systems; ++s) {
+        for (uint32_t f = 0; f < sizes[s]; f += kBatchBoundsChunk)
+            blk.push_back(make_uint2(c->h_sys_off[s] + f, std::min(kBatchBoundsChunk, sizes[s] - f)));
+        blk_off[s + 1] = (uint32_t)blk.size();
+    }
+    c->nbblk = (uint32_t)blk.size();
+#define TRYB(expr)                \
+    do {                          \
+        st = (expr);              \
+        if (st != BH_OK) return st; \
+    } while (0)
+    TRYB(dalloc(c, &c->sys_off, systems + 1));
+    TRYB(dalloc(c, &c->sys_prm, systems));
+    TRYB(dalloc(c, &c->roots, systems));
+    TRYB(dalloc(c, &c->bblk, c->nbblk));
+    TRYB(dalloc(c, &c->bblk_off, systems + 1));
+    TRYB(dalloc(c, &c->bpart, c->nbblk));
+    hipStream_t s = c->stream;
+    TRYB(hip_check(c, hipMemcpyAsync(c->sys_off, c->h_sys_off.data(), (systems + 1) * 4,
+                                     hipMemcpyHostToDevice, s), "sys_off"));
+    TRYB(hip_check(c, hipMemcpyAsync(c->bblk, blk.data(), blk.size() * sizeof(uint2),
+                                     hipMemcpyHostToDevice, s), "blocks"));
+    TRYB(hip_check(c, hipMemcpyAsync(c->bblk_off, blk_off.data(), blk_off.size() * 4,
+                                     hipMemcpyHostToDevice, s), "blocks"));
+    for (int v = 0; v < 3; ++v) {
+        const uint32_t per = 64u << v;  // sinks per wave for 1, 2, 4 sinks per lane
+        std::vector<uint4> wk;
+        for (uint32_t q = 0; q < systems; ++q)
+            for (uint32_t f = 0; f < sizes[q]; f += per)
+                wk.push_back(make_uint4(c->h_sys_off[q] + f, std::min(per, sizes[q] - f), q, 0u));
+        c->n_work[v] = (uint32_t)wk.size();
+        TRYB(dalloc(c, &c->work[v], wk.size()));
+        TRYB(hip_check(c, hipMemcpyAsync(c->work[v], wk.data(), wk.size() * sizeof(uint4),
+                                         hipMemcpyHostToDevice, s), "work"));
+    }
+#undef TRYB
+    return hip_check(c, hipStreamSynchronize(s), "batched init");
+}
+
+bh_status bh_batch_info(bh_ctx* c, uint32_t* systems, uint32_t* offsets) {
+    if (!c || !systems) return BH_ERR_ARG;
+    *systems = c->nsys;
+    if (offsets)
+        for (uint32_t s = 0; s <= c->nsys; ++s) offsets[s] = c->h_sys_off[s];
+    return BH_OK;
+}
+
 static bh_status set_particles(bh_ctx* c, uint32_t n, const float* x, const float* y,
                                const float* m, const float* vx, const float* vy, bool device) {
     if (!c || !x || !y) return BH_ERR_ARG;
     if (n > c->cfg.n_max) return fail(c, BH_ERR_ARG, "n=%u exceeds n_max=%u", n, c->cfg.n_max);
+    if (c->nsys && n != c->h_sys_off[c->nsys])
+        return fail(c, BH_ERR_ARG, "batched systems hold %u particles, got %u", c->h_sys_off[c->nsys], n);
     if ((vx == nullptr) != (vy == nullptr)) return fail(c, BH_ERR_ARG, "vx and vy go together");
     hipStream_t s = c->stream;
     const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -572,6 +705,85 @@ bh_status bh_leapfrog(bh_ctx* c, int steps, float dt, double theta) {
     return hip_check(c, hipStreamSynchronize(c->stream), "leapfrog");
 }
 
+// one kick-drift-kick step on the context's stream (no host synchronisation, no profiling)
+static bh_status leapfrog_step(bh_ctx* c, float dt, double theta) {
+    const float hdt = 0.5f * dt;
+    const int cur = c->cur;
+    launch_kick_drift(c->stream, c->x[cur], c->y[cur], c->vx[cur], c->vy[cur], c->ax, c->ay, c->n,
+                      hdt, dt);
+    bh_status st = do_build(c, nullptr);
+    if (st == BH_OK) st = do_forces(c, theta, false);
+    if (st != BH_OK) return st;
+    const int nc = c->cur;
+    launch_kick(c->stream, c->vx[nc], c->vy[nc], c->ax, c->ay, c->n, hdt);
+    return launch_status(c, "leapfrog step");
+}
+
+static void graph_reset(bh_ctx* c) {
+    for (int p = 0; p < 2; ++p) {
+        if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+        if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
+        c->gexec[p] = nullptr;
+        c->graph[p] = nullptr;
+    }
+    c->g_valid = false;
+}
+
+bh_status bh_leapfrog_graph(bh_ctx* c, int steps, float dt, double theta) {
+    if (!c || steps < 0) return BH_ERR_ARG;
+    if (c->n == 0) return fail(c, BH_ERR_STATE, "no particles");
+    bh_status st;
+    if (!c->have_acc) {
+        st = do_build(c, nullptr);
+        if (st == BH_OK) st = do_forces(c, theta, false);
+        if (st != BH_OK) return st;
+    }
+    if (!c->g_valid || c->g_dt != dt || c->g_theta != theta || c->g_variant != c->variant) {
+        graph_reset(c);
+        // capture one step from each ping-pong parity; the host-side state the build updates
+        // (current buffer, sort parity) is the same on replay, so it is restored afterwards
+        const bool prof = c->profile;
+        c->profile = false;
+        const int cur0 = c->cur;
+        st = hip_check(c, hipStreamSynchronize(c->stream), "graph sync");
+        for (int k = 0; k < 2 && st == BH_OK; ++k) {
+            const int p = c->cur;
+            st = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                           "begin capture");
+            if (st != BH_OK) break;
+            bh_status cs = leapfrog_step(c, dt, theta);
+            hipGraph_t g = nullptr;
+            st = hip_check(c, hipStreamEndCapture(c->stream, &g), "end capture");
+            if (cs != BH_OK) st = cs;
+            if (st == BH_OK) {
+                c->graph[p] = g;
+                st = hip_check(c, hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0),
+                               "graph instantiate");
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+        }
+        c->cur = cur0;
+        c->profile = prof;
+        if (st != BH_OK) {
+            graph_reset(c);
+            return st;
+        }
+        c->g_valid = true;
+        c->g_dt = dt;
+        c->g_theta = theta;
+        c->g_variant = c->variant;
+    }
+    for (int k = 0; k < steps; ++k) {
+        st = hip_check(c, hipGraphLaunch(c->gexec[c->cur], c->stream), "graph launch");
+        if (st != BH_OK) return st;
+        c->cur ^= 1;
+    }
+    c->built = true;
+    c->have_acc = true;
+    return hip_check(c, hipStreamSynchronize(c->stream), "graph steps");
+}
+
 bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* ax, double* ay,
                          double* total) {
     if (!c || (!sinks && ns)) return BH_ERR_ARG;
@@ -583,9 +795,6 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
     const uint64_t need_s = 2ull * ns, need_p = (uint64_t)(3ull * S + 3ull) * ns;
     if (need_s > c->dsink_cap) {
         if (c->dsinks) (void)hipFree(c->dsinks);
-    if (c->p_row) (void)hipFree(c->p_row);
-    if (c->p_col) (void)hipFree(c->p_col);
-    if (c->p_val) (void)hipFree(c->p_val);
         c->dsinks = nullptr;
         c->dsink_cap = 0;
         bh_status st = hip_check(c, hipMalloc((void**)&c->dsinks, need_s * 4), "hipMalloc sinks");
@@ -609,7 +818,7 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
     const double eps = c->cfg.softening;
     launch_direct_f64(s, grav ? kKernelGravity : kKernelTsne, c->pos4, c->id[c->cur], c->n,
                       c->dsinks, ns, grav ? (double)c->cfg.G : 1.0, eps * eps, c->where,
-                      c->dsinks + ns, c->dpart, S, out);
+                      c->dsinks + ns, c->dpart, S, out, c->nsys ? c->sys_off : nullptr, c->nsys);
     st = launch_status(c, "direct launch");
     if (st != BH_OK) return st;
     const size_t b = (size_t)ns * 8;
@@ -985,6 +1194,433 @@ int bh_traverse_variant_count(void) { return kTraverseVariants; }
 bh_status bh_synchronize(bh_ctx* c) {
     if (!c) return BH_ERR_ARG;
     return hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// learned far-field correction
+// ---------------------------------------------------------------------------------------------
+struct bh_mlp {
+    std::string err;
+    hipStream_t stream = nullptr;
+    float* params = nullptr;  // FP32 master copy
+    float *m1 = nullptr, *m2 = nullptr;
+    MlpBf16* wbf = nullptr;
+    int step = 0;
+    float *X = nullptr, *T = nullptr;  // data set
+    uint32_t cap = 0, n = 0;
+    float* S = nullptr;  // training scratch
+    size_t S_cap = 0;
+    float* partials = nullptr;
+    size_t part_cap = 0;
+    double* loss_part = nullptr;
+    size_t loss_cap = 0;
+    uint32_t* order = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+bh_status mfail(bh_mlp* m, bh_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (m) m->err = buf;
+    return st;
+}
+
+bh_status mcheck(bh_mlp* m, hipError_t e, const char* what) {
+    if (e == hipSuccess) return BH_OK;
+    return mfail(m, e == hipErrorOutOfMemory ? BH_ERR_OOM : BH_ERR_HIP, "%s: %s", what,
+                 hipGetErrorString(e));
+}
+
+template <typename T>
+bh_status malloc_m(bh_mlp* m, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) ? count * sizeof(T) : 16);
+    if (e != hipSuccess) return mcheck(m, e, "hipMalloc");
+    m->allocs.push_back(q);
+    *p = (T*)q;
+    return BH_OK;
+}
+
+// grow-only scratch owned by the network object (freed with it)
+template <typename T>
+bh_status ensure_m(bh_mlp* m, T** p, size_t* cap, size_t count) {
+    if (count <= *cap) return BH_OK;
+    if (*p) {
+        for (auto& a : m->allocs)
+            if (a == (void*)*p) a = nullptr;
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    bh_status st = malloc_m(m, p, count);
+    if (st == BH_OK) *cap = count;
+    return st;
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+bh_status mlp_upload_params(bh_mlp* m, const float* host) {
+    bh_status st = mcheck(m, hipMemcpyAsync(m->params, host, kMlpParams * 4, hipMemcpyHostToDevice,
+                                            m->stream), "params");
+    if (st != BH_OK) return st;
+    st = mcheck(m, hipMemsetAsync(m->m1, 0, kMlpParams * 4, m->stream), "adam");
+    if (st == BH_OK) st = mcheck(m, hipMemsetAsync(m->m2, 0, kMlpParams * 4, m->stream), "adam");
+    m->step = 0;
+    launch_mlp_prep(m->stream, m->params, m->wbf);
+    if (st == BH_OK) st = mcheck(m, hipGetLastError(), "prep");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "params");
+    return st;
+}
+
+// host buffer -> temporary device buffer (caller frees)
+template <typename T>
+bh_status to_device(bh_mlp* m, const T* h, size_t count, T** d) {
+    *d = nullptr;
+    hipError_t e = hipMalloc((void**)d, count * sizeof(T) ? count * sizeof(T) : 16);
+    if (e != hipSuccess) return mcheck(m, e, "hipMalloc");
+    if (count) e = hipMemcpyAsync(*d, h, count * sizeof(T), hipMemcpyHostToDevice, m->stream);
+    return mcheck(m, e, "upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t bh_mlp_param_count(void) { return kMlpParams; }
+
+const char* bh_mlp_last_error(const bh_mlp* m) { return m ? m->err.c_str() : "null network"; }
+
+void bh_mlp_destroy(bh_mlp* m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* p : m->allocs)
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+bh_status bh_mlp_create(bh_mlp** out, uint32_t max_samples, uint64_t seed) {
+    if (!out) return BH_ERR_ARG;
+    *out = nullptr;
+    bh_mlp* m = new (std::nothrow) bh_mlp();
+    if (!m) return BH_ERR_OOM;
+    bh_status st = mcheck(m, hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream");
+    if (st == BH_OK) st = malloc_m(m, &m->params, kMlpParams);
+    if (st == BH_OK) st = malloc_m(m, &m->m1, kMlpParams);
+    if (st == BH_OK) st = malloc_m(m, &m->m2, kMlpParams);
+    if (st == BH_OK) st = malloc_m(m, &m->wbf, 1);
+    if (st == BH_OK) st = malloc_m(m, &m->X, (size_t)max_samples * kMlpIn);
+    if (st == BH_OK) st = malloc_m(m, &m->T, (size_t)max_samples * kMlpOut);
+    if (st == BH_OK) st = malloc_m(m, &m->order, max_samples);
+    if (st != BH_OK) {
+        *out = m;  // error string readable; caller destroys
+        return st;
+    }
+    m->cap = max_samples;
+    // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases
+    std::vector<float> h(kMlpParams, 0.f);
+    uint64_t s = seed;
+    auto fill = [&](int off, int fan_in, int count) {
+        const double a = 1.0 / std::sqrt((double)fan_in);
+        for (int i = 0; i < count; ++i) {
+            const double u = (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+            h[off + i] = (float)((2.0 * u - 1.0) * a);
+        }
+    };
+    fill(kMlpOffW1, kMlpIn, kMlpIn * kMlpH);
+    fill(kMlpOffW2, kMlpH, kMlpH * kMlpH);
+    fill(kMlpOffW3, kMlpH, kMlpH * kMlpOut);
+    st = mlp_upload_params(m, h.data());
+    *out = m;
+    return st;
+}
+
+bh_status bh_mlp_set_params(bh_mlp* m, const float* params, uint32_t count) {
+    if (!m || !params) return BH_ERR_ARG;
+    if (count != (uint32_t)kMlpParams) return mfail(m, BH_ERR_ARG, "expected %d parameters", kMlpParams);
+    return mlp_upload_params(m, params);
+}
+
+bh_status bh_mlp_get_params(bh_mlp* m, float* params, uint32_t count) {
+    if (!m || !params) return BH_ERR_ARG;
+    if (count != (uint32_t)kMlpParams) return mfail(m, BH_ERR_ARG, "expected %d parameters", kMlpParams);
+    bh_status st = mcheck(m, hipMemcpyAsync(params, m->params, kMlpParams * 4, hipMemcpyDeviceToHost,
+                                            m->stream), "params");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "params");
+    return st;
+}
+
+bh_status bh_mlp_forward(bh_mlp* m, const float* X, uint32_t n, float* Y, int bf16_mfma) {
+    if (!m || (n && (!X || !Y))) return BH_ERR_ARG;
+    if (!n) return BH_OK;
+    float *dX = nullptr, *dY = nullptr;
+    bh_status st = to_device(m, X, (size_t)n * kMlpIn, &dX);
+    if (st == BH_OK) st = mcheck(m, hipMalloc((void**)&dY, (size_t)n * kMlpOut * 4), "hipMalloc");
+    if (st == BH_OK) {
+        if (bf16_mfma) launch_mlp_infer_bf16(m->stream, dX, n, m->wbf, dY);
+        else launch_mlp_forward_f32(m->stream, dX, n, m->params, dY);
+        st = mcheck(m, hipGetLastError(), "forward");
+    }
+    if (st == BH_OK)
+        st = mcheck(m, hipMemcpyAsync(Y, dY, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, m->stream), "Y");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "forward");
+    (void)hipStreamSynchronize(m->stream);
+    if (dX) (void)hipFree(dX);
+    if (dY) (void)hipFree(dY);
+    return st;
+}
+
+static bh_status mlp_grad_device(bh_mlp* m, const float* dX, const float* dT, const uint32_t* order,
+                                 uint32_t B) {
+    bh_status st = ensure_m(m, &m->S, &m->S_cap, mlp_scratch_floats(B));
+    if (st == BH_OK) st = ensure_m(m, &m->partials, &m->part_cap, (size_t)mlp_grad_chunks(B) * kMlpParams);
+    if (st == BH_OK) st = ensure_m(m, &m->loss_part, &m->loss_cap, mlp_loss_blocks(B));
+    if (st != BH_OK) return st;
+    launch_mlp_grad(m->stream, dX, dT, order, B, m->params, m->S, m->loss_part, m->partials);
+    return mcheck(m, hipGetLastError(), "gradient");
+}
+
+bh_status bh_mlp_loss_grad(bh_mlp* m, const float* X, const float* T, uint32_t n, float* grads,
+                           double* loss) {
+    if (!m || !X || !T || !n) return BH_ERR_ARG;
+    float *dX = nullptr, *dT = nullptr, *dG = nullptr;
+    bh_status st = to_device(m, X, (size_t)n * kMlpIn, &dX);
+    if (st == BH_OK) st = to_device(m, T, (size_t)n * kMlpOut, &dT);
+    if (st == BH_OK) st = mcheck(m, hipMalloc((void**)&dG, kMlpParams * 4), "hipMalloc");
+    if (st == BH_OK) st = mlp_grad_device(m, dX, dT, nullptr, n);
+    if (st == BH_OK) {
+        launch_mlp_adam(m->stream, m->params, m->partials, mlp_grad_chunks(n), m->m1, m->m2, 0.f, 1,
+                        dG, 0);
+        st = mcheck(m, hipGetLastError(), "gradient sum");
+    }
+    std::vector<double> lp(mlp_loss_blocks(n));
+    if (st == BH_OK && grads)
+        st = mcheck(m, hipMemcpyAsync(grads, dG, kMlpParams * 4, hipMemcpyDeviceToHost, m->stream), "grads");
+    if (st == BH_OK)
+        st = mcheck(m, hipMemcpyAsync(lp.data(), m->loss_part, lp.size() * 8, hipMemcpyDeviceToHost,
+                                      m->stream), "loss");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "gradient");
+    (void)hipStreamSynchronize(m->stream);
+    if (dX) (void)hipFree(dX);
+    if (dT) (void)hipFree(dT);
+    if (dG) (void)hipFree(dG);
+    if (st == BH_OK && loss) {
+        double s = 0.0;
+        for (double v : lp) s += v;
+        *loss = s / (2.0 * n);  // mean over the n * 2 outputs
+    }
+    return st;
+}
+
+bh_status bh_mlp_adam_step(bh_mlp* m, const float* grads, float lr) {
+    if (!m || !grads) return BH_ERR_ARG;
+    float* dG = nullptr;
+    bh_status st = to_device(m, grads, kMlpParams, &dG);
+    if (st == BH_OK) {
+        launch_mlp_adam_grad(m->stream, m->params, dG, m->m1, m->m2, lr, ++m->step);
+        launch_mlp_prep(m->stream, m->params, m->wbf);
+        st = mcheck(m, hipGetLastError(), "adam");
+    }
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "adam");
+    (void)hipStreamSynchronize(m->stream);
+    if (dG) (void)hipFree(dG);
+    return st;
+}
+
+bh_status bh_mlp_dataset_clear(bh_mlp* m) {
+    if (!m) return BH_ERR_ARG;
+    m->n = 0;
+    return BH_OK;
+}
+
+bh_status bh_mlp_dataset_size(bh_mlp* m, uint32_t* n) {
+    if (!m || !n) return BH_ERR_ARG;
+    *n = m->n;
+    return BH_OK;
+}
+
+bh_status bh_mlp_dataset_get(bh_mlp* m, float* X, float* T, uint32_t n) {
+    if (!m) return BH_ERR_ARG;
+    if (n > m->n) return mfail(m, BH_ERR_ARG, "data set holds %u samples", m->n);
+    bh_status st = BH_OK;
+    if (X) st = mcheck(m, hipMemcpyAsync(X, m->X, (size_t)n * kMlpIn * 4, hipMemcpyDeviceToHost, m->stream), "X");
+    if (st == BH_OK && T)
+        st = mcheck(m, hipMemcpyAsync(T, m->T, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, m->stream), "T");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "data set");
+    return st;
+}
+
+bh_status bh_mlp_dataset_set(bh_mlp* m, const float* X, const float* T, uint32_t n) {
+    if (!m || (n && (!X || !T))) return BH_ERR_ARG;
+    if (n > m->cap) return mfail(m, BH_ERR_CAPACITY, "%u samples exceed the capacity %u", n, m->cap);
+    bh_status st = BH_OK;
+    if (n) st = mcheck(m, hipMemcpyAsync(m->X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, m->stream), "X");
+    if (st == BH_OK && n)
+        st = mcheck(m, hipMemcpyAsync(m->T, T, (size_t)n * kMlpOut * 4, hipMemcpyHostToDevice, m->stream), "T");
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "data set");
+    if (st == BH_OK) m->n = n;
+    return st;
+}
+
+bh_status bh_mlp_train(bh_mlp* m, int epochs, uint32_t batch, float lr, uint64_t seed,
+                       double* final_loss) {
+    if (!m || epochs < 0 || batch == 0) return BH_ERR_ARG;
+    if (m->n == 0) return mfail(m, BH_ERR_STATE, "empty data set");
+    const uint32_t n = m->n;
+    const uint32_t B = std::min(batch, n);
+    std::vector<uint32_t> perm(n);
+    for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+    uint64_t s = seed;
+    bh_status st = BH_OK;
+    for (int ep = 0; ep < epochs && st == BH_OK; ++ep) {
+        for (uint32_t i = n - 1; i > 0; --i) {  // Fisher-Yates
+            const uint32_t j = (uint32_t)(splitmix64(s) % (uint64_t)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        st = mcheck(m, hipMemcpyAsync(m->order, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                                      m->stream), "order");
+        for (uint32_t b0 = 0; b0 + B <= n && st == BH_OK; b0 += B) {  // full batches only
+            st = mlp_grad_device(m, m->X, m->T, m->order + b0, B);
+            if (st != BH_OK) break;
+            launch_mlp_adam(m->stream, m->params, m->partials, mlp_grad_chunks(B), m->m1, m->m2, lr,
+                            ++m->step, nullptr, 1);
+            st = mcheck(m, hipGetLastError(), "adam");
+        }
+        // the host permutation buffer is rewritten next epoch: wait for the upload to be used
+        if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "epoch");
+    }
+    if (st == BH_OK) {
+        launch_mlp_prep(m->stream, m->params, m->wbf);
+        st = mcheck(m, hipGetLastError(), "prep");
+    }
+    if (st == BH_OK && final_loss) {
+        // loss over the whole data set in chunks (forward + backward kernels, loss partials only)
+        double tot = 0.0;
+        const uint32_t C = 65536;
+        for (uint32_t o = 0; o < n && st == BH_OK; o += C) {
+            const uint32_t b = std::min(C, n - o);
+            st = mlp_grad_device(m, m->X + (size_t)o * kMlpIn, m->T + (size_t)o * kMlpOut, nullptr, b);
+            std::vector<double> lp(mlp_loss_blocks(b));
+            if (st == BH_OK)
+                st = mcheck(m, hipMemcpyAsync(lp.data(), m->loss_part, lp.size() * 8,
+                                              hipMemcpyDeviceToHost, m->stream), "loss");
+            if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "loss");
+            for (double v : lp) tot += v;
+        }
+        *final_loss = tot / (2.0 * n);
+    }
+    if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "train");
+    return st;
+}
+
+bh_status bh_learned_enable(bh_ctx* c, int enable) {
+    if (!c) return BH_ERR_ARG;
+    if (c->cfg.kernel != BH_KERNEL_GRAVITY)
+        return fail(c, BH_ERR_STATE, "the learned correction is trained for the gravity kernel");
+    if (enable && !c->m2) {
+        bh_status st = dalloc(c, &c->m2, c->cap_nodes);
+        if (st != BH_OK) return st;
+    }
+    c->learned = enable != 0;
+    c->built = false;  // second moments appear with the next build
+    return BH_OK;
+}
+
+bh_status bh_learned_collect(bh_ctx* c, bh_mlp* m, double theta, double band,
+                             const uint32_t* sinks, uint32_t ns, uint32_t max_pairs_per_sink,
+                             uint32_t* added) {
+    if (!c || !m || (!sinks && ns) || !(theta > 0.0) || band < 0.0 || max_pairs_per_sink == 0)
+        return BH_ERR_ARG;
+    if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
+    if (ns == 0) return BH_OK;
+    for (uint32_t k = 0; k < ns; ++k)
+        if (sinks[k] >= c->n) return fail(c, BH_ERR_ARG, "sink index %u out of range", sinks[k]);
+    hipStream_t s = c->stream;
+    uint32_t *d_sc = nullptr, *d_ss = nullptr, *pairs = nullptr, *counts = nullptr, *offs = nullptr,
+             *spart = nullptr;
+    const size_t np = (size_t)ns * max_pairs_per_sink;
+    bh_status st = hip_check(c, hipMalloc((void**)&d_sc, (size_t)ns * 4), "hipMalloc");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&d_ss, (size_t)ns * 4), "hipMalloc");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&pairs, np * 4), "hipMalloc");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&counts, (size_t)ns * 4), "hipMalloc");
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&offs, (size_t)ns * 4), "hipMalloc");
+    const size_t nsp = scan_partials_size(ns);
+    if (st == BH_OK) st = hip_check(c, hipMalloc((void**)&spart, (nsp + 1) * 4), "hipMalloc");
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(d_sc, sinks, (size_t)ns * 4, hipMemcpyHostToDevice, s), "sinks");
+    uint32_t last[1] = {0};
+    if (st == BH_OK) {
+        launch_sinks_sorted(s, c->id[c->cur], c->n, d_sc, ns, c->where, d_ss);
+        const float inv_t2 = (float)(1.0 / (theta * theta));
+        const float inv_b2 = band > 0.0 ? (float)(1.0 / (band * band)) : 0.f;
+        launch_collect_pairs(s, c->pos4, c->nodes, c->prm, d_ss, ns, inv_t2, inv_b2,
+                             max_pairs_per_sink, pairs, counts);
+        launch_exclusive_scan(s, counts, offs, ns, spart, spart + nsp);
+        st = launch_status(c, "collect");
+    }
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&last[0], spart + nsp, 4, hipMemcpyDeviceToHost, s), "total");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "collect");
+    const uint64_t total = last[0];
+    if (st == BH_OK && m->n + total > m->cap)
+        st = fail(c, BH_ERR_CAPACITY, "data set capacity %u exceeded (%u + %llu)", m->cap, m->n,
+                  (unsigned long long)total);
+    if (st == BH_OK) {
+        const double eps = c->cfg.softening;
+        launch_label_pairs(s, c->pos4, c->nodes, c->m2, d_ss, pairs, counts, offs, ns,
+                           max_pairs_per_sink, eps * eps, m->X + (size_t)m->n * kMlpIn,
+                           m->T + (size_t)m->n * kMlpOut, (uint32_t)total);
+        st = launch_status(c, "label");
+    }
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "label");
+    (void)hipStreamSynchronize(s);
+    for (uint32_t* p : {d_sc, d_ss, pairs, counts, offs, spart})
+        if (p) (void)hipFree(p);
+    if (st == BH_OK) {
+        m->n += (uint32_t)total;
+        if (added) *added = (uint32_t)total;
+    }
+    return st;
+}
+
+bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double band, float* ax,
+                            float* ay, double* total) {
+    if (!c || !m || !(theta > 0.0) || band < 0.0) return BH_ERR_ARG;
+    if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
+    hipStream_t s = c->stream;
+    const float inv_t2 = (float)(1.0 / (theta * theta));
+    const float inv_b2 = band > 0.0 ? (float)(1.0 / (band * band)) : 0.f;
+    const float eps = c->cfg.softening;
+    phase_begin(c, 5);
+    c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->m2, c->prm, c->n,
+                                             inv_t2, inv_b2, eps * eps, c->cfg.G, m->wbf, c->ax,
+                                             c->ay, total ? c->tot : nullptr, c->counters,
+                                             batch_of(c, c->variant < 4 ? 0 : 1));
+    phase_end(c, 5);
+    if (total) launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
+    c->have_acc = true;
+    bh_status st = launch_status(c, "learned traverse");
+    if (st != BH_OK) return st;
+    const uint32_t n = c->n;
+    if (ax || ay) {
+        launch_unsort2(s, c->id[c->cur], n, c->ax, c->ay, c->out_a, c->out_b);
+        if (ax) st = hip_check(c, hipMemcpyAsync(ax, c->out_a, (size_t)n * 4, hipMemcpyDeviceToHost, s), "ax");
+        if (st == BH_OK && ay) st = hip_check(c, hipMemcpyAsync(ay, c->out_b, (size_t)n * 4, hipMemcpyDeviceToHost, s), "ay");
+    }
+    if (st == BH_OK && total) st = hip_check(c, hipMemcpyAsync(c->h_dsum, c->dsum, 8, hipMemcpyDeviceToHost, s), "total");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "learned forces");
+    if (st == BH_OK && total) *total = *c->h_dsum;
+    return st;
 }
 
 }  // extern "C"

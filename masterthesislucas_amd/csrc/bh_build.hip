@@ -52,6 +52,9 @@ __global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* __restri
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
+__device__ void fill_params(BuildParams* prm, float4 r, uint64_t mnx64, uint64_t mny64,
+                            uint64_t mxx64, uint64_t mxy64, uint32_t tree_mode, uint32_t n);
+
 __global__ __launch_bounds__(kBlock) void k_bounds_final(const float4* __restrict__ partials,
                                                          uint32_t nb, uint32_t n,
                                                          BuildParams* __restrict__ prm,
@@ -74,6 +77,47 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const float4* __restric
         mxx64 = bh_sfp_from_f32(r.z);
         mxy64 = bh_sfp_from_f32(r.w);
     }
+    fill_params(prm, r, mnx64, mny64, mxx64, mxy64, tree_mode, n);
+}
+
+// batched systems: bounds per system from per-block partials over each system's blocks
+__global__ __launch_bounds__(kBlock) void k_bounds_seg_partial(const float* __restrict__ xs,
+                                                               const float* __restrict__ ys,
+                                                               const uint2* __restrict__ blk,
+                                                               float4* __restrict__ partials) {
+    const uint2 bk = blk[blockIdx.x];  // (first particle, count)
+    float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = threadIdx.x; i < bk.y; i += kBlock) {
+        const float x = xs[bk.x + i], y = ys[bk.x + i];
+        r = minmax4(r, make_float4(x, y, x, y));
+    }
+    r = block_minmax4(r);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// one block per system: partials [blk_off[s], blk_off[s+1]) -> sys_prm[s]
+__global__ __launch_bounds__(kBlock) void k_bounds_seg_final(const float4* __restrict__ partials,
+                                                             const uint32_t* __restrict__ blk_off,
+                                                             const uint32_t* __restrict__ sys_off,
+                                                             BuildParams* __restrict__ sys_prm,
+                                                             BuildParams* __restrict__ prm) {
+    const uint32_t s = blockIdx.x;
+    float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t b = blk_off[s] + threadIdx.x; b < blk_off[s + 1]; b += kBlock)
+        r = minmax4(r, partials[b]);
+    r = block_minmax4(r);
+    if (threadIdx.x != 0) return;
+    fill_params(sys_prm + s, r, bh_sfp_from_f32(r.x), bh_sfp_from_f32(r.y), bh_sfp_from_f32(r.z),
+                bh_sfp_from_f32(r.w), kTreeQuadTree, sys_off[s + 1] - sys_off[s]);
+    if (s == 0) {
+        prm->n = sys_off[gridDim.x];
+        prm->max_level = 0;
+        prm->overfull_leaves = 0;
+    }
+}
+
+__device__ void fill_params(BuildParams* prm, float4 r, uint64_t mnx64, uint64_t mny64,
+                            uint64_t mxx64, uint64_t mxy64, uint32_t tree_mode, uint32_t n) {
     uint64_t side = (tree_mode == kTreeQuadTree) ? bh_sfp_quadtree_side(mnx64, mny64, mxx64, mxy64)
                                                  : bh_sfp_morton_extent(mnx64, mny64, mxx64, mxy64);
     if ((side & ~BH_SFP_SIGN) == 0) side = 0x3FF0000000000000ull;  // all points coincide: side 1
@@ -115,6 +159,31 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ xs,
     else
         code = (K)((bh_spread32(qy) << 1) | bh_spread32(qx));
     keys[i] = code;
+    vals[i] = i;
+}
+
+// batched systems: key = system << 2B | code, with each system's own root square
+__global__ __launch_bounds__(kBlock) void k_morton_seg(const float* __restrict__ xs,
+                                                       const float* __restrict__ ys, uint32_t n,
+                                                       const uint32_t* __restrict__ sys_off,
+                                                       uint32_t nsys,
+                                                       const BuildParams* __restrict__ sys_prm,
+                                                       int B, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = nsys;  // largest s with sys_off[s] <= i
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sys_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    const BuildParams* prm = sys_prm + lo;
+    const bh_sfp_quant Q = prm->quant;
+    const uint64_t dx = bh_sfp_add(bh_sfp_from_f32(xs[i]), prm->neg_minx64);
+    const uint64_t dy = bh_sfp_add(bh_sfp_from_f32(ys[i]), prm->neg_miny64);
+    const uint64_t qx = bh_sfp_quantize(dx, Q, B);
+    const uint64_t qy = bh_sfp_quantize(dy, Q, B);
+    keys[i] = ((uint64_t)lo << (2 * B)) | (bh_spread32(qy) << 1) | bh_spread32(qx);
     vals[i] = i;
 }
 
@@ -333,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ 
 // ------------------------------------------------------------------------------------------
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ codes, uint32_t n,
-                                                        int B, int cap, int fixed_depth,
+                                                        int B, int R, int cap, int fixed_depth,
                                                         uint8_t* __restrict__ leaf_lvl,
                                                         uint8_t* __restrict__ lcp1,
                                                         uint32_t* __restrict__ cnt) {
@@ -364,11 +433,12 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
     if (adaptive) {
         int m = -1;
         for (int t = 0; t <= cap; ++t) m = max(m, (int)s_w[threadIdx.x + t]);
-        leaf = min(m + 1, B);  // more than `cap` particles in the finest cell: stop at depth B
+        leaf = max(min(m + 1, B), R);  // more than `cap` in the finest cell: stop at depth B
     } else {
         leaf = fixed_depth;
     }
-    const int lp = (i == 0) ? -1 : lcp_level(codes[i - 1], codes[i], B);
+    // levels below R only separate batched systems: every system's tree starts at level R
+    const int lp = max((i == 0) ? -1 : lcp_level(codes[i - 1], codes[i], B), R - 1);
     leaf_lvl[i] = (uint8_t)leaf;
     lcp1[i] = (uint8_t)(lp + 1);
     cnt[i] = (leaf > lp) ? (uint32_t)(leaf - lp) : 0u;
@@ -440,10 +510,11 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit_heads(uint32_t n,
 // one thread per node: subtree extent, flags, side, parent and the parent's child slot
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_tree_emit_nodes(const K* __restrict__ codes,
-                                                            uint32_t n, int B,
+                                                            uint32_t n, int B, int R,
                                                             const uint8_t* __restrict__ lcp1,
                                                             const uint32_t* __restrict__ noff,
                                                             const BuildParams* __restrict__ prm,
+                                                            const BuildParams* __restrict__ sys_prm,
                                                             int single_child_pp,
                                                             Node* __restrict__ nodes,
                                                             uint32_t* __restrict__ parent,
@@ -457,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit_nodes(const K* __restrict_
     const int l = (int)(meta & kMetaLevelMask);
     const K ci = codes[i];
     uint32_t e;
-    if (l == 0) {
+    if (l == 0) {  // the single root of a one-system tree (R = 0)
         e = n;
     } else {
         const int sh = 2 * (B - l);
@@ -470,14 +541,15 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit_nodes(const K* __restrict_
         const int shc = 2 * (B - l - 1);
         if (((uint32_t)(ci >> shc) & 3u) == ((uint32_t)(codes[e - 1] >> shc) & 3u)) meta |= kMetaPP;
     }
-    const float sl = ldexpf(prm->side, -l);
+    const BuildParams* sp = R ? sys_prm + (uint32_t)(ci >> (2 * (B - R))) : prm;
+    const float sl = ldexpf(sp->side, -(l - R));
     Node* nd = nodes + k;
     nd->l2 = sl * sl;
     nd->skip = (e < n) ? noff[e] : total;
     nd->count = e - i;
     nd->meta = meta;
     uint32_t par = kNone;
-    if (l > 0) {
+    if (l > R) {
         if (!(meta & kMetaHead)) {
             par = k - 1;
         } else {
@@ -757,6 +829,15 @@ __global__ __launch_bounds__(kBlock) void k_up_top(Node* __restrict__ nodes,
     }
 }
 
+// batched systems: the root of system s is the first node of its first particle
+__global__ __launch_bounds__(kBlock) void k_batch_roots(const uint32_t* __restrict__ sys_off,
+                                                        uint32_t nsys,
+                                                        const uint32_t* __restrict__ noff,
+                                                        uint32_t* __restrict__ roots) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < nsys) roots[s] = sys_off[s] < sys_off[s + 1] ? noff[sys_off[s]] : kNone;
+}
+
 // tree statistics on request: deepest leaf level and leaves at the depth limit holding more
 // than the leaf capacity (one partial per block, summed by the host)
 __global__ __launch_bounds__(kBlock) void k_tree_stats(const Node* __restrict__ nodes,
@@ -904,25 +985,46 @@ void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float*
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
-                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes) {
+                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes, int R,
+                 const BuildParams* sys_prm) {
     const uint32_t g = div_up(n, kBlock);
     if (wide)
-        k_tree_levels<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)codes, n, B, cap,
+        k_tree_levels<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)codes, n, B, R, cap,
                                                      fixed_depth, leaf_lvl, lcp1, cnt);
     else
-        k_tree_levels<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)codes, n, B, cap,
+        k_tree_levels<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)codes, n, B, R, cap,
                                                      fixed_depth, leaf_lvl, lcp1, cnt);
     launch_exclusive_scan(s, cnt, noff, n, scan_partials, &prm->n_nodes);
     k_tree_emit_heads<<<g, kBlock, 0, s>>>(n, lcp1, cnt, noff, nodes, (uint4*)child4, cap_nodes);
     const uint32_t gn = div_up(cap_nodes, kBlock);
     if (wide)
-        k_tree_emit_nodes<uint64_t><<<gn, kBlock, 0, s>>>((const uint64_t*)codes, n, B, lcp1, noff,
-                                                          prm, single_child_pp, nodes, parent,
-                                                          child4, cap_nodes);
+        k_tree_emit_nodes<uint64_t><<<gn, kBlock, 0, s>>>((const uint64_t*)codes, n, B, R, lcp1,
+                                                          noff, prm, sys_prm, single_child_pp,
+                                                          nodes, parent, child4, cap_nodes);
     else
-        k_tree_emit_nodes<uint32_t><<<gn, kBlock, 0, s>>>((const uint32_t*)codes, n, B, lcp1, noff,
-                                                          prm, single_child_pp, nodes, parent,
-                                                          child4, cap_nodes);
+        k_tree_emit_nodes<uint32_t><<<gn, kBlock, 0, s>>>((const uint32_t*)codes, n, B, R, lcp1,
+                                                          noff, prm, sys_prm, single_child_pp,
+                                                          nodes, parent, child4, cap_nodes);
+}
+
+void launch_bounds_batched(hipStream_t s, const float* xs, const float* ys, const uint2* blk,
+                           uint32_t nblk, const uint32_t* blk_off, const uint32_t* sys_off,
+                           uint32_t nsys, float4* partials, BuildParams* sys_prm,
+                           BuildParams* prm) {
+    k_bounds_seg_partial<<<nblk, kBlock, 0, s>>>(xs, ys, blk, partials);
+    k_bounds_seg_final<<<nsys, kBlock, 0, s>>>(partials, blk_off, sys_off, sys_prm, prm);
+}
+
+void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint32_t n,
+                           const uint32_t* sys_off, uint32_t nsys, const BuildParams* sys_prm,
+                           int B, uint64_t* keys, uint32_t* vals) {
+    k_morton_seg<<<div_up(n, kBlock), kBlock, 0, s>>>(xs, ys, n, sys_off, nsys, sys_prm, B, keys,
+                                                     vals);
+}
+
+void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
+                        const uint32_t* noff, uint32_t* roots) {
+    k_batch_roots<<<div_up(nsys, kBlock), kBlock, 0, s>>>(sys_off, nsys, noff, roots);
 }
 
 uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }

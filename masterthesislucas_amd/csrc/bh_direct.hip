@@ -12,7 +12,9 @@ template <int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict__ pos4, uint32_t n,
                                                        const uint32_t* __restrict__ sinks,
                                                        uint32_t ns, double G, double eps2,
-                                                       double* __restrict__ part, uint32_t S) {
+                                                       double* __restrict__ part, uint32_t S,
+                                                       const uint32_t* __restrict__ sys_off,
+                                                       uint32_t nsys) {
     __shared__ double red[3][kWavesPerBlock][kWave];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t s = blockIdx.x * kWave + lane;
@@ -21,7 +23,17 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict_
     const float4 me = pos4[si];
     const double px = me.x, py = me.y;
     const uint32_t split = blockIdx.y;
-    const uint64_t lo = (uint64_t)n * split / S, hi = (uint64_t)n * (split + 1) / S;
+    uint64_t r0 = 0, r1 = n;  // the sink's own system (batched) or all particles
+    if (sys_off) {
+        uint32_t a = 0, b = nsys;
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (sys_off[mid] <= si) a = mid; else b = mid;
+        }
+        r0 = sys_off[a];
+        r1 = sys_off[a + 1];
+    }
+    const uint64_t lo = r0 + (r1 - r0) * split / S, hi = r0 + (r1 - r0) * (split + 1) / S;
     double sx = 0.0, sy = 0.0, st = 0.0;
     for (uint64_t j = lo + (uint64_t)w; j < hi; j += kWavesPerBlock) {
         const float4 q = pos4[j];
@@ -103,15 +115,23 @@ uint32_t direct_splits(uint32_t n, uint32_t ns) {
 void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
                        uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
-                       double* out) {
+                       double* out, const uint32_t* sys_off, uint32_t nsys) {
     k_invert<<<div_up(n, kBlock), kBlock, 0, s>>>(id, n, where);
     k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, sinks_caller, ns, sinks_sorted);
     dim3 g(div_up(ns, kWave), S);
     if (kernel == kKernelGravity)
-        k_direct_f64<kKernelGravity><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S);
+        k_direct_f64<kKernelGravity><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part,
+                                                          S, sys_off, nsys);
     else
-        k_direct_f64<kKernelTsne><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S);
+        k_direct_f64<kKernelTsne><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S,
+                                                       sys_off, nsys);
     k_direct_sum<<<div_up(ns, kBlock), kBlock, 0, s>>>(part, S, ns, out);
+}
+
+void launch_sinks_sorted(hipStream_t s, const uint32_t* id, uint32_t n, const uint32_t* sinks_caller,
+                         uint32_t ns, uint32_t* where, uint32_t* sinks_sorted) {
+    k_invert<<<div_up(n, kBlock), kBlock, 0, s>>>(id, n, where);
+    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, sinks_caller, ns, sinks_sorted);
 }
 
 }  // namespace bh

@@ -30,10 +30,21 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
 void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float* x, const float* y,
                    const float* m, const float* vx, const float* vy, const uint32_t* id, float* xo,
                    float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4);
+// R: level of the tree roots (0; batched systems: the system id occupies the levels above R)
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
-                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes);
+                 Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes, int R,
+                 const BuildParams* sys_prm);
+void launch_bounds_batched(hipStream_t s, const float* xs, const float* ys, const uint2* blk,
+                           uint32_t nblk, const uint32_t* blk_off, const uint32_t* sys_off,
+                           uint32_t nsys, float4* partials, BuildParams* sys_prm,
+                           BuildParams* prm);
+void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint32_t n,
+                           const uint32_t* sys_off, uint32_t nsys, const BuildParams* sys_prm,
+                           int B, uint64_t* keys, uint32_t* vals);
+void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
+                        const uint32_t* noff, uint32_t* roots);
 uint32_t up_chunks(uint32_t cap_nodes);
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
                    float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes);
@@ -49,10 +60,17 @@ void launch_tree_slots(hipStream_t s, const uint32_t* codes, const Node* nodes,
 
 constexpr int kTraverseVariants = 12;
 // returns the number of blocks launched (one counter partial each)
+// batched systems: one work entry (first sink, count, system, -) per wave and the systems'
+// roots; work == nullptr for a one-system context
+struct TraverseBatch {
+    const uint4* work = nullptr;
+    uint32_t n_work = 0;
+    const uint32_t* roots = nullptr;
+};
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                      const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                      float eps2, float scale, float* ax, float* ay, float* tot,
-                     unsigned long long* counters);
+                     unsigned long long* counters, TraverseBatch batch);
 uint32_t reduce_grid(uint32_t n);
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
                        double* out);
@@ -65,10 +83,11 @@ void launch_kick(hipStream_t s, float* vx, float* vy, const float* ax, const flo
                  float hdt);
 
 uint32_t direct_splits(uint32_t n, uint32_t ns);
+// sys_off (nsys + 1 offsets, may be null): sources restricted to the sink's own system
 void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
                        uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
-                       double* out);
+                       double* out, const uint32_t* sys_off = nullptr, uint32_t nsys = 0);
 
 void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
                        float* y, float* m, uint32_t* id);
@@ -82,5 +101,35 @@ void launch_tsne_kl(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
                     uint32_t n, const float* yx, const float* yy, const double* z,
                     double* partials, double* out);
 void launch_sum_f64(hipStream_t s, const double* v, uint32_t nv, double* out);
+
+// learned far-field correction (bh_mlp.hip)
+struct MlpBf16;
+void launch_mlp_prep(hipStream_t s, const float* params, MlpBf16* w);
+void launch_mlp_infer_bf16(hipStream_t s, const float* X, uint32_t n, const MlpBf16* w, float* Y);
+void launch_mlp_forward_f32(hipStream_t s, const float* X, uint32_t n, const float* p, float* Y);
+size_t mlp_scratch_floats(uint32_t B);
+uint32_t mlp_grad_chunks(uint32_t B);
+uint32_t mlp_loss_blocks(uint32_t B);
+void launch_mlp_grad(hipStream_t s, const float* X, const float* T, const uint32_t* order,
+                     uint32_t B, const float* p, float* S, double* loss_part, float* partials);
+void launch_mlp_adam(hipStream_t s, float* p, const float* partials, uint32_t nchunks, float* m1,
+                     float* m2, float lr, int step, float* grad_out, int apply);
+void launch_mlp_adam_grad(hipStream_t s, float* p, const float* g, float* m1, float* m2, float lr,
+                          int step);
+void launch_collect_pairs(hipStream_t s, const float4* pos4, const Node* nodes,
+                          const BuildParams* prm, const uint32_t* sinks, uint32_t ns,
+                          float inv_theta2, float inv_band2, uint32_t cap, uint32_t* pairs,
+                          uint32_t* counts);
+void launch_sinks_sorted(hipStream_t s, const uint32_t* id, uint32_t n, const uint32_t* sinks_caller,
+                         uint32_t ns, uint32_t* where, uint32_t* sinks_sorted);
+uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
+                                 const float4* m2, const BuildParams* prm, uint32_t n,
+                                 float inv_theta2, float inv_band2, float eps2, float scale,
+                                 const MlpBf16* w, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters, TraverseBatch batch);
+void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
+                        const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
+                        const uint32_t* offsets, uint32_t ns, uint32_t cap, double eps2, float* X,
+                        float* T, uint32_t total);
 
 }  // namespace bh

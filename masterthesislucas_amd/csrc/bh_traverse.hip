@@ -11,6 +11,7 @@
 // batch of at most kFoldEvery monopoles is one compensated addition.
 #include "bh_common.h"
 #include "bh_launch.h"
+#include "bh_mlp.h"
 
 namespace bh {
 
@@ -58,17 +59,59 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane(v);
 }
 
+// Node record as raw bits (cx, cy, mass, l2 | skip, first, count, meta), fetched with scalar
+// loads at a wave-uniform index.
+struct NodeBits {
+    uint4 a, b;
+    __device__ __forceinline__ float cx() const { return __uint_as_float(a.x); }
+    __device__ __forceinline__ float cy() const { return __uint_as_float(a.y); }
+    __device__ __forceinline__ float mass() const { return __uint_as_float(a.z); }
+    __device__ __forceinline__ float l2() const { return __uint_as_float(a.w); }
+    __device__ __forceinline__ uint32_t skip() const { return b.x; }
+    __device__ __forceinline__ uint32_t first() const { return b.y; }
+    __device__ __forceinline__ uint32_t count() const { return b.z; }
+    __device__ __forceinline__ uint32_t meta() const { return b.w; }
+};
+__device__ __forceinline__ NodeBits load_node(const Node* __restrict__ nodes, uint32_t k) {
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + k);
+    return NodeBits{p[0], p[1]};
+}
+// The sinks and the subtree a wave walks: one system of a one-system context (the whole tree,
+// 64 NS consecutive sinks per wave) or, for batched systems, a (first sink, count, system)
+// entry of the work table and that system's root.
+__device__ __forceinline__ void wave_work(const TraverseBatch& batch, uint32_t gw, uint32_t n,
+                                          int ns, const Node* __restrict__ nodes,
+                                          const BuildParams* __restrict__ prm, uint32_t& base,
+                                          uint32_t& count, uint32_t& k0, uint32_t& end0) {
+    if (batch.work) {
+        base = count = k0 = end0 = 0;
+        if (gw < batch.n_work) {
+            const uint4 wk = batch.work[gw];
+            const uint32_t r = uniform_u32(batch.roots[uniform_u32(wk.z)]);
+            if (r != kNone) {
+                base = uniform_u32(wk.x);
+                count = uniform_u32(wk.y);
+                k0 = r;
+                end0 = uniform_u32(nodes[r].skip);
+            }
+        }
+    } else {
+        base = gw * (uint32_t)(kWave * ns);
+        count = n > base ? min(n - base, (uint32_t)(kWave * ns)) : 0u;
+        k0 = 0;
+        end0 = uniform_u32(prm->n_nodes);
+    }
+}
+
 // Traversal variants (picked per workload by the autotuner):
 //   NS:      sinks per lane; a wave walks the tree once for 64 NS Morton-consecutive particles,
 //            so every node fetch feeds NS independent interactions per lane
 //   BRANCHY: monopoles only when some lane accepts (else computed masked, unconditionally)
 //   PPU:     sources fetched per scalar load in the particle-particle loop
-//
-// Per-lane state that is a function of lane masks (walking / accepting / opening) is held as
-// lane booleans, which live in scalar register pairs: the acceptance test is the only vector
-// work per node and sink besides the interaction itself. Interaction counts are wave totals
-// (popcounts of the acceptance masks).
-template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU>
+// BATCHED: the wave's sinks and subtree come from the work table of a batched context.
+// The node records for both possible next visits (descend: k + 1, skip: the subtree end) are
+// fetched while the current one is used.
+template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
                                                      const BuildParams* __restrict__ prm,
@@ -76,77 +119,88 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                                      float scale, float* __restrict__ ax_out,
                                                      float* __restrict__ ay_out,
                                                      float* __restrict__ tot_out,
-                                                     unsigned long long* __restrict__ counters) {
+                                                     unsigned long long* __restrict__ counters,
+                                                     TraverseBatch batch) {
     __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
     __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (wb * kWavesPerBlock + w) * (uint32_t)(kWave * NS);
+    uint32_t base, count = 0, k0 = 0, end0 = 0;
+    if constexpr (BATCHED) {
+        wave_work(batch, wb * kWavesPerBlock + w, n, NS, nodes, prm, base, count, k0, end0);
+    } else {
+        base = (wb * kWavesPerBlock + w) * (uint32_t)(kWave * NS);
+    }
     uint32_t idx[NS];
-    uint64_t onm[NS];  // lanes still walking the current subtree for sink s
+    bool valid[NS];
     float px[NS], py[NS];
+    uint64_t active[NS];
+    uint64_t any = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         idx[s] = base + s * kWave + lane;
-        const bool valid = idx[s] < n;
-        onm[s] = uniform_u64(__ballot(valid));
-        const float4 me = valid ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        valid[s] = BATCHED ? (uint32_t)(s * kWave + lane) < count : idx[s] < n;
+        const float4 me = valid[s] ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
         px[s] = me.x;
         py[s] = me.y;
+        active[s] = __ballot(valid[s]);
+        any |= active[s];
     }
-    const uint32_t nn = uniform_u32(prm->n_nodes);
+    const uint32_t nn = prm->n_nodes;
+    if constexpr (!BATCHED) end0 = nn;
 
     // Compensated totals (ax, ay, at) and their Kahan terms; monopoles of consecutive visits are
     // first gathered in a short plain batch (bx, by, bt) that is folded into the compensated
     // totals at every particle-particle visit and at least every kFoldEvery visits.
     float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
+    uint32_t npn[NS], npp[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] =
-        by[s] = bt[s] = 0.f;
-    uint64_t cnt_pn = 0, cnt_pp = 0;  // wave totals
-    uint32_t k = 0, end_cur = nn;
+    for (int s = 0; s < NS; ++s) {
+        ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] = by[s] = bt[s] = 0.f;
+        npn[s] = npp[s] = 0u;
+    }
+    uint32_t k = k0, end_cur = end0;
     int sp = 0, since_fold = 0;
-    for (;;) {
-        while (k >= end_cur && sp > 0) {
+    Node cur = nodes[min(k0, nn - 1)];
+    while (any) {
+        while (k == end_cur && sp > 0) {
             --sp;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) onm[s] = uniform_u64(st_mask[w][sp][s]);
-            end_cur = uniform_u32(st_end[w][sp]);
+            for (int s = 0; s < NS; ++s) active[s] = st_mask[w][sp][s];
+            end_cur = st_end[w][sp];
         }
         if (k >= end_cur) break;
-        uint64_t walking = 0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) walking |= onm[s];
-        if (!walking) {  // nobody left in this subtree frame
-            k = end_cur;
-            continue;
-        }
-        const Node cur = nodes[k];
         // skip > k for every node of a well-formed tree; max() keeps the walk finite regardless
         const uint32_t next = max(cur.skip, k + 1);
+        // the next node is k+1 (descend) or `next` (skip): fetch both while this one is used
+        const Node nd_down = nodes[min(k + 1, nn - 1)];
+        const Node nd_skip = nodes[min(next, nn - 1)];
         const float lim = cur.l2 * inv_theta2;
-        uint64_t openm[NS];
-        uint64_t open_any = 0, acc_any = 0;
+        uint64_t open[NS];
+        uint64_t open_any = 0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
+            const bool on = (active[s] >> lane) & 1ull;
             const float dx = cur.cx - px[s], dy = cur.cy - py[s];
             const float d2 = fmaf(dx, dx, dy * dy);
-            const uint64_t accm = __ballot(d2 > lim) & onm[s];
-            if (!BRANCHY || accm) {
-                const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
+            const bool acc = on && (d2 > lim);
+            if (!BRANCHY || __ballot(acc)) {
                 accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
-                cnt_pn += __popcll(accm);
+                npn[s] += acc ? 1u : 0u;
             }
-            openm[s] = onm[s] & ~accm;
-            open_any |= openm[s];
-            acc_any |= accm;
+            open[s] = __ballot(on && !acc);
+            open_any |= open[s];
         }
         bool down = false;
         if (open_any) {
             if (cur.meta & kMetaPP) {
+                bool mine[NS];
                 float lx[NS], ly[NS], lt[NS];
 #pragma unroll
-                for (int s = 0; s < NS; ++s) lx[s] = ly[s] = lt[s] = 0.f;
+                for (int s = 0; s < NS; ++s) {
+                    mine[s] = (open[s] >> lane) & 1ull;
+                    lx[s] = ly[s] = lt[s] = 0.f;
+                }
                 const uint32_t f = cur.first, e = f + cur.count;
                 // PPU sources per scalar fetch; pos4 is padded past n, lanes mask the tail
                 for (uint32_t j = f; j < e; j += PPU) {
@@ -159,16 +213,14 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         for (int s = 0; s < NS; ++s) {
                             const float qx = q[t].x - px[s], qy = q[t].y - py[s];
                             const float q2 = fmaf(qx, qx, qy * qy);
-                            const bool use = __builtin_amdgcn_inverse_ballot_w64(openm[s]) &&
-                                             (j + t < e) && (j + t != idx[s]);
+                            const bool use = mine[s] && (j + t < e) && (j + t != idx[s]);
                             accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s], ly[s], lt[s]);
                         }
                     }
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    const uint64_t self = __ballot(idx[s] >= f && idx[s] < e) & openm[s];
-                    cnt_pp += (uint64_t)__popcll(openm[s]) * cur.count - (uint64_t)__popcll(self);
+                    npp[s] += mine[s] ? cur.count - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
                     kahan_add(ax[s], cax[s], lx[s] + bx[s]);
                     kahan_add(ay[s], cay[s], ly[s] + by[s]);
                     if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
@@ -176,15 +228,18 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                 }
                 since_fold = 0;
             } else {
-                // lanes that accepted sit out the subtree: remember the frame, narrow the masks
-                if (acc_any && sp < kStackDepth) {
+                bool changed = false;
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (lane == 0) st_mask[w][sp][s] = onm[s];
-                        onm[s] = openm[s];
+                for (int s = 0; s < NS; ++s) changed |= (open[s] != active[s]);
+                if (changed && sp < kStackDepth) {
+                    if (lane == 0) {
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) st_mask[w][sp][s] = active[s];
+                        st_end[w][sp] = end_cur;
                     }
-                    if (lane == 0) st_end[w][sp] = end_cur;
                     ++sp;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) active[s] = open[s];
                     end_cur = next;
                 }
                 down = true;
@@ -201,19 +256,268 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             since_fold = 0;
         }
         k = down ? k + 1 : next;
+        cur = down ? nd_down : nd_skip;
     }
+    unsigned long long s_pn = 0, s_pp = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         kahan_add(ax[s], cax[s], bx[s]);
         kahan_add(ay[s], cay[s], by[s]);
         if (TOT) kahan_add(at[s], cat[s], bt[s]);
-        if (idx[s] < n) {
+        if (valid[s]) {
+            ax_out[idx[s]] = scale * (ax[s] - cax[s]);
+            ay_out[idx[s]] = scale * (ay[s] - cay[s]);
+            if (TOT) tot_out[idx[s]] = scale * (at[s] - cat[s]);
+        }
+        s_pn += valid[s] ? npn[s] : 0u;
+        s_pp += valid[s] ? npp[s] : 0u;
+    }
+    if (counters) {  // one partial per block (no same-address device atomics)
+        __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
+        s_pn = wave_sum_u64(s_pn);
+        s_pp = wave_sum_u64(s_pp);
+        if (lane == 0) {
+            s_cnt[w][0] = s_pn;
+            s_cnt[w][1] = s_pp;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long a = 0, b = 0;
+            for (int k2 = 0; k2 < kWavesPerBlock; ++k2) {
+                a += s_cnt[k2][0];
+                b += s_cnt[k2][1];
+            }
+            counters[2 * (size_t)wb] = a;
+            counters[2 * (size_t)wb + 1] = b;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Traversal with the learned far-field correction (gravity). The walk and every acceptance
+// decision are those of k_traverse; in addition, each accepted cell whose l / r is at least
+// `band` (all accepted cells when band = 0) gets its monopole corrected by the MLP. Lanes that
+// take such a cell write its features to a per-wave LDS ring of BF16 rows; whenever 64 rows
+// are pending, the wave evaluates the MLP on them with MFMA (mlp_wave_infer) and adds each
+// correction to its owner's monopole batch, walking the 64 results in ring order so the sum
+// order is fixed.
+// ------------------------------------------------------------------------------------------
+template <int NS, bool TOT>
+__global__ __launch_bounds__(kBlock) void k_traverse_learned(
+    const float4* __restrict__ pos4, const Node* __restrict__ nodes,
+    const float4* __restrict__ m2, const BuildParams* __restrict__ prm, uint32_t n,
+    float inv_theta2, float inv_band2, float eps2, float scale, const MlpBf16* __restrict__ wg,
+    float* __restrict__ ax_out, float* __restrict__ ay_out, float* __restrict__ tot_out,
+    unsigned long long* __restrict__ counters, TraverseBatch batch) {
+    constexpr int kRing = 64 * (NS + 1);  // pending < 64 before a visit adds at most 64 NS
+    __shared__ MlpBf16 ws;
+    __shared__ __attribute__((aligned(16))) short q_x[kWavesPerBlock][kRing * kMlpInPad];
+    __shared__ float q_scale[kWavesPerBlock][kRing];
+    __shared__ uint16_t q_owner[kWavesPerBlock][kRing];
+    __shared__ float2 q_out[kWavesPerBlock][64];
+    __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
+    __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&ws);
+        for (uint32_t i = threadIdx.x; i < sizeof(MlpBf16) / 4; i += kBlock) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t base, count, k0, end0;
+    wave_work(batch, wb * kWavesPerBlock + w, n, NS, nodes, prm, base, count, k0, end0);
+    uint32_t idx[NS];
+    uint64_t onm[NS];
+    float px[NS], py[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        idx[s] = base + s * kWave + lane;
+        const bool valid = (uint32_t)(s * kWave + lane) < count;
+        onm[s] = uniform_u64(__ballot(valid));
+        const float4 me = valid ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        px[s] = me.x;
+        py[s] = me.y;
+    }
+    float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] =
+        by[s] = bt[s] = 0.f;
+    uint64_t cnt_pn = 0, cnt_pp = 0;
+    uint32_t q_head = 0, q_tail = 0;  // ring positions (uniform)
+    short* xw = q_x[w];
+
+    // evaluate the 64 rows at q_head (all pending rows when `count` < 64; the rest are padding)
+    auto flush = [&](uint32_t count) {
+        const uint32_t h = q_head % kRing;
+        if (count < 64) {  // zero the padding rows so the MFMA reads finite values
+            for (uint32_t e = count; e < 64; ++e) {
+                if (lane < kMlpInPad) xw[(h + e) * kMlpInPad + lane] = 0;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        float o0[4], o1[4];
+        mlp_wave_infer(&ws, xw + h * kMlpInPad, o0, o1);
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const uint32_t e = 16 * sb + lane;
+            if (lane < 16) {
+                const float sc = q_scale[w][h + e];
+                q_out[w][e] = make_float2(o0[sb] * sc, o1[sb] * sc);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t e = 0; e < count; ++e) {
+            const uint32_t o = uniform_u32(q_owner[w][h + e]);
+            const float2 v = q_out[w][e];
+            if ((int)(o & 63u) == lane) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    if ((int)(o >> 6) == s) {
+                        bx[s] += v.x;
+                        by[s] += v.y;
+                    }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        q_head += 64;
+    };
+
+    const uint32_t last = uniform_u32(prm->n_nodes) - 1;
+    uint32_t k = k0, end_cur = end0;
+    int sp = 0, since_fold = 0;
+    for (;;) {
+        while (k >= end_cur && sp > 0) {
+            --sp;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) onm[s] = uniform_u64(st_mask[w][sp][s]);
+            end_cur = uniform_u32(st_end[w][sp]);
+        }
+        if (k >= end_cur) break;
+        uint64_t walking = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) walking |= onm[s];
+        if (!walking) {
+            k = end_cur;
+            continue;
+        }
+        const NodeBits cur = load_node(nodes, k);
+        const uint32_t next = uniform_u32(max(cur.skip(), k + 1));
+        const uint32_t meta = uniform_u32(cur.meta());
+        const float lim = cur.l2() * inv_theta2;
+        const float lim_band = cur.l2() * inv_band2;
+        uint64_t openm[NS];
+        uint64_t open_any = 0, acc_any = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const float dx = cur.cx() - px[s], dy = cur.cy() - py[s];
+            const float d2 = fmaf(dx, dx, dy * dy);
+            const uint64_t accm = uniform_u64(__ballot(d2 > lim) & onm[s]);
+            if (accm) {
+                const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
+                accum<kKernelGravity, TOT>(dx, dy, d2, cur.mass(), eps2, acc, bx[s], by[s], bt[s]);
+                cnt_pn += __popcll(accm);
+                const uint64_t bandm =
+                    uniform_u64(inv_band2 > 0.f ? (accm & __ballot(!(d2 > lim_band))) : accm);
+                if (bandm) {
+                    const float4 mm = m2[k];
+                    if (__builtin_amdgcn_inverse_ballot_w64(bandm)) {
+                        float f[kMlpIn];
+                        const float sc = mlp_features(dx, dy, d2, cur.l2(), cur.mass(),
+                                                      cur.count(), mm.x, mm.y, mm.z, f);
+                        const uint32_t pos =
+                            (q_tail + (uint32_t)__popcll(bandm & lane_mask_lt())) % kRing;
+                        bf16x4* row = reinterpret_cast<bf16x4*>(xw + pos * kMlpInPad);
+                        row[0] = pack_bf16x4(f[0], f[1], f[2], f[3]);
+                        row[1] = pack_bf16x4(f[4], f[5], f[6], f[7]);
+                        row[2] = pack_bf16x4(f[8], f[9], 1.0f, 0.f);
+                        row[3] = pack_bf16x4(0.f, 0.f, 0.f, 0.f);
+                        q_scale[w][pos] = sc * scale;
+                        q_owner[w][pos] = (uint16_t)(s * 64 + lane);
+                    }
+                    q_tail += (uint32_t)__popcll(bandm);
+                }
+            }
+            openm[s] = onm[s] & ~accm;
+            open_any |= openm[s];
+            acc_any |= accm;
+        }
+        while (q_tail - q_head >= 64) flush(64);
+        const bool is_pp = (meta & kMetaPP) != 0;
+        const bool down = open_any && !is_pp;
+        const uint32_t knext = uniform_u32(down ? k + 1 : next);
+        const uint32_t touch = nodes[min(knext, last)].meta;
+        if (open_any) {
+            if (is_pp) {
+                float lx[NS], ly[NS], lt[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) lx[s] = ly[s] = lt[s] = 0.f;
+                const uint32_t f0 = uniform_u32(cur.first()), cnt = uniform_u32(cur.count());
+                const uint32_t e0 = f0 + cnt;
+                bool mine[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) mine[s] = __builtin_amdgcn_inverse_ballot_w64(uniform_u64(openm[s]));
+                for (uint32_t j = f0; j < e0; ++j) {
+                    const float4 q = pos4[uniform_u32(j)];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const float qx = q.x - px[s], qy = q.y - py[s];
+                        const float q2 = fmaf(qx, qx, qy * qy);
+                        const bool use = mine[s] && j != idx[s];
+                        accum<kKernelGravity, TOT>(qx, qy, q2, q.z, eps2, use, lx[s], ly[s], lt[s]);
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const uint64_t self = __ballot(idx[s] >= f0 && idx[s] < e0) & openm[s];
+                    cnt_pp += (uint64_t)__popcll(openm[s]) * cnt - (uint64_t)__popcll(self);
+                    kahan_add(ax[s], cax[s], lx[s] + bx[s]);
+                    kahan_add(ay[s], cay[s], ly[s] + by[s]);
+                    if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
+                    bx[s] = by[s] = bt[s] = 0.f;
+                }
+                since_fold = 0;
+            } else {
+                if (acc_any && sp < kStackDepth) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (lane == 0) st_mask[w][sp][s] = onm[s];
+                        onm[s] = openm[s];
+                    }
+                    if (lane == 0) st_end[w][sp] = end_cur;
+                    ++sp;
+                    end_cur = next;
+                }
+            }
+        }
+        if (++since_fold >= kFoldEvery) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                kahan_add(ax[s], cax[s], bx[s]);
+                kahan_add(ay[s], cay[s], by[s]);
+                if (TOT) kahan_add(at[s], cat[s], bt[s]);
+                bx[s] = by[s] = bt[s] = 0.f;
+            }
+            since_fold = 0;
+        }
+        asm volatile("; prefetched %0" ::"s"(touch));
+        k = knext;
+    }
+    if (q_tail != q_head) flush(q_tail - q_head);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        kahan_add(ax[s], cax[s], bx[s]);
+        kahan_add(ay[s], cay[s], by[s]);
+        if (TOT) kahan_add(at[s], cat[s], bt[s]);
+        if ((uint32_t)(s * kWave + lane) < count) {
             ax_out[idx[s]] = scale * (ax[s] - cax[s]);
             ay_out[idx[s]] = scale * (ay[s] - cay[s]);
             if (TOT) tot_out[idx[s]] = scale * (at[s] - cat[s]);
         }
     }
-    if (counters) {  // one partial per block (no same-address device atomics)
+    if (counters) {
         __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
         if (lane == 0) {
             s_cnt[w][0] = cnt_pn;
@@ -311,16 +615,14 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 
 // variant v: sinks per lane {1, 2, 4}[v / 4]; monopoles branchy (v % 4 < 2) or masked;
 // particle-particle fetch of 1 (v even) or 4 (v odd) sources
-template <int KERNEL, bool TOT>
-static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const float4* pos4,
-                                 const Node* nodes, const BuildParams* prm, float inv_theta2,
-                                 float eps2, float scale, float* ax, float* ay, float* tot,
-                                 unsigned long long* counters) {
-    const int ns = variant < 4 ? 1 : (variant < 8 ? 2 : 4);
-    const uint32_t g = div_up(n, (uint64_t)kBlock * ns);
+template <int KERNEL, bool TOT, bool BATCHED>
+static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, const float4* pos4,
+                            const Node* nodes, const BuildParams* prm, float inv_theta2,
+                            float eps2, float scale, float* ax, float* ay, float* tot,
+                            unsigned long long* counters, TraverseBatch batch) {
 #define BH_TRAV(NS, BR, U)                                                                        \
-    k_traverse<KERNEL, TOT, NS, BR, U><<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2,     \
-                                                             eps2, scale, ax, ay, tot, counters)
+    k_traverse<KERNEL, TOT, NS, BR, U, BATCHED><<<g, kBlock, 0, s>>>(                             \
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch)
     switch (variant) {
         case 0: BH_TRAV(1, true, 1); break;
         case 1: BH_TRAV(1, true, 4); break;
@@ -336,26 +638,63 @@ static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const f
         default: BH_TRAV(4, false, 4); break;
     }
 #undef BH_TRAV
+}
+
+template <int KERNEL, bool TOT>
+static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const float4* pos4,
+                                 const Node* nodes, const BuildParams* prm, float inv_theta2,
+                                 float eps2, float scale, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters, TraverseBatch batch) {
+    const int ns = variant < 4 ? 1 : (variant < 8 ? 2 : 4);
+    const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
+                                  : div_up(n, (uint64_t)kBlock * ns);
+    if (batch.work)
+        traverse_launch<KERNEL, TOT, true>(s, variant, g, n, pos4, nodes, prm, inv_theta2, eps2,
+                                           scale, ax, ay, tot, counters, batch);
+    else
+        traverse_launch<KERNEL, TOT, false>(s, variant, g, n, pos4, nodes, prm, inv_theta2, eps2,
+                                            scale, ax, ay, tot, counters, batch);
     return g;
 }
 
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                          const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                          float eps2, float scale, float* ax, float* ay, float* tot,
-                         unsigned long long* counters) {
+                         unsigned long long* counters, TraverseBatch batch) {
     if (kernel == kKernelGravity) {
         if (tot)
             return traverse_variant<kKernelGravity, true>(s, variant, n, pos4, nodes, prm,
                                                           inv_theta2, eps2, scale, ax, ay, tot,
-                                                          counters);
+                                                          counters, batch);
         return traverse_variant<kKernelGravity, false>(s, variant, n, pos4, nodes, prm, inv_theta2,
-                                                       eps2, scale, ax, ay, tot, counters);
+                                                       eps2, scale, ax, ay, tot, counters, batch);
     }
     if (tot)
         return traverse_variant<kKernelTsne, true>(s, variant, n, pos4, nodes, prm, inv_theta2,
-                                                   eps2, scale, ax, ay, tot, counters);
+                                                   eps2, scale, ax, ay, tot, counters, batch);
     return traverse_variant<kKernelTsne, false>(s, variant, n, pos4, nodes, prm, inv_theta2, eps2,
-                                                scale, ax, ay, tot, counters);
+                                                scale, ax, ay, tot, counters, batch);
+}
+
+uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
+                                 const float4* m2, const BuildParams* prm, uint32_t n,
+                                 float inv_theta2, float inv_band2, float eps2, float scale,
+                                 const MlpBf16* w, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters, TraverseBatch batch) {
+    const int ns = variant < 4 ? 1 : 2;
+    const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
+                                  : div_up(n, (uint64_t)kBlock * ns);
+#define BH_TRAVL(NS, TOT)                                                                         \
+    k_traverse_learned<NS, TOT><<<g, kBlock, 0, s>>>(pos4, nodes, m2, prm, n, inv_theta2,        \
+                                                      inv_band2, eps2, scale, w, ax, ay, tot,     \
+                                                      counters, batch)
+    if (ns == 1) {
+        if (tot) BH_TRAVL(1, true); else BH_TRAVL(1, false);
+    } else {
+        if (tot) BH_TRAVL(2, true); else BH_TRAVL(2, false);
+    }
+#undef BH_TRAVL
+    return g;
 }
 
 uint32_t reduce_grid(uint32_t n) {

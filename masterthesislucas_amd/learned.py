@@ -1,0 +1,139 @@
+"""Learned far-field correction: the network object and the data-set / training / inference
+calls of the C ABI (include/bh.h, bh_mlp_* and bh_learned_*).
+
+No counterpart in the reference. The network is a 10 -> 64 -> 64 -> 2 SiLU MLP that predicts
+the residual between the exact sum over an accepted cell's particles and the cell's monopole
+(features and units: masterthesislucas_amd/csrc/bh_mlp.h). `FarFieldMLP.collect` labels pairs
+taken from a built gravity context by FP64 direct sums over the cells, `train` runs Adam on the
+GPU, and `BHContext.forces_learned` applies the network inside the traversal (BF16 MFMA).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .solver import BHContext, _ptr
+
+N_IN, N_HIDDEN, N_OUT = 10, 64, 2
+
+
+def param_layout():
+    """(name, shape) of each parameter block in the flat FP32 vector."""
+    return [("W1", (N_IN, N_HIDDEN)), ("b1", (N_HIDDEN,)), ("W2", (N_HIDDEN, N_HIDDEN)),
+            ("b2", (N_HIDDEN,)), ("W3", (N_HIDDEN, N_OUT)), ("b3", (N_OUT,))]
+
+
+def unflatten(flat):
+    out, o = {}, 0
+    for name, shape in param_layout():
+        size = int(np.prod(shape))
+        out[name] = np.asarray(flat[o:o + size]).reshape(shape)
+        o += size
+    return out
+
+
+def flatten(params):
+    return np.concatenate([np.asarray(params[name], np.float32).reshape(-1)
+                           for name, _ in param_layout()])
+
+
+class FarFieldMLP:
+    """Owner of one bh_mlp (weights, Adam state, data set of up to `max_samples` pairs)."""
+
+    def __init__(self, max_samples=1 << 21, seed=1952732):
+        L = _lib.load()
+        h = C.c_void_p()
+        st = L.bh_mlp_create(C.byref(h), int(max_samples), int(seed))
+        self._h = h.value
+        if st != _lib.BH_OK:
+            msg = self._err()
+            self.close()
+            raise _lib.BHError(st, msg)
+        self.count = int(L.bh_mlp_param_count())
+
+    def _err(self):
+        m = _lib.load().bh_mlp_last_error(self._h) if self._h else b""
+        return m.decode() if m else ""
+
+    def _c(self, st):
+        if st != _lib.BH_OK:
+            raise _lib.BHError(st, self._err())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().bh_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # parameters -------------------------------------------------------------------------
+    def get_params(self):
+        p = np.empty(self.count, np.float32)
+        self._c(_lib.load().bh_mlp_get_params(self._h, _ptr(p), self.count))
+        return p
+
+    def set_params(self, flat):
+        p = np.ascontiguousarray(flat, np.float32)
+        self._c(_lib.load().bh_mlp_set_params(self._h, _ptr(p), int(p.size)))
+
+    # compute ----------------------------------------------------------------------------
+    def forward(self, X, mfma=False):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], N_OUT), np.float32)
+        self._c(_lib.load().bh_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y), int(mfma)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.empty(self.count, np.float32)
+        loss = C.c_double(0)
+        self._c(_lib.load().bh_mlp_loss_grad(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(g),
+                                             C.byref(loss)))
+        return loss.value, g
+
+    def adam_step(self, grads, lr):
+        g = np.ascontiguousarray(grads, np.float32)
+        self._c(_lib.load().bh_mlp_adam_step(self._h, _ptr(g), float(lr)))
+
+    # data set and training ----------------------------------------------------------------
+    def dataset_size(self):
+        n = C.c_uint32(0)
+        self._c(_lib.load().bh_mlp_dataset_size(self._h, C.byref(n)))
+        return n.value
+
+    def dataset(self):
+        n = self.dataset_size()
+        X, T = np.empty((n, N_IN), np.float32), np.empty((n, N_OUT), np.float32)
+        self._c(_lib.load().bh_mlp_dataset_get(self._h, _ptr(X), _ptr(T), n))
+        return X, T
+
+    def set_dataset(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        self._c(_lib.load().bh_mlp_dataset_set(self._h, _ptr(X), _ptr(T), X.shape[0]))
+
+    def clear(self):
+        self._c(_lib.load().bh_mlp_dataset_clear(self._h))
+
+    def collect(self, ctx: BHContext, theta, sinks, band=0.0, max_pairs_per_sink=1024):
+        """Append the labelled pairs of `sinks` (caller order) from ctx's current tree."""
+        sk = np.ascontiguousarray(sinks, np.uint32)
+        added = C.c_uint32(0)
+        ctx._c(_lib.load().bh_learned_collect(ctx.handle, self._h, float(theta), float(band),
+                                              _ptr(sk), sk.size, int(max_pairs_per_sink),
+                                              C.byref(added)))
+        return added.value
+
+    def train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        loss = C.c_double(0)
+        self._c(_lib.load().bh_mlp_train(self._h, int(epochs), int(batch), float(lr), int(seed),
+                                         C.byref(loss)))
+        return loss.value

@@ -37,7 +37,9 @@ class BHContext:
     """Owner of one solver context of the C ABI (include/bh.h)."""
 
     def __init__(self, n_max, kernel="gravity", tree_mode="quadtree", max_children=16,
-                 fixed_depth=0, code_bits=32, G=1.0, softening=0.0, theta=0.5):
+                 fixed_depth=0, code_bits=32, G=1.0, softening=0.0, theta=0.5, sizes=None):
+        """One system of up to n_max particles, or, with `sizes`, len(sizes) independent
+        systems of those sizes stored one after the other (bh_create_batched)."""
         L = _lib.load()
         cfg = BHConfig()
         check(L.bh_config_default(C.byref(cfg)))
@@ -56,7 +58,11 @@ class BHContext:
         self.tree_mode = tree_mode
         self.n = 0
         h = C.c_void_p()
-        st = L.bh_create(C.byref(h), C.byref(cfg))
+        if sizes is None:
+            st = L.bh_create(C.byref(h), C.byref(cfg))
+        else:
+            sz = np.ascontiguousarray(sizes, np.uint32)
+            st = L.bh_create_batched(C.byref(h), C.byref(cfg), _ptr(sz), int(sz.size))
         self._h = h.value
         if st != _lib.BH_OK:
             msg = L.bh_last_error(self._h).decode() if self._h else ""
@@ -122,6 +128,33 @@ class BHContext:
         """Traversal only: results stay on the device (used for timing)."""
         self._c(_lib.load().bh_forces_device(self._h, float(theta), None, None, None))
 
+    def leapfrog_graph(self, steps, dt, theta):
+        """Leapfrog steps replayed from captured HIP graphs (same results as leapfrog)."""
+        self._c(_lib.load().bh_leapfrog_graph(self._h, int(steps), float(dt), float(theta)))
+
+    def batch_info(self):
+        """(number of systems, offsets[systems + 1]); (0, None) for a one-system context."""
+        ns = C.c_uint32(0)
+        self._c(_lib.load().bh_batch_info(self._h, C.byref(ns), None))
+        if ns.value == 0:
+            return 0, None
+        off = np.empty(ns.value + 1, np.uint32)
+        self._c(_lib.load().bh_batch_info(self._h, C.byref(ns), _ptr(off)))
+        return ns.value, off
+
+    def learned_enable(self, on=True):
+        """Keep cell second moments in the builds (the learned correction needs them)."""
+        self._c(_lib.load().bh_learned_enable(self._h, int(on)))
+
+    def forces_learned(self, mlp, theta, band=0.0, want_total=False):
+        """Forces with the learned correction on accepted cells with l/r >= band."""
+        ax, ay = np.empty(self.n, np.float32), np.empty(self.n, np.float32)
+        tot = C.c_double(0)
+        self._c(_lib.load().bh_forces_learned(self._h, mlp.handle, float(theta), float(band),
+                                              _ptr(ax), _ptr(ay),
+                                              C.byref(tot) if want_total else None))
+        return ax, ay, tot.value
+
     def leapfrog(self, steps, dt, theta):
         self._c(_lib.load().bh_leapfrog(self._h, int(steps), float(dt), float(theta)))
 
@@ -184,6 +217,12 @@ class BHContext:
 
     def set_variant(self, variant):
         self._c(_lib.load().bh_set_traverse_variant(self._h, int(variant)))
+        self._variant = int(variant)
+
+    @property
+    def variant(self):
+        """Traversal variant in use (the library starts with 4: two sinks per lane)."""
+        return getattr(self, "_variant", 4)
 
     @staticmethod
     def variant_count():

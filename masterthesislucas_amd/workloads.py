@@ -47,10 +47,12 @@ def uniform_disk(n: int, seed: int = SEED_DISK, radius: float = 1.0) -> Particle
     return Particles(_f32(out[:, 0]), _f32(out[:, 1]), np.full(n, 1.0 / n, np.float32), z, z.copy())
 
 
-def plummer_radii(rng, n: int, a: float) -> np.ndarray:
+def plummer_radii(rng, n: int, a: float, rmax: float = 0.0) -> np.ndarray:
     """Radii of a 2D Plummer disk, surface density ~ (1 + R^2/a^2)^-2, whose enclosed mass
-    fraction is R^2/(R^2+a^2): R = a sqrt(u/(1-u)), u ~ U[0,1). Not truncated."""
-    u = rng.random(n)
+    fraction is R^2/(R^2+a^2): R = a sqrt(u/(1-u)), u ~ U[0,1). rmax > 0 truncates at rmax
+    (u ~ U[0, umax) with umax = rmax^2 / (rmax^2 + a^2)); rmax = 0: not truncated."""
+    umax = rmax * rmax / (rmax * rmax + a * a) if rmax > 0 else 1.0
+    u = rng.random(n) * umax
     return a * np.sqrt(u / (1.0 - u))
 
 
@@ -77,10 +79,12 @@ def two_plummer_disks(n: int = 1 << 20, a: float = 0.1, sep: float = 0.5, G: flo
     return Particles(_f32(x), _f32(y), np.full(n, 1.0 / n, np.float32), _f32(vx), _f32(vy))
 
 
-def plummer(n: int = 1 << 22, a: float = 1.0, seed: int = SEED_PLUMMER) -> Particles:
-    """One 2D Plummer disk of scale radius a centred at the origin, zero velocity, masses 1/N."""
+def plummer(n: int = 1 << 22, a: float = 1.0, seed: int = SEED_PLUMMER,
+            rmax: float = 0.0) -> Particles:
+    """One 2D Plummer disk of scale radius a centred at the origin (truncated at rmax when
+    rmax > 0), zero velocity, masses 1/N."""
     rng = np.random.default_rng(seed)
-    R = plummer_radii(rng, n, a)
+    R = plummer_radii(rng, n, a, rmax)
     phi = rng.random(n) * 2.0 * np.pi
     z = np.zeros(n, np.float32)
     return Particles(_f32(R * np.cos(phi)), _f32(R * np.sin(phi)), np.full(n, 1.0 / n, np.float32),
@@ -94,15 +98,19 @@ def batched_sizes(systems: int = 256, nmin: int = 1024, nmax: int = 262144,
     return np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), systems))).astype(np.int64)
 
 
-def batched_systems(systems: int = 256, seed: int = SEED_BATCHED):
-    """Independent systems: even-numbered ones uniform disks, odd-numbered Plummer (a = 0.1)."""
+def batched_systems(systems: int = 256, seed: int = SEED_BATCHED, plummer_rmax: float = 3.0):
+    """Independent systems: even-numbered ones uniform disks, odd-numbered Plummer (a = 0.1)
+    truncated at R = plummer_rmax (30 a). Untruncated, a few far outliers per system set the
+    QuadTree root square; the cluster then sits under a chain of one-child cells, which the
+    reference rule (a cell with at most one child is summed particle by particle over all its
+    occupants, nBodySolverBH.h:68) turns into an all-pairs sum over the cluster."""
     sizes = batched_sizes(systems, seed=seed)
     out = []
     for k, nk in enumerate(sizes):
         if k % 2 == 0:
             out.append(uniform_disk(int(nk), seed=seed + 1 + k))
         else:
-            p = plummer(int(nk), a=0.1, seed=seed + 1 + k)
+            p = plummer(int(nk), a=0.1, seed=seed + 1 + k, rmax=plummer_rmax)
             out.append(p)
     return out
 

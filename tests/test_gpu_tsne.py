@@ -146,3 +146,21 @@ def test_tsne_rejects_bad_input(gpu):
     rp2[-1] += 1
     with pytest.raises(_lib.BHError):
         tsne.TSNE(rp2, col, val, (yx, yy))
+
+
+def test_direct_sum_between_steps_keeps_p_matrix(gpu):
+    # regression: the FP64 direct sum grows its own buffers and must leave P untouched
+    n = 2000
+    rp, col, val, yx, yy = _problem(n, seed=16)
+    a = tsne.TSNE(rp, col, val, (yx, yy))
+    b = tsne.TSNE(rp, col, val, (yx, yy))
+    try:
+        a.timeStep()
+        b.timeStep()
+        a.ctx.direct_fp64(np.arange(0, n, 3, dtype=np.uint32))
+        da = a.timeStep(return_derivative=True)
+        db = b.timeStep(return_derivative=True)
+        assert np.array_equal(da[0], db[0]) and np.array_equal(da[1], db[1]) and da[2] == db[2]
+    finally:
+        a.close()
+        b.close()
