@@ -52,11 +52,22 @@ def cpu_baseline(p, theta, eps, n):
         qt.forces(O.KERNEL_GRAVITY, G=1.0, m=1.0 / n, eps2=eps * eps, theta=theta, nthreads=1)
         secs = time.perf_counter() - t0
         kind = "port"
+    # all host cores (up to 16, this host's share of the machine): the oracle's restatement of
+    # the same QuadTree + traversal, threaded over sinks
+    cores = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    qt = O.QuadTree(x, y, 16)
+    qt.forces(O.KERNEL_GRAVITY, G=1.0, m=1.0 / n, eps2=eps * eps, theta=theta, nthreads=cores)
+    secs_all = time.perf_counter() - t0
     return {"value": n / secs, "unit": "force evals/s", "cores": 1, "kind": kind,
             "seconds": secs,
             "sample": f"one updateTree + solveNbody step (QuadTree, leaf capacity 16, "
                       f"theta={theta}) over all {n} particles of the benchmark's initial state, "
-                      f"FP64, single thread"}
+                      f"FP64, single thread",
+            "all_cores": {"value": n / secs_all, "cores": cores, "kind": "port",
+                          "seconds": secs_all,
+                          "sample": "the same step, tree build serial, traversal threaded over "
+                                    "sinks"}}
 
 
 def learned_comparison(args, eps):
@@ -187,6 +198,54 @@ def batched_comparison(args, eps, theta, dt):
             "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n}
 
 
+def tsne_run(args):
+    """t-SNE Barnes-Hut gradient descent, N = args.tsne_n: a seeded 10-component isotropic
+    Gaussian mixture in 784-D (means ~ N(0, 4 I), sigma 1), perplexity 30, exact k = 90 kNN,
+    symmetrised P in FP32, embedding N(0, 1e-4 I), early exaggeration 12 (48 with the gradient's
+    factor 4) for 250 of args.tsne_iters iterations then 4, theta 0.5, KL in FP64. The reference
+    publishes about 0.23 s per iteration (tree + repulsion, one CPU core, MNIST N = 70,000)."""
+    import torch
+
+    from masterthesislucas_amd import tsne as T
+    from masterthesislucas_amd.workloads import SEED_TSNE
+
+    n = args.tsne_n
+    t0 = time.perf_counter()
+    X, _ = T.gaussian_mixture(n, dim=784, components=10, seed=SEED_TSNE)
+    rp, col, val = T.joint_probabilities(X, perplexity=30.0, k=90, device="cuda")
+    del X
+    torch.cuda.empty_cache()
+    t_p = time.perf_counter() - t0
+    y0 = T.initial_embedding(n, seed=SEED_TSNE, std=1e-2)
+    run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
+    kl0, _ = run.costFunction()
+    run.run(2)  # warm
+    run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
+    run.ctx.profile_enable(True)
+    run.ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    run.run(args.tsne_iters)
+    wall = time.perf_counter() - t0
+    ph = run.ctx.profile_read(reset=True)
+    run.ctx.profile_enable(False)
+    pn, pp = run.ctx.counters()  # last iteration's repulsion (before the KL rebuilds the tree)
+    kl1, z = run.costFunction()
+    rep_ms = sum(ph[k][0] for k in ("bounds+morton", "sort", "gather", "tree", "upward",
+                                     "traversal")) / args.tsne_iters
+    out = {"workload": f"Gaussian mixture 10 x 784-D, N={n}, perplexity 30, kNN k=90, theta 0.5",
+           "n": n, "nnz_P": int(col.size), "p_matrix_s": t_p, "iterations": args.tsne_iters,
+           "ms_per_iteration": 1e3 * wall / args.tsne_iters,
+           "repulsion_ms_per_iteration": rep_ms,
+           "iterations_per_s": args.tsne_iters / wall,
+           "repulsion_force_evals_per_s": n / (rep_ms / 1e3),
+           "interactions_per_point": (pn + pp) / n,
+           "kl_initial": kl0, "kl_final": kl1, "z_final": z,
+           "reference_cpu_s_per_iteration_published": 0.23,
+           "speedup_vs_published_reference_repulsion": 0.23 / (rep_ms / 1e3)}
+    run.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +272,10 @@ def main():
                     help="skip the batched-systems run (256 systems, graph-captured steps)")
     ap.add_argument("--batched-systems", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--no-tsne", action="store_true",
+                    help="skip the t-SNE run (70,000 points, 1,000 iterations)")
+    ap.add_argument("--tsne-n", type=int, default=70000)
+    ap.add_argument("--tsne-iters", type=int, default=1000)
     ap.add_argument("--variant", type=int, default=-1,
                     help="traversal variant; -1 = autotune on the workload before timing")
     args = ap.parse_args()
@@ -346,6 +409,8 @@ def main():
         result["batched"] = batched_comparison(args, eps, theta, dt)
     if not args.no_learned:
         result["learned"] = learned_comparison(args, eps)
+    if not args.no_tsne:
+        result["tsne"] = tsne_run(args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (reference BH, one core)")
         result["cpu_baseline"] = cpu_baseline(p, theta, eps, n)
