@@ -219,8 +219,19 @@ def tsne_run(args):
     y0 = T.initial_embedding(n, seed=SEED_TSNE, std=1e-2)
     run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
     kl0, _ = run.costFunction()
-    run.run(2)  # warm
+    # warm, then tune the walk (variant, waves per sink group) on the tree of the embedding at
+    # the end of the run: the late, spread-out embedding has the most expensive walks
+    run.run(args.tsne_iters)
+    if args.tsne_split > 0:
+        best, tuning = (args.variant if args.variant >= 0 else 4, args.tsne_split), None
+    else:
+        from masterthesislucas_amd.autotune import tune_split
+        best, tuning = tune_split(run.ctx, 0.5)
+    log(f"[bench] t-SNE walk: variant {best[0]}, split {best[1]}")
+    run.close()
     run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
+    run.ctx.set_variant(best[0])
+    run.ctx.set_split(best[1])
     run.ctx.profile_enable(True)
     run.ctx.profile_read(reset=True)
     t0 = time.perf_counter()
@@ -236,9 +247,12 @@ def tsne_run(args):
            "n": n, "nnz_P": int(col.size), "p_matrix_s": t_p, "iterations": args.tsne_iters,
            "ms_per_iteration": 1e3 * wall / args.tsne_iters,
            "repulsion_ms_per_iteration": rep_ms,
+           "phase_ms_per_iteration": {k: round(v[0] / args.tsne_iters, 5) for k, v in ph.items()},
            "iterations_per_s": args.tsne_iters / wall,
            "repulsion_force_evals_per_s": n / (rep_ms / 1e3),
            "interactions_per_point": (pn + pp) / n,
+           "traversal_variant": best[0], "traversal_split": best[1],
+           "tuning_ms": ({k: round(v, 4) for k, v in tuning.items()} if tuning else None),
            "kl_initial": kl0, "kl_final": kl1, "z_final": z,
            "reference_cpu_s_per_iteration_published": 0.23,
            "speedup_vs_published_reference_repulsion": 0.23 / (rep_ms / 1e3)}
@@ -276,6 +290,8 @@ def main():
                     help="skip the t-SNE run (70,000 points, 1,000 iterations)")
     ap.add_argument("--tsne-n", type=int, default=70000)
     ap.add_argument("--tsne-iters", type=int, default=1000)
+    ap.add_argument("--tsne-split", type=int, default=0,
+                    help="waves per sink group in the t-SNE walk; 0 = tune (variant and split)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="traversal variant; -1 = autotune on the workload before timing")
     args = ap.parse_args()

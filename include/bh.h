@@ -242,13 +242,22 @@ bh_status bh_synchronize(bh_ctx* ctx);
 
 /* ---- autotuning: kernel variants of the traversal (no reference counterpart; the reference
  * GPU traversal has one fixed shape, nBodySolverGpuBH.cu:260-348) ----
- * variant v in [0, bh_traverse_variant_count()): v / 4 selects 1, 2 or 4 sink particles per
- * lane (a wave serves 64, 128 or 256 Morton-consecutive particles); (v % 4) / 2 selects
- * monopoles computed only when a lane accepts (0) or always, masked (1); v % 2 selects 1 or 4
- * sources per scalar fetch in the particle-particle loop. Results are identical up to FP32
- * summation order. */
+ * variant v in [0, bh_traverse_variant_count()). For v < 12: v / 4 selects 1, 2 or 4 sink
+ * particles per lane (a wave serves 64, 128 or 256 Morton-consecutive particles); (v % 4) / 2
+ * selects monopoles computed only when a lane accepts (0) or always, masked (1); v % 2 selects
+ * 1 or 4 sources per scalar fetch in the particle-particle loop. For 12 <= v < 18: (v - 12) / 2
+ * selects 1, 2 or 4 sinks per lane, (v - 12) % 2 the monopole mode as above, and the
+ * particle-particle sources of a cell are loaded by the lanes with one vector load (up to 64 at
+ * a time) and broadcast. Results are identical up to FP32 summation order. */
 int bh_traverse_variant_count(void);
 bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
+/* Tree-split walk for small N (no reference counterpart): `split` waves (1..8) share each sink
+ * group, wave f owning the pre-order node range [f nodes / split, (f+1) nodes / split); every
+ * interaction is evaluated by exactly one wave and the partial results are summed in wave
+ * order. Multiplies the waves of a launch when the sinks alone cannot fill the device.
+ * Results equal split 1 up to FP32 summation order; interaction counts are identical.
+ * Batched contexts accept only 1; bh_forces_learned always walks with split 1. */
+bh_status bh_set_traverse_split(bh_ctx* ctx, int split);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,7 @@ SIGNATURES = {
     "bh_synchronize": (_i32, [_vp]),
     "bh_traverse_variant_count": (_i32, []),
     "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
+    "bh_set_traverse_split": (_i32, [_vp, C.c_int]),
 }
 
 _lib = None
@@ -110,6 +111,14 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        # PyTorch ships its own HIP runtime (soname libamdhip64.so.7) and asks for it by the
+        # unversioned file name. Loaded after this library, it would come in as a second runtime
+        # next to /opt/rocm's, and the runtime initialised second sees no devices. Importing
+        # torch first makes this library bind to torch's copy: one runtime per process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

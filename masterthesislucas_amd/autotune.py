@@ -2,7 +2,9 @@
 
 The traversal has a few compile-time shapes (see bh_set_traverse_variant in include/bh.h):
 whether monopoles are computed only when a lane accepts or always (masked), and how many
-source particles each scalar fetch brings into the particle-particle loop. Which one is fastest
+source particles each scalar fetch brings into the particle-particle loop. For small N the walk
+can also be split over several waves per sink group (bh_set_traverse_split), which multiplies the
+waves of a launch at the price of re-reading the path to each wave's tree range. Which is fastest
 depends on the particle distribution, N and theta, so the tuner times every variant on the
 caller's own context (already holding the workload) with the library's HIP-event phase timers
 and keeps the fastest. Choices are remembered per (kernel, tree mode, log2 N, theta) in a JSON
@@ -60,3 +62,18 @@ def tune(ctx, theta, repeats=3, save=False, path=TABLE):
         with open(path, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     return best, times
+
+
+def tune_split(ctx, theta, splits=(1, 2, 4, 8), variants=None, repeats=3):
+    """Time every (variant, split) pair on ctx's current tree, select the fastest and return
+    ((variant, split), {"variant/split": ms})."""
+    variants = range(ctx.variant_count()) if variants is None else variants
+    times = {}
+    for s in splits:
+        ctx.set_split(s)
+        for v in variants:
+            times[(v, s)] = time_variant(ctx, v, theta, repeats)
+    best = min(times, key=times.get)
+    ctx.set_variant(best[0])
+    ctx.set_split(best[1])
+    return best, {f"{v}/{s}": ms for (v, s), ms in times.items()}

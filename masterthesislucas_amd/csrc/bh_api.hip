@@ -61,7 +61,7 @@ struct bh_ctx {
     float *out_a = nullptr, *out_b = nullptr, *out_c = nullptr, *out_d = nullptr;
     void *keys_a = nullptr, *keys_b = nullptr;
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
-    uint32_t* hist = nullptr;
+    uint32_t* sort_scratch = nullptr;  // radix sort counts, look-back words, totals, tickets
     uint32_t* scan_part = nullptr;
     uint8_t *leaf_lvl = nullptr, *lcp1 = nullptr;
     uint32_t *cnt = nullptr, *noff = nullptr;
@@ -92,6 +92,7 @@ struct bh_ctx {
     float* p_val = nullptr;
     uint32_t p_n = 0, p_nnz = 0;
     float *ty_x[3]{}, *ty_y[3]{};
+    float2* ty_xy = nullptr;  // interleaved copy of the current embedding (staged per iteration)
     int ty_cur = 0;  // ty[ty_cur] current, ty[(ty_cur+2)%3] previous
     bool ty_set = false;
     float *rep_x = nullptr, *rep_y = nullptr, *tder_x = nullptr, *tder_y = nullptr;
@@ -126,6 +127,9 @@ struct bh_ctx {
     int g_variant = -1;
 
     int variant = 4;  // traversal kernel variant (see launch_traverse)
+    uint32_t split = 1;         // waves per sink group in the traversal (one-system contexts)
+    float* part = nullptr;      // split traversal outputs: x, y, total parts, F n_max each
+    uint32_t part_split = 0;    // F the part buffer was sized for
     uint32_t trav_blocks = 0;  // blocks of the last traversal launch
     bool profile = false;
     PhaseTimer phases[BH_NUM_PHASES];
@@ -237,8 +241,8 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 1);
     bool in_b = false;
     const int key_bits = c->nsys ? 2 * c->qbits + 2 * c->R : 2 * c->B;
-    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, key_bits, c->wide, c->hist,
-                      c->scan_part, &in_b);
+    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, key_bits, c->wide, c->sort_scratch,
+                      &in_b);
     phase_end(c, 1);
     c->keys_in_b = in_b;
     const void* codes = in_b ? c->keys_b : c->keys_a;
@@ -287,12 +291,26 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     const float inv_theta2 = (float)(1.0 / (theta * theta));
     const float eps = c->cfg.softening;
     const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
+    // a split walk writes F parts of n per output, summed in part order afterwards
+    const uint32_t F = c->nsys ? 1u : c->split;
+    const size_t pn = (size_t)F * c->cfg.n_max;
+    float* ox = F > 1 ? c->part : c->ax;
+    float* oy = F > 1 ? c->part + pn : c->ay;
+    float* ot = F > 1 ? c->part + 2 * pn : c->tot;
     phase_begin(c, 5);
+    if (!c->nsys && (uint64_t)c->n * 16 <= (3ull << 20))  // small trees: warm every L2 first
+        launch_warm_l2(s, c->nodes, c->prm, c->pos4, c->n, c->vals_b);
     c->trav_blocks = launch_traverse(s, grav ? kKernelGravity : kKernelTsne, c->variant, c->pos4,
                                      c->nodes, c->prm, c->n, inv_theta2, eps * eps,
-                                     grav ? c->cfg.G : 1.0f, c->ax, c->ay,
-                                     want_total ? c->tot : nullptr, c->counters,
-                                     batch_of(c, c->variant < 4 ? 0 : (c->variant < 8 ? 1 : 2)));
+                                     grav ? c->cfg.G : 1.0f, ox, oy, want_total ? ot : nullptr,
+                                     c->counters,
+                                     batch_of(c, variant_ns_index(c->variant)),
+                                     F);
+    if (F > 1) {
+        launch_sum_parts(s, ox, F, c->n, c->ax);
+        launch_sum_parts(s, oy, F, c->n, c->ay);
+        if (want_total) launch_sum_parts(s, ot, F, c->n, c->tot);
+    }
     phase_end(c, 5);
     if (want_total)
         launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
@@ -333,6 +351,7 @@ void bh_destroy(bh_ctx* c) {
     if (c->p_col) (void)hipFree(c->p_col);
     if (c->p_val) (void)hipFree(c->p_val);
     if (c->dpart) (void)hipFree(c->dpart);
+    if (c->part) (void)hipFree(c->part);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -429,9 +448,8 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     }
     TRY(dalloc(c, &c->vals_a, N));
     TRY(dalloc(c, &c->vals_b, N));
-    const uint32_t nh = sort_blocks(N) * 256u;
-    TRY(dalloc(c, &c->hist, nh));
-    TRY(dalloc(c, &c->scan_part, std::max(scan_partials_size(N), scan_partials_size(nh)) + 256));
+    TRY(dalloc(c, &c->sort_scratch, sort_scratch_words(N)));
+    TRY(dalloc(c, &c->scan_part, scan_partials_size(N) + 256));
     TRY(dalloc(c, &c->leaf_lvl, N));
     TRY(dalloc(c, &c->lcp1, N));
     TRY(dalloc(c, &c->cnt, N));
@@ -445,7 +463,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     TRY(dalloc(c, &c->bounds_part, kMaxBoundsBlocks));
     TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
     TRY(dalloc(c, &c->dsum, 1));
-    c->max_blocks = traverse_blocks(N) + extra_blocks;
+    c->max_blocks = traverse_blocks(N) * kMaxSplit + extra_blocks;
     TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
     if (cfg->kernel == BH_KERNEL_TSNE) {
@@ -453,6 +471,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
             TRY(dalloc(c, &c->ty_x[k], N));
             TRY(dalloc(c, &c->ty_y[k], N));
         }
+        TRY(dalloc(c, &c->ty_xy, N));
         TRY(dalloc(c, &c->rep_x, N));
         TRY(dalloc(c, &c->rep_y, N));
         TRY(dalloc(c, &c->tder_x, N));
@@ -924,7 +943,7 @@ static bh_status tsne_stage_build(bh_ctx* c) {
     c->n = n;
     c->cur = 0;
     launch_tsne_stage(c->stream, c->ty_x[c->ty_cur], c->ty_y[c->ty_cur], n, c->x[0], c->y[0],
-                      c->m[0], c->id[0]);
+                      c->m[0], c->id[0], c->ty_xy);
     return do_build(c, nullptr);
 }
 
@@ -939,7 +958,8 @@ static bh_status tsne_iteration(bh_ctx* c, const bh_tsne_params* p, int32_t t, b
     const int cur = c->ty_cur, prev = (cur + 2) % 3, next = (cur + 1) % 3;
     const float exag = (float)(t < p->exag_switch ? p->exag_early : p->exag_late);
     const float mu = (float)(t < p->momentum_switch ? p->momentum_early : p->momentum_late);
-    launch_tsne_update(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[cur], c->ty_y[cur],
+    // ty_xy holds the current embedding: tsne_stage_build staged it for this iteration
+    launch_tsne_update(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[cur], c->ty_y[cur], c->ty_xy,
                        c->ty_x[prev], c->ty_y[prev], c->rep_x, c->rep_y, c->dsum, exag,
                        (float)p->learning_rate, mu, c->ty_x[next], c->ty_y[next],
                        keep_deriv ? c->tder_x : nullptr, keep_deriv ? c->tder_y : nullptr);
@@ -1190,6 +1210,32 @@ bh_status bh_set_traverse_variant(bh_ctx* c, int variant) {
 }
 
 int bh_traverse_variant_count(void) { return kTraverseVariants; }
+
+bh_status bh_set_traverse_split(bh_ctx* c, int split) {
+    if (!c) return BH_ERR_ARG;
+    if (split < 1 || split > (int)kMaxSplit)
+        return fail(c, BH_ERR_ARG, "traversal split %d outside [1, %u]", split, kMaxSplit);
+    if (c->nsys && split != 1)
+        return fail(c, BH_ERR_ARG, "batched contexts traverse with split 1");
+    if (split > 1 && (uint32_t)split > c->part_split) {
+        // grown in place of the old buffer: 3 outputs x F parts x n_max
+        bh_status st = hip_check(c, hipStreamSynchronize(c->stream), "split sync");
+        if (st != BH_OK) return st;
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        c->part_split = 0;
+        st = hip_check(c, hipMalloc(&c->part, 3ull * split * c->cfg.n_max * sizeof(float)),
+                       "hipMalloc split parts");
+        if (st != BH_OK) {
+            c->split = 1;
+            return st;
+        }
+        c->part_split = (uint32_t)split;
+    }
+    if ((uint32_t)split != c->split) c->g_valid = false;  // captured steps hold the old walk
+    c->split = (uint32_t)split;
+    return BH_OK;
+}
 
 bh_status bh_synchronize(bh_ctx* c) {
     if (!c) return BH_ERR_ARG;
@@ -1605,7 +1651,7 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
     c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->m2, c->prm, c->n,
                                              inv_t2, inv_b2, eps * eps, c->cfg.G, m->wbf, c->ax,
                                              c->ay, total ? c->tot : nullptr, c->counters,
-                                             batch_of(c, c->variant < 4 ? 0 : 1));
+                                             batch_of(c, variant_ns(c->variant) == 1 ? 0 : 1));
     phase_end(c, 5);
     if (total) launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
     c->have_acc = true;

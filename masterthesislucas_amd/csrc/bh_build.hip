@@ -265,61 +265,110 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 }
 
 // ------------------------------------------------------------------------------------------
-// LSD radix sort, 8-bit digits. Per pass: per-tile digit histogram -> device scan (digit-major)
-// -> stable scatter. The scatter ranks keys inside each wave with ballot match masks, then
-// stages the tile in LDS in digit order so global writes come out as contiguous digit runs.
+// LSD radix sort, 8-bit digits, one launch per pass after one counting launch ("one-sweep").
+//   k_radix_counts: one block per 4096-key tile; the tile's digit counts for every pass (the
+//     multiset of digits of a pass does not depend on the order) and the clearing of the tile's
+//     look-back words of every pass.
+//   k_radix_totals: per (pass, digit) the total over tiles.
+//   k_radix_pass: a block takes the next tile ticket, ranks its keys inside each wave with ballot
+//     match masks, publishes the tile's digit counts, and gets the tile's global digit offsets
+//     by decoupled look-back over the tiles ticketed before it; it then stages the tile in LDS
+//     in digit order so global writes come out as contiguous digit runs. The sort is stable.
+// A look-back word holds a 2-bit state (0 not ready, 1 tile count, 2 inclusive prefix) above a
+// 30-bit count, so one relaxed device-scope atomic carries both and no fence is needed. A block
+// only waits on tiles whose tickets were taken before its own (their blocks are running), so
+// every wait ends.
 // ------------------------------------------------------------------------------------------
 constexpr int kSortItems = 16;                      // keys per lane
 constexpr int kSortTile = kBlock * kSortItems;      // 4096 keys per block
 constexpr int kSortWaveKeys = kWave * kSortItems;   // 1024 keys per wave
+constexpr int kSortMaxPasses = 8;
+constexpr uint32_t kLbCount = 1u << 30, kLbPrefix = 2u << 30, kLbMask = kLbCount - 1;
+constexpr uint32_t kLbSpinLimit = 1u << 22;
+constexpr int kLbWindow = 8;  // look-back loads in flight per digit
+static_assert(kBlock == 256, "one thread per digit");
 
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, uint32_t n,
-                                                       int shift, uint32_t nblocks,
-                                                       uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kWavesPerBlock][256];
-    const int w = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < kWavesPerBlock * 256; d += kBlock) (&h[0][0])[d] = 0u;
+__global__ __launch_bounds__(kBlock) void k_radix_counts(const K* __restrict__ keys, uint32_t n,
+                                                         int passes, uint32_t ntiles,
+                                                         uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ status) {
+    __shared__ uint32_t h[kSortMaxPasses][256];
+    for (int i = threadIdx.x; i < kSortMaxPasses * 256; i += kBlock) (&h[0][0])[i] = 0u;
+    for (int p = 0; p < passes; ++p)
+        status[((size_t)p * ntiles + blockIdx.x) * 256 + threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
+    K k[kSortItems];  // all loads issued before the first use
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+        k[r] = i < n ? keys[i] : (K)0;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        if (base + r * kBlock + threadIdx.x >= n) continue;
+        for (int p = 0; p < passes; ++p)
+            atomicAdd(&h[p][(uint32_t)(k[r] >> (8 * p)) & 0xFFu], 1u);
     }
     __syncthreads();
-    const int d = threadIdx.x;
-    uint32_t s = 0;
-    for (int k = 0; k < kWavesPerBlock; ++k) s += h[k][d];
-    hist[(uint32_t)d * nblocks + blockIdx.x] = s;
+    for (int p = 0; p < passes; ++p)
+        counts[((size_t)p * 256 + threadIdx.x) * ntiles + blockIdx.x] = h[p][threadIdx.x];
+}
+
+// one wave per (pass, digit) row of counts; block 0 also clears the pass tickets
+__global__ __launch_bounds__(kBlock) void k_radix_totals(const uint32_t* __restrict__ counts,
+                                                         uint32_t ntiles, int passes,
+                                                         uint32_t* __restrict__ totals,
+                                                         uint32_t* __restrict__ tickets) {
+    if (blockIdx.x == 0 && threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0u;
+    const int lane = threadIdx.x & 63;
+    const uint32_t row = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (row >= (uint32_t)passes * 256u) return;
+    unsigned long long s = 0;
+    for (uint32_t j = lane; j < ntiles; j += kWave) s += counts[(size_t)row * ntiles + j];
+    s = wave_sum_u64(s);
+    if (lane == 0) totals[row] = (uint32_t)s;
 }
 
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ keys_in,
-                                                          const uint32_t* __restrict__ vals_in,
-                                                          K* __restrict__ keys_out,
-                                                          uint32_t* __restrict__ vals_out,
-                                                          uint32_t n, int shift, uint32_t nblocks,
-                                                          const uint32_t* __restrict__ offsets) {
+__global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ keys_in,
+                                                       const uint32_t* __restrict__ vals_in,
+                                                       K* __restrict__ keys_out,
+                                                       uint32_t* __restrict__ vals_out,
+                                                       uint32_t n, int shift,
+                                                       const uint32_t* __restrict__ totals,
+                                                       uint32_t* __restrict__ status,
+                                                       uint32_t* __restrict__ ticket,
+                                                       uint32_t* __restrict__ err) {
     __shared__ K s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wave[kWavesPerBlock][256];  // wave-local counts, then wave offsets
     __shared__ uint32_t s_start[256];                  // tile-local start of each digit
     __shared__ uint32_t s_base[256];                   // global start of each digit for this tile
+    __shared__ uint32_t s_tile;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     for (int d = threadIdx.x; d < kWavesPerBlock * 256; d += kBlock) (&s_wave[0][0])[d] = 0u;
     __syncthreads();
+    const uint32_t tile = s_tile;
 
-    const uint32_t wbase = blockIdx.x * kSortTile + w * kSortWaveKeys;
+    const uint32_t wbase = tile * kSortTile + w * kSortWaveKeys;
     const uint64_t lt = lane_mask_lt();
     K key[kSortItems];
     uint32_t val[kSortItems];
     uint32_t rank[kSortItems];
+    // all loads are issued before the ranking (whose LDS traffic would otherwise make each
+    // item wait for its own load)
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = wbase + r * kWave + lane;
-        const bool valid = i < n;
-        key[r] = valid ? keys_in[i] : (K)0;
-        val[r] = valid ? vals_in[i] : 0u;
+        key[r] = i < n ? keys_in[i] : (K)0;
+        val[r] = i < n ? vals_in[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const bool valid = wbase + r * kWave + lane < n;
         const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -336,16 +385,54 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     __syncthreads();
     {
         const int d = threadIdx.x;
-        uint32_t run = 0;
+        uint32_t cnt = 0;
         for (int k = 0; k < kWavesPerBlock; ++k) {
             const uint32_t c = s_wave[k][d];
-            s_wave[k][d] = run;
-            run += c;
+            s_wave[k][d] = cnt;
+            cnt += c;
         }
         uint32_t tot;
-        const uint32_t start = block_exclusive_scan(run, &tot);
-        s_start[d] = start;
-        s_base[d] = offsets[(uint32_t)d * nblocks + blockIdx.x];
+        s_start[d] = block_exclusive_scan(cnt, &tot);
+        uint32_t* mine = status + (size_t)tile * 256 + d;
+        uint32_t excl = 0;
+        if (tile == 0) {  // block-uniform: the first tile starts at the digit's global offset
+            excl = block_exclusive_scan(totals[d], &tot);
+        } else {
+            __hip_atomic_store(mine, kLbCount | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // look back kLbWindow tiles per round with independent loads, nearest first; stop at
+            // the first inclusive prefix (tile 0 always publishes one), wait at a tile not ready
+            int j = (int)tile - 1;
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done) {
+                uint32_t v[kLbWindow];
+#pragma unroll
+                for (int k = 0; k < kLbWindow; ++k)
+                    v[k] = j - k >= 0 ? __hip_atomic_load(status + (size_t)(j - k) * 256 + d,
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : kLbPrefix;
+                int used = 0;
+#pragma unroll
+                for (int k = 0; k < kLbWindow; ++k) {
+                    if (done || used < k) continue;  // stopped earlier in this round
+                    if ((v[k] & ~kLbMask) == 0u) continue;  // not ready: poll again from here
+                    excl += v[k] & kLbMask;
+                    done = (v[k] & kLbPrefix) != 0u;
+                    used = k + 1;
+                }
+                j -= used;
+                if (!done && used < kLbWindow) {
+                    if (++spins > kLbSpinLimit) {  // never expected: record it and stop waiting
+                        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __hip_atomic_store(mine, kLbPrefix | (excl + cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        s_base[d] = excl;
     }
     __syncthreads();
 #pragma unroll
@@ -357,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
         s_vals[pos] = val[r];
     }
     __syncthreads();
-    const uint32_t tile_n = min((uint32_t)kSortTile, n - blockIdx.x * kSortTile);
+    const uint32_t tile_n = min((uint32_t)kSortTile, n - tile * kSortTile);
     for (uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
         const K k = s_keys[p];
         const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
@@ -946,29 +1033,45 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
 
 uint32_t sort_blocks(uint32_t n) { return div_up(n, kSortTile); }
 
+// counts [8][256][tiles] + look-back words [8][tiles][256] + totals [8][256] + tickets and an
+// error word
+uint64_t sort_scratch_words(uint32_t n) {
+    return 2ull * kSortMaxPasses * 256 * sort_blocks(n) + kSortMaxPasses * 256 + 16;
+}
+
 void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys_b,
-                       uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* hist,
-                       uint32_t* scan_partials, bool* result_in_b) {
-    const uint32_t nb = sort_blocks(n);
-    const uint32_t nh = nb * 256u;
+                       uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* scratch,
+                       bool* result_in_b) {
+    const uint32_t nt = sort_blocks(n);
     const int passes = (key_bits + 7) / 8;
+    uint32_t* counts = scratch;
+    uint32_t* status = counts + (size_t)kSortMaxPasses * 256 * nt;
+    uint32_t* totals = status + (size_t)kSortMaxPasses * 256 * nt;
+    uint32_t* tickets = totals + kSortMaxPasses * 256;
+    uint32_t* err = tickets + kSortMaxPasses;
+    if (wide)
+        k_radix_counts<uint64_t><<<nt, kBlock, 0, s>>>((const uint64_t*)keys_a, n, passes, nt,
+                                                       counts, status);
+    else
+        k_radix_counts<uint32_t><<<nt, kBlock, 0, s>>>((const uint32_t*)keys_a, n, passes, nt,
+                                                       counts, status);
+    k_radix_totals<<<div_up(passes * 256, kWavesPerBlock), kBlock, 0, s>>>(counts, nt, passes,
+                                                                            totals, tickets);
     void* kin = keys_a;
     uint32_t* vin = vals_a;
     void* kout = keys_b;
     uint32_t* vout = vals_b;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
+        uint32_t* st = status + (size_t)p * nt * 256;
+        const uint32_t* tp = totals + p * 256;
         if (wide)
-            k_radix_hist<uint64_t><<<nb, kBlock, 0, s>>>((const uint64_t*)kin, n, shift, nb, hist);
+            k_radix_pass<uint64_t><<<nt, kBlock, 0, s>>>((const uint64_t*)kin, vin,
+                                                         (uint64_t*)kout, vout, n, 8 * p, tp, st,
+                                                         tickets + p, err);
         else
-            k_radix_hist<uint32_t><<<nb, kBlock, 0, s>>>((const uint32_t*)kin, n, shift, nb, hist);
-        launch_exclusive_scan(s, hist, hist, nh, scan_partials, nullptr);
-        if (wide)
-            k_radix_scatter<uint64_t><<<nb, kBlock, 0, s>>>((const uint64_t*)kin, vin,
-                                                            (uint64_t*)kout, vout, n, shift, nb, hist);
-        else
-            k_radix_scatter<uint32_t><<<nb, kBlock, 0, s>>>((const uint32_t*)kin, vin,
-                                                            (uint32_t*)kout, vout, n, shift, nb, hist);
+            k_radix_pass<uint32_t><<<nt, kBlock, 0, s>>>((const uint32_t*)kin, vin,
+                                                         (uint32_t*)kout, vout, n, 8 * p, tp, st,
+                                                         tickets + p, err);
         void* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }

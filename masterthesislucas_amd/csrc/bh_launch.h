@@ -24,9 +24,10 @@ uint32_t scan_partials_size(uint64_t n);
 void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n,
                            uint32_t* partials, uint32_t* total);
 uint32_t sort_blocks(uint32_t n);
+uint64_t sort_scratch_words(uint32_t n);  // uint32 words of scratch launch_radix_sort needs
 void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys_b,
-                       uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* hist,
-                       uint32_t* scan_partials, bool* result_in_b);
+                       uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* scratch,
+                       bool* result_in_b);
 void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float* x, const float* y,
                    const float* m, const float* vx, const float* vy, const uint32_t* id, float* xo,
                    float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4);
@@ -58,7 +59,20 @@ void launch_tree_slots(hipStream_t s, const uint32_t* codes, const Node* nodes,
                        const BuildParams* prm, uint32_t cap_nodes, int depth, uint32_t* first_child,
                        uint32_t* first_particle, uint32_t* count, uint64_t slots);
 
-constexpr int kTraverseVariants = 12;
+// Traversal variants: sinks per lane NS in {1, 2, 4}, monopoles evaluated only when some lane
+// accepts (branchy) or always under a mask, and the particle-particle source fetch: PPU sources
+// per scalar load (1 or 4), or PPU = 0: each lane loads one of up to 64 sources with one vector
+// load and the wave broadcasts them with readlane (one memory round trip per cell).
+// Variants 0..11: NS = {1,2,4}[v / 4], branchy = (v % 4) < 2, PPU = v % 2 ? 4 : 1;
+// variants 12..17: NS = {1,2,4}[(v - 12) / 2], branchy = (v - 12) % 2 == 0, PPU = 0.
+constexpr int kTraverseVariants = 18;
+inline int variant_ns(int v) {
+    const int i = v < 12 ? v / 4 : (v - 12) / 2;
+    return 1 << i;
+}
+inline int variant_ns_index(int v) { return v < 12 ? v / 4 : (v - 12) / 2; }
+inline bool variant_branchy(int v) { return v < 12 ? (v % 4) < 2 : (v - 12) % 2 == 0; }
+inline int variant_ppu(int v) { return v < 12 ? (v % 2 ? 4 : 1) : 0; }
 // returns the number of blocks launched (one counter partial each)
 // batched systems: one work entry (first sink, count, system, -) per wave and the systems'
 // roots; work == nullptr for a one-system context
@@ -70,11 +84,17 @@ struct TraverseBatch {
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                      const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                      float eps2, float scale, float* ax, float* ay, float* tot,
-                     unsigned long long* counters, TraverseBatch batch);
+                     unsigned long long* counters, TraverseBatch batch, uint32_t F = 1);
+// F > 1 (one-system contexts): F waves per sink group, outputs are F parts of n each
+constexpr uint32_t kMaxSplit = 8;
+void launch_sum_parts(hipStream_t s, const float* part, uint32_t F, uint32_t n, float* out);
 uint32_t reduce_grid(uint32_t n);
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
                        double* out);
 uint32_t traverse_blocks(uint32_t n);
+// streams the node records and sorted positions into every XCD's L2 when they fit (else no-op)
+void launch_warm_l2(hipStream_t s, const Node* nodes, const BuildParams* prm, const float4* pos4,
+                    uint32_t n, uint32_t* sink);
 void launch_unsort2(hipStream_t s, const uint32_t* id, uint32_t n, const float* a, const float* b,
                     float* ao, float* bo);
 void launch_kick_drift(hipStream_t s, float* x, float* y, float* vx, float* vy, const float* ax,
@@ -89,13 +109,14 @@ void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
                        double* out, const uint32_t* sys_off = nullptr, uint32_t nsys = 0);
 
+// stage also writes the interleaved copy xy that the update's neighbour gathers read
 void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
-                       float* y, float* m, uint32_t* id);
+                       float* y, float* m, uint32_t* id, float2* xy);
 void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
                         const float* val, uint32_t n, const float* yx, const float* yy,
-                        const float* pyx, const float* pyy, const float* rep_x,
-                        const float* rep_y, const double* z, float exag, float lr, float mu,
-                        float* nyx, float* nyy, float* dx, float* dy);
+                        const float2* xy, const float* pyx, const float* pyy,
+                        const float* rep_x, const float* rep_y, const double* z, float exag,
+                        float lr, float mu, float* nyx, float* nyy, float* dx, float* dy);
 uint32_t tsne_kl_blocks(uint32_t n);
 void launch_tsne_kl(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col, const float* val,
                     uint32_t n, const float* yx, const float* yy, const double* z,

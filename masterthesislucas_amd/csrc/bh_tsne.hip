@@ -16,11 +16,14 @@ __global__ __launch_bounds__(kBlock) void k_tsne_stage(const float* __restrict__
                                                        const float* __restrict__ yy, uint32_t n,
                                                        float* __restrict__ x, float* __restrict__ y,
                                                        float* __restrict__ m,
-                                                       uint32_t* __restrict__ id) {
+                                                       uint32_t* __restrict__ id,
+                                                       float2* __restrict__ xy) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    x[i] = yx[i];
-    y[i] = yy[i];
+    const float a = yx[i], b = yy[i];
+    x[i] = a;
+    y[i] = b;
+    xy[i] = make_float2(a, b);
     m[i] = 1.0f;
     id[i] = i;
 }
@@ -28,12 +31,15 @@ __global__ __launch_bounds__(kBlock) void k_tsne_stage(const float* __restrict__
 // One wave per row i: attraction exag * sum_j P_ij w_ij (y_j - y_i) with w = 1/(1+|y_j-y_i|^2)
 // (updateAttractive, tsne.h:356-381), derivative d_i = (4/Z) rep_i + attraction
 // (updateRepulsive scaling, tsne.h:291-294), then y' = y + lr d + mu (y - y_prev)
-// (updatePoints, tsne.h:179-206).
+// (updatePoints, tsne.h:179-206). The neighbours' positions are gathered from the interleaved
+// copy xy (one 8-byte load per non-zero instead of two 4-byte ones: the gathers, not the bytes,
+// bound this kernel).
 __global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restrict__ row_ptr,
                                                         const uint32_t* __restrict__ col,
                                                         const float* __restrict__ val, uint32_t n,
                                                         const float* __restrict__ yx,
                                                         const float* __restrict__ yy,
+                                                        const float2* __restrict__ xy,
                                                         const float* __restrict__ pyx,
                                                         const float* __restrict__ pyy,
                                                         const float* __restrict__ rep_x,
@@ -51,8 +57,8 @@ __global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restri
     float sx = 0.f, sy = 0.f;
     const uint32_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
     for (uint32_t e = e0 + lane; e < e1; e += kWave) {
-        const uint32_t c = col[e];
-        const float ddx = yx[c] - xi, ddy = yy[c] - yi;
+        const float2 yc = xy[col[e]];
+        const float ddx = yc.x - xi, ddy = yc.y - yi;
         const float q = __builtin_amdgcn_rcpf(1.0f + fmaf(ddx, ddx, ddy * ddy));
         const float s = exag * val[e] * q;
         sx = fmaf(s, ddx, sx);
@@ -126,18 +132,18 @@ __global__ __launch_bounds__(kBlock) void k_sum_f64(const double* __restrict__ v
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
-                       float* y, float* m, uint32_t* id) {
-    k_tsne_stage<<<div_up(n, kBlock), kBlock, 0, s>>>(yx, yy, n, x, y, m, id);
+                       float* y, float* m, uint32_t* id, float2* xy) {
+    k_tsne_stage<<<div_up(n, kBlock), kBlock, 0, s>>>(yx, yy, n, x, y, m, id, xy);
 }
 
 void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
                         const float* val, uint32_t n, const float* yx, const float* yy,
-                        const float* pyx, const float* pyy, const float* rep_x,
-                        const float* rep_y, const double* z, float exag, float lr, float mu,
-                        float* nyx, float* nyy, float* dx, float* dy) {
-    k_tsne_update<<<div_up(n, kWavesPerBlock), kBlock, 0, s>>>(row_ptr, col, val, n, yx, yy, pyx,
-                                                               pyy, rep_x, rep_y, z, exag, lr, mu,
-                                                               nyx, nyy, dx, dy);
+                        const float2* xy, const float* pyx, const float* pyy,
+                        const float* rep_x, const float* rep_y, const double* z, float exag,
+                        float lr, float mu, float* nyx, float* nyy, float* dx, float* dy) {
+    k_tsne_update<<<div_up(n, kWavesPerBlock), kBlock, 0, s>>>(row_ptr, col, val, n, yx, yy, xy,
+                                                               pyx, pyy, rep_x, rep_y, z, exag, lr,
+                                                               mu, nyx, nyy, dx, dy);
 }
 
 uint32_t tsne_kl_blocks(uint32_t n) {

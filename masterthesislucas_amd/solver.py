@@ -224,6 +224,15 @@ class BHContext:
         """Traversal variant in use (the library starts with 4: two sinks per lane)."""
         return getattr(self, "_variant", 4)
 
+    def set_split(self, split):
+        """Waves per sink group in the traversal (1..8); see bh_set_traverse_split."""
+        self._c(_lib.load().bh_set_traverse_split(self._h, int(split)))
+        self._split = int(split)
+
+    @property
+    def split(self):
+        return getattr(self, "_split", 1)
+
     @staticmethod
     def variant_count():
         return int(_lib.load().bh_traverse_variant_count())

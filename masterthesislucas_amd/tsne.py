@@ -169,6 +169,8 @@ class TSNE:
         self.ctx._c(L.bh_tsne_set_p(self.ctx.handle, n, _ptr(row_ptr), _ptr(col), _ptr(val),
                                     int(col.size)))
         self.ctx._c(L.bh_tsne_set_embedding(self.ctx.handle, _ptr(yx), _ptr(yy)))
+        # the context now holds the n embedding points: its per-particle reads size to n
+        self.ctx.n = n
 
     @property
     def learnRate(self):

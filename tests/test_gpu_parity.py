@@ -42,6 +42,35 @@ def test_morton32_codes_and_order_bitexact(gpu):
     assert tuple(b[:2]) == t["bounds"][:2]
 
 
+@pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
+@pytest.mark.parametrize("n", [1, 4095, 4097, 1_000_003])
+def test_sort_many_tiles_with_duplicates(gpu, mode, n):
+    """The one-sweep radix sort over many 4096-key tiles (ragged last tile): codes and order
+    bit-exact against the oracle's stable sort, with a third of the points on 7 spots so that
+    equal codes must keep their input order across tiles."""
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    y = rng.uniform(-1, 1, n).astype(np.float32)
+    dup = rng.random(n) < 1 / 3
+    spot = rng.integers(0, 7, n)
+    x[dup] = (spot[dup] * 0.25 - 0.75).astype(np.float32)
+    y[dup] = (spot[dup] * -0.2 + 0.5).astype(np.float32)
+    ctx = _ctx(n, kernel="gravity", tree_mode=mode, softening=EPS)
+    ctx.set_particles(x, y)
+    ctx.build()
+    codes, perm = ctx.morton()
+    xd, yd = x.astype(np.float64), y.astype(np.float64)
+    minx, miny, maxx, maxy = xd.min(), yd.min(), xd.max(), yd.max()
+    if mode == "quadtree":
+        ref = O.morton64(xd, yd, minx, miny, O.quadtree_side(minx, miny, maxx, maxy), 32)
+    else:
+        ref = O.morton32(xd, yd, minx, miny, max(maxx + (-minx), maxy + (-miny))).astype(np.uint64)
+    rperm = O.stable_sort(ref)
+    assert np.array_equal(perm, rperm)
+    assert np.array_equal(codes, ref[rperm])
+    ctx.close()
+
+
 def test_morton32_caller_bounds(gpu):
     """updateTree honours the caller's minPos/maxPos in the Morton solvers."""
     p = uniform_disk(4096, seed=3)
@@ -311,7 +340,7 @@ def test_rebuild_reuses_context(gpu):
         assert np.array_equal(ax, fx) and np.array_equal(ay, fy)
 
 
-@pytest.mark.parametrize("variant", range(12))
+@pytest.mark.parametrize("variant", range(18))
 @pytest.mark.parametrize("kernel", ["gravity", "tsne"])
 def test_every_traversal_variant(gpu, variant, kernel):
     """All autotuner choices give the oracle's forces (they differ only in summation order)."""

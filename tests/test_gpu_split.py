@@ -1,0 +1,134 @@
+"""Tree-split traversal (bh_set_traverse_split): F waves share a sink group, each owning a
+pre-order range of the tree, and the F partial results are summed in wave order.
+
+Every (sink, node) interaction is evaluated by exactly one wave, so the interaction counters
+must equal those of the one-wave walk exactly, and the forces must agree with it up to FP32
+summation order. The oracle's forces are checked with the same tolerance as the one-wave walk
+(median 1e-5, p99 1e-4 relative).
+"""
+import numpy as np
+import pytest
+
+from masterthesislucas_amd import BHContext, tsne
+from masterthesislucas_amd._lib import BHError
+from masterthesislucas_amd.workloads import plummer, two_plummer_disks, uniform_disk
+from oracle import oracle as O
+from tests.helpers import oracle_forces, oracle_tree, rel_err
+
+pytestmark = pytest.mark.gpu
+
+EPS = float(np.float32(1e-3))
+
+
+def _forces(p, kernel, variant, split, theta, m=True):
+    ctx = BHContext(p.n, kernel=kernel, tree_mode="quadtree", softening=EPS)
+    ctx.set_particles(p.x, p.y, p.m if m else None)
+    ctx.set_variant(variant)
+    ctx.set_split(split)
+    ctx.build()
+    ax, ay, tot = ctx.forces(theta)
+    cnt = ctx.counters()
+    ctx.close()
+    return ax, ay, tot, cnt
+
+
+@pytest.mark.parametrize("variant", [0, 4, 9])
+@pytest.mark.parametrize("split", [2, 3, 8])
+def test_split_equals_one_wave_walk(gpu, variant, split):
+    p = two_plummer_disks(30000, seed=61)
+    ax1, ay1, t1, c1 = _forces(p, "gravity", variant, 1, 0.5)
+    ax, ay, t, c = _forces(p, "gravity", variant, split, 0.5)
+    assert c == c1  # each interaction counted by exactly one wave
+    r = rel_err(ax, ay, ax1.astype(np.float64), ay1.astype(np.float64))
+    assert np.median(r) <= 1e-6 and r.max() <= 1e-4, (np.median(r), r.max())
+    assert abs(t - t1) <= 1e-6 * abs(t1)
+
+
+@pytest.mark.parametrize("kernel", ["gravity", "tsne"])
+def test_split_matches_oracle(gpu, kernel):
+    p = uniform_disk(20000, seed=62)
+    grav = kernel == "gravity"
+    ax, ay, tot, (pn, pp) = _forces(p, kernel, 4, 4, 0.5, m=grav)
+    t = oracle_tree(p.x, p.y, p.m if grav else None, mode="quadtree")
+    okern = O.KERNEL_GRAVITY if grav else O.KERNEL_TSNE
+    bx, by, bt, cnt = oracle_forces(t, okern, 0.5, eps=EPS if grav else 0.0)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= 1e-5 and np.quantile(r, 0.99) <= 1e-4
+    assert abs(tot - bt.sum()) <= 1e-5 * abs(bt.sum())
+    assert abs(int(pn) - int(cnt[0])) <= 1e-3 * cnt[0]
+    assert abs(int(pp) - int(cnt[1])) <= 1e-3 * cnt[1]
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 100, 1000])
+def test_split_small_systems(gpu, n):
+    """More waves than nodes: some waves own an empty range and must write zero parts."""
+    p = plummer(n, a=0.1, seed=63)
+    ax1, ay1, _, c1 = _forces(p, "gravity", 4, 1, 0.5)
+    ax, ay, _, c = _forces(p, "gravity", 4, 8, 0.5)
+    assert c == c1
+    assert np.allclose(ax, ax1, rtol=1e-5, atol=1e-6 * np.abs(ax1).max(initial=1.0))
+    assert np.allclose(ay, ay1, rtol=1e-5, atol=1e-6 * np.abs(ay1).max(initial=1.0))
+
+
+def test_split_setter_rejects_bad_values(gpu):
+    ctx = BHContext(100, kernel="gravity")
+    for bad in (0, 9, -1):
+        with pytest.raises(BHError):
+            ctx.set_split(bad)
+    ctx.set_split(8)
+    ctx.set_split(2)  # shrinking keeps the larger part buffer
+    ctx.close()
+    b = BHContext(10, kernel="gravity", sizes=[5, 5])
+    with pytest.raises(BHError):
+        b.set_split(2)
+    b.set_split(1)
+    b.close()
+
+
+def test_split_graph_steps_equal_eager_steps(gpu):
+    p = plummer(20000, a=0.1, seed=64)
+    ctxs = []
+    for _ in range(2):
+        c = BHContext(p.n, kernel="gravity", softening=EPS)
+        c.set_particles(p.x, p.y, p.m, p.vx + 0.1 * p.y, p.vy - 0.1 * p.x)
+        c.set_split(4)
+        ctxs.append(c)
+    eager, graph = ctxs
+    for steps in (1, 2, 3):
+        eager.leapfrog(steps, 1e-4, 0.5)
+        graph.leapfrog_graph(steps, 1e-4, 0.5)
+        for a, b in zip(eager.get_particles(), graph.get_particles()):
+            assert np.array_equal(a, b), steps
+    # changing the split re-captures the steps
+    eager.set_split(2)
+    graph.set_split(2)
+    eager.leapfrog(2, 1e-4, 0.5)
+    graph.leapfrog_graph(2, 1e-4, 0.5)
+    for a, b in zip(eager.get_particles(), graph.get_particles()):
+        assert np.array_equal(a, b)
+    for c in ctxs:
+        c.close()
+
+
+def test_split_tsne_iterations_track_one_wave_walk(gpu):
+    n = 4000
+    X, _ = tsne.gaussian_mixture(n, dim=30, seed=65)
+    rp, col, val = tsne.joint_probabilities(X, perplexity=20.0, k=60)
+    yx, yy = tsne.initial_embedding(n, seed=65, std=1.0)
+    runs = []
+    for split in (1, 4):
+        t = tsne.TSNE(rp, col, val, (yx, yy), min_theta=0.5, max_theta=0.5)
+        t.ctx.set_split(split)
+        try:
+            dx, dy, z = t.timeStep(return_derivative=True)
+            runs.append((dx, dy, z, t.embeddedPoints.copy()))
+            # per-particle reads through the t-SNE context are sized to its n points
+            ax, ay, _ = t.ctx.forces(0.5)
+            assert ax.shape == (n,) and ay.shape == (n,)
+        finally:
+            t.close()
+    (dx1, dy1, z1, y1), (dx4, dy4, z4, y4) = runs
+    assert abs(z4 - z1) <= 1e-6 * z1
+    scale = np.hypot(dx1, dy1).max()
+    assert np.max(np.hypot(dx4 - dx1, dy4 - dy1)) <= 1e-5 * scale
+    assert np.max(np.abs(y4 - y1)) <= 1e-5 * np.abs(y1).max()
