@@ -43,7 +43,7 @@ def test_morton32_codes_and_order_bitexact(gpu):
 
 
 @pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
-@pytest.mark.parametrize("n", [1, 4095, 4097, 1_000_003])
+@pytest.mark.parametrize("n", [1, 4095, 4097, 262_145, 1_000_003])
 def test_sort_many_tiles_with_duplicates(gpu, mode, n):
     """The one-sweep radix sort over many 4096-key tiles (ragged last tile): codes and order
     bit-exact against the oracle's stable sort, with a third of the points on 7 spots so that
