@@ -13,6 +13,8 @@
 #include "bh_launch.h"
 #include "bh_mlp.h"
 
+#include <type_traits>
+
 namespace bh {
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -258,23 +260,49 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                             }
                         }
                     } else {
-                    // PPU sources per scalar fetch; pos4 is padded past n, lanes mask the tail
-                    for (uint32_t j = f; j < e; j += PPU) {
-                        float4 q[PPU > 0 ? PPU : 1];
+                        // PPU sources per scalar fetch (pos4 is padded past n, lanes mask the
+                        // tail). Only sink slots with open lanes take part: in about half of the
+                        // cells a pair of slots has one slot with no lane near the cell.
+                        auto pp_slots = [&](auto lo_c, auto hi_c) {
+                            constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
+                            for (uint32_t j = f; j < e; j += PPU) {
+                                float4 q[PPU > 0 ? PPU : 1];
 #pragma unroll
-                        for (int t = 0; t < PPU; ++t) q[t] = pos4[j + t];
+                                for (int t = 0; t < PPU; ++t) q[t] = pos4[j + t];
 #pragma unroll
-                        for (int t = 0; t < PPU; ++t) {
+                                for (int t = 0; t < PPU; ++t) {
 #pragma unroll
-                            for (int s = 0; s < NS; ++s) {
-                                const float qx = q[t].x - px[s], qy = q[t].y - py[s];
-                                const float q2 = fmaf(qx, qx, qy * qy);
-                                const bool use = mine[s] && (j + t < e) && (j + t != idx[s]);
-                                accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s], ly[s],
-                                                   lt[s]);
+                                    for (int s = LO; s < HI; ++s) {
+                                        const float qx = q[t].x - px[s], qy = q[t].y - py[s];
+                                        const float q2 = fmaf(qx, qx, qy * qy);
+                                        const bool use =
+                                            mine[s] && (j + t < e) && (j + t != idx[s]);
+                                        accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s],
+                                                           ly[s], lt[s]);
+                                    }
+                                }
                             }
+                        };
+                        auto pp_pair = [&](auto p_c) {
+                            constexpr int P = decltype(p_c)::value;
+                            const bool a = open[P] != 0ull, c = open[P + 1] != 0ull;
+                            if (a && c)
+                                pp_slots(std::integral_constant<int, P>{},
+                                         std::integral_constant<int, P + 2>{});
+                            else if (a)
+                                pp_slots(std::integral_constant<int, P>{},
+                                         std::integral_constant<int, P + 1>{});
+                            else if (c)
+                                pp_slots(std::integral_constant<int, P + 1>{},
+                                         std::integral_constant<int, P + 2>{});
+                        };
+                        if constexpr (NS == 1) {
+                            pp_slots(std::integral_constant<int, 0>{},
+                                     std::integral_constant<int, 1>{});
+                        } else {
+                            pp_pair(std::integral_constant<int, 0>{});
+                            if constexpr (NS == 4) pp_pair(std::integral_constant<int, 2>{});
                         }
-                    }
                     }
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
