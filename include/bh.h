@@ -248,7 +248,11 @@ bh_status bh_synchronize(bh_ctx* ctx);
  * 1 or 4 sources per scalar fetch in the particle-particle loop. For 12 <= v < 18: (v - 12) / 2
  * selects 1, 2 or 4 sinks per lane, (v - 12) % 2 the monopole mode as above, and the
  * particle-particle sources of a cell are loaded by the lanes with one vector load (up to 64 at
- * a time) and broadcast. Results are identical up to FP32 summation order. */
+ * a time) and broadcast. Variants 18 <= v < 36 take the shape of v - 18 with the lane masks
+ * kept in scalar registers (acceptance mask = ballot of the distance test and the active mask;
+ * fewer vector instructions per visit, faster for some shapes and workloads, slower for
+ * others). Split walks (bh_set_traverse_split) use the vector-mask form of the shape. Results
+ * are identical up to FP32 summation order. */
 int bh_traverse_variant_count(void);
 bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
 /* Tree-split walk for small N (no reference counterpart): `split` waves (1..8) share each sink

@@ -64,15 +64,24 @@ void launch_tree_slots(hipStream_t s, const uint32_t* codes, const Node* nodes,
 // per scalar load (1 or 4), or PPU = 0: each lane loads one of up to 64 sources with one vector
 // load and the wave broadcasts them with readlane (one memory round trip per cell).
 // Variants 0..11: NS = {1,2,4}[v / 4], branchy = (v % 4) < 2, PPU = v % 2 ? 4 : 1;
-// variants 12..17: NS = {1,2,4}[(v - 12) / 2], branchy = (v - 12) % 2 == 0, PPU = 0.
-constexpr int kTraverseVariants = 18;
-inline int variant_ns(int v) {
-    const int i = v < 12 ? v / 4 : (v - 12) / 2;
-    return 1 << i;
+// variants 12..17: NS = {1,2,4}[(v - 12) / 2], branchy = (v - 12) % 2 == 0, PPU = 0;
+// variants 18..35: the shapes of v - 18 with the lane masks in scalar registers.
+constexpr int kTraverseVariantsVector = 18;
+constexpr int kTraverseVariants = 2 * kTraverseVariantsVector;
+inline int variant_shape(int v) { return v % kTraverseVariantsVector; }
+inline int variant_ns_index(int v) {
+    v = variant_shape(v);
+    return v < 12 ? v / 4 : (v - 12) / 2;
 }
-inline int variant_ns_index(int v) { return v < 12 ? v / 4 : (v - 12) / 2; }
-inline bool variant_branchy(int v) { return v < 12 ? (v % 4) < 2 : (v - 12) % 2 == 0; }
-inline int variant_ppu(int v) { return v < 12 ? (v % 2 ? 4 : 1) : 0; }
+inline int variant_ns(int v) { return 1 << variant_ns_index(v); }
+inline bool variant_branchy(int v) {
+    v = variant_shape(v);
+    return v < 12 ? (v % 4) < 2 : (v - 12) % 2 == 0;
+}
+inline int variant_ppu(int v) {
+    v = variant_shape(v);
+    return v < 12 ? (v % 2 ? 4 : 1) : 0;
+}
 // returns the number of blocks launched (one counter partial each)
 // batched systems: one work entry (first sink, count, system, -) per wave and the systems'
 // roots; work == nullptr for a one-system context

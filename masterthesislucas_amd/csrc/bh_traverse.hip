@@ -115,12 +115,18 @@ __device__ __forceinline__ void wave_work(const TraverseBatch& batch, uint32_t g
 // BATCHED: the wave's sinks and subtree come from the work table of a batched context.
 // The node records for both possible next visits (descend: k + 1, skip: the subtree end) are
 // fetched while the current one is used.
+// SMASK: lane masks in scalar registers: the acceptance mask is the ballot of the distance test
+//   and'ed with the active mask, a lane's predicate its inverse ballot, the interaction count a
+//   wave popcount, and 1/theta^2 sits in a vector register so the limit costs one multiply.
+//   Fewer vector instructions per visit; which form is faster depends on the variant and the
+//   workload (measured both ways), so both are variants.
 // SPLIT: F waves share one sink group; wave f of the group owns the pre-order node range
 // [f nn / F, (f+1) nn / F). A wave skips subtrees that miss its range, adds the monopoles and
 // particle-particle sums only of nodes it owns, and writes partial results to part f of the
 // output; every interaction is counted by exactly one wave. This multiplies the number of waves
 // when there are too few sinks to fill the GPU.
-template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED, bool SPLIT>
+template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED, bool SPLIT,
+          bool SMASK>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
                                                      const BuildParams* __restrict__ prm,
@@ -174,11 +180,14 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     // totals at every particle-particle visit and at least every kFoldEvery visits.
     float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
     uint32_t npn[NS], npp[NS];
+    uint64_t cnt_pn = 0;  // SMASK: accepted (sink, node) pairs of the wave
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] = by[s] = bt[s] = 0.f;
         npn[s] = npp[s] = 0u;
     }
+    float inv_t2 = inv_theta2;
+    if constexpr (SMASK) asm volatile("v_mov_b32 %0, %1" : "=v"(inv_t2) : "s"(inv_theta2));
     uint32_t k = k0, end_cur = end0;
     int sp = 0, since_fold = 0;
     Node cur = nodes[min(k0, nn - 1)];
@@ -202,20 +211,31 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             owned = k >= pstart;
             enter = next > pstart ? ~0ull : 0ull;  // the subtree reaches this wave's range
         }
-        const float lim = cur.l2 * inv_theta2;
+        const float lim = cur.l2 * inv_t2;
         uint64_t open[NS];
         uint64_t open_any = 0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const bool on = (active[s] >> lane) & 1ull;
             const float dx = cur.cx - px[s], dy = cur.cy - py[s];
             const float d2 = fmaf(dx, dx, dy * dy);
-            const bool acc = on && (d2 > lim);
-            if (owned && (!BRANCHY || __ballot(acc))) {
-                accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
-                npn[s] += acc ? 1u : 0u;
+            if constexpr (SMASK) {
+                const uint64_t accm = __ballot(d2 > lim) & active[s];
+                if (owned && (!BRANCHY || accm)) {
+                    accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2,
+                                       __builtin_amdgcn_inverse_ballot_w64(accm), bx[s], by[s],
+                                       bt[s]);
+                    cnt_pn += (uint64_t)__popcll(accm);
+                }
+                open[s] = active[s] & ~accm & enter;
+            } else {
+                const bool on = (active[s] >> lane) & 1ull;
+                const bool acc = on && (d2 > lim);
+                if (owned && (!BRANCHY || __ballot(acc))) {
+                    accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
+                    npn[s] += acc ? 1u : 0u;
+                }
+                open[s] = __ballot(on && !acc) & enter;
             }
-            open[s] = __ballot(on && !acc) & enter;
             open_any |= open[s];
         }
         bool down = false;
@@ -226,7 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                     float lx[NS], ly[NS], lt[NS];
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        mine[s] = (open[s] >> lane) & 1ull;
+                        mine[s] = SMASK ? __builtin_amdgcn_inverse_ballot_w64(open[s])
+                                        : ((open[s] >> lane) & 1ull) != 0ull;
                         lx[s] = ly[s] = lt[s] = 0.f;
                     }
                     const uint32_t f = cur.first, e = f + cur.count;
@@ -343,7 +364,20 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             since_fold = 0;
         }
         k = down ? k + 1 : next;
-        cur = down ? nd_down : nd_skip;
+        // select the next record word by word on the integer bit patterns: a uniform select
+        // of uniform values, kept in scalar registers
+        cur.cx = __uint_as_float(__builtin_amdgcn_readfirstlane(
+            down ? __float_as_uint(nd_down.cx) : __float_as_uint(nd_skip.cx)));
+        cur.cy = __uint_as_float(__builtin_amdgcn_readfirstlane(
+            down ? __float_as_uint(nd_down.cy) : __float_as_uint(nd_skip.cy)));
+        cur.mass = __uint_as_float(__builtin_amdgcn_readfirstlane(
+            down ? __float_as_uint(nd_down.mass) : __float_as_uint(nd_skip.mass)));
+        cur.l2 = __uint_as_float(__builtin_amdgcn_readfirstlane(
+            down ? __float_as_uint(nd_down.l2) : __float_as_uint(nd_skip.l2)));
+        cur.skip = down ? nd_down.skip : nd_skip.skip;
+        cur.first = down ? nd_down.first : nd_skip.first;
+        cur.count = down ? nd_down.count : nd_skip.count;
+        cur.meta = down ? nd_down.meta : nd_skip.meta;
     }
     unsigned long long s_pn = 0, s_pp = 0;
 #pragma unroll
@@ -362,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     }
     if (counters) {  // one partial per block (no same-address device atomics)
         __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
-        s_pn = wave_sum_u64(s_pn);
+        s_pn = wave_sum_u64(s_pn) + cnt_pn;
         s_pp = wave_sum_u64(s_pp);
         if (lane == 0) {
             s_cnt[w][0] = s_pn;
@@ -709,9 +743,23 @@ static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, 
                             const Node* nodes, const BuildParams* prm, float inv_theta2,
                             float eps2, float scale, float* ax, float* ay, float* tot,
                             unsigned long long* counters, TraverseBatch batch) {
-#define BH_TRAV(NS, BR, U)                                                                        \
-    k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false><<<g, kBlock, 0, s>>>(                      \
+#define BH_TRAV(NS, BR, U) BH_TRAV_M(NS, BR, U, false)
+#define BH_TRAV_M(NS, BR, U, M)                                                                   \
+    k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false, M><<<g, kBlock, 0, s>>>(                   \
         pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch, 1u)
+    if (variant >= kTraverseVariantsVector) {
+        // the same 18 shapes with scalar lane masks
+        const int v = variant - kTraverseVariantsVector;
+        const int ns = variant_ns(v), ppu = variant_ppu(v);
+        const bool br = variant_branchy(v);
+#define BH_TRAV_S(NS)                                                                             \
+    if (ppu == 0) { if (br) BH_TRAV_M(NS, true, 0, true); else BH_TRAV_M(NS, false, 0, true); }   \
+    else if (ppu == 1) { if (br) BH_TRAV_M(NS, true, 1, true); else BH_TRAV_M(NS, false, 1, true); } \
+    else { if (br) BH_TRAV_M(NS, true, 4, true); else BH_TRAV_M(NS, false, 4, true); }
+        if (ns == 1) { BH_TRAV_S(1) } else if (ns == 2) { BH_TRAV_S(2) } else { BH_TRAV_S(4) }
+#undef BH_TRAV_S
+        return;
+    }
     switch (variant) {
         case 0: BH_TRAV(1, true, 1); break;
         case 1: BH_TRAV(1, true, 4); break;
@@ -733,6 +781,7 @@ static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, 
         default: BH_TRAV(4, false, 0); break;
     }
 #undef BH_TRAV
+#undef BH_TRAV_M
 }
 
 // split traversal: the variant's NS and branchiness, particle-particle sources by scalar load
@@ -747,7 +796,7 @@ static uint32_t traverse_split(hipStream_t s, int variant, uint32_t F, uint32_t 
     const uint32_t g = div_up(waves, kWavesPerBlock);
     TraverseBatch none;
 #define BH_TRAVS(NS, BR, U)                                                                       \
-    k_traverse<KERNEL, TOT, NS, BR, U, false, true><<<g, kBlock, 0, s>>>(                         \
+    k_traverse<KERNEL, TOT, NS, BR, U, false, true, false><<<g, kBlock, 0, s>>>(                  \
         pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F)
 #define BH_TRAVS_U(NS, BR)                                                                        \
     if (lane_pp) BH_TRAVS(NS, BR, 0); else BH_TRAVS(NS, BR, 1)

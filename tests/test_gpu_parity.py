@@ -340,7 +340,7 @@ def test_rebuild_reuses_context(gpu):
         assert np.array_equal(ax, fx) and np.array_equal(ay, fy)
 
 
-@pytest.mark.parametrize("variant", range(18))
+@pytest.mark.parametrize("variant", range(36))
 @pytest.mark.parametrize("kernel", ["gravity", "tsne"])
 def test_every_traversal_variant(gpu, variant, kernel):
     """All autotuner choices give the oracle's forces (they differ only in summation order)."""
