@@ -214,6 +214,29 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return off + x - v;
 }
 
+// the same for a block of THREADS threads
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_exclusive_scan_t(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[THREADS / kWave];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int k = 0; k < THREADS / kWave; ++k) {
+        const uint32_t s = wsum[k];
+        if (k < w) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
                                                         uint32_t* __restrict__ partials) {
     const uint32_t base = blockIdx.x * kScanTile;
@@ -279,8 +302,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 // only waits on tiles whose tickets were taken before its own (their blocks are running), so
 // every wait ends.
 // ------------------------------------------------------------------------------------------
-// Keys per lane: 16 (4096-key tiles) for large sorts; small sorts use 8 (2048-key tiles) so
-// that more blocks share the work (measured: 70k keys 20% faster, 1M keys 14% slower).
+// Tiles of 4096 keys for large sorts, 2048 for sorts of at most kSortSmallMax keys (more
+// blocks share a small sort: 70k keys 20% faster, while 1M keys are 14% slower with them).
+// kSortItems* is the tile in keys per thread of a 256-thread block (the counting kernel); the
+// pass kernel ranks a tile with 4 keys per lane (1024 threads, or 512 for small tiles), which
+// measured faster than 8 or 16 keys per lane on fewer waves.
 constexpr int kSortItemsLarge = 16, kSortItemsSmall = 8;
 constexpr uint32_t kSortSmallMax = 1u << 18;  // sorts up to this size use the small tiles
 __host__ __device__ constexpr int sort_tile(int items) { return kBlock * items; }
@@ -333,8 +359,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_totals(const uint32_t* __restr
     if (lane == 0) totals[row] = (uint32_t)s;
 }
 
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ keys_in,
+template <typename K, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_radix_pass(const K* __restrict__ keys_in,
                                                        const uint32_t* __restrict__ vals_in,
                                                        K* __restrict__ keys_out,
                                                        uint32_t* __restrict__ vals_out,
@@ -343,16 +369,17 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
                                                        uint32_t* __restrict__ status,
                                                        uint32_t* __restrict__ ticket,
                                                        uint32_t* __restrict__ err) {
-    constexpr int kTile = sort_tile(ITEMS), kWaveKeys = kWave * ITEMS;
+    constexpr int kTile = THREADS * ITEMS, kWaveKeys = kWave * ITEMS, kW = THREADS / kWave;
+    static_assert(THREADS >= 256, "one thread per digit");
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
-    __shared__ uint32_t s_wave[kWavesPerBlock][256];  // wave-local counts, then wave offsets
+    __shared__ uint32_t s_wave[kW][256];  // wave-local counts, then wave offsets
     __shared__ uint32_t s_start[256];                  // tile-local start of each digit
     __shared__ uint32_t s_base[256];                   // global start of each digit for this tile
     __shared__ uint32_t s_tile;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int d = threadIdx.x; d < kWavesPerBlock * 256; d += kBlock) (&s_wave[0][0])[d] = 0u;
+    for (int d = threadIdx.x; d < kW * 256; d += THREADS) (&s_wave[0][0])[d] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
 
@@ -387,20 +414,25 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     }
     __syncthreads();
     {
-        const int d = threadIdx.x;
+        // threads 0..255 own one digit each; the scans take every thread of the block
+        const int d = threadIdx.x & 255;
+        const bool dig = threadIdx.x < 256;
         uint32_t cnt = 0;
-        for (int k = 0; k < kWavesPerBlock; ++k) {
-            const uint32_t c = s_wave[k][d];
-            s_wave[k][d] = cnt;
-            cnt += c;
+        if (dig) {
+            for (int k = 0; k < kW; ++k) {
+                const uint32_t c = s_wave[k][d];
+                s_wave[k][d] = cnt;
+                cnt += c;
+            }
         }
         uint32_t tot;
-        s_start[d] = block_exclusive_scan(cnt, &tot);
+        const uint32_t start = block_exclusive_scan_t<THREADS>(cnt, &tot);
+        if (dig) s_start[d] = start;
         uint32_t* mine = status + (size_t)tile * 256 + d;
         uint32_t excl = 0;
         if (tile == 0) {  // block-uniform: the first tile starts at the digit's global offset
-            excl = block_exclusive_scan(totals[d], &tot);
-        } else {
+            excl = block_exclusive_scan_t<THREADS>(dig ? totals[d] : 0u, &tot);
+        } else if (dig) {
             __hip_atomic_store(mine, kLbCount | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // look back kLbWindow tiles per round with independent loads, nearest first; stop at
             // the first inclusive prefix (tile 0 always publishes one), wait at a tile not ready
@@ -433,9 +465,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
                 }
             }
         }
-        __hip_atomic_store(mine, kLbPrefix | (excl + cnt), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        s_base[d] = excl;
+        if (dig) {
+            __hip_atomic_store(mine, kLbPrefix | (excl + cnt), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_base[d] = excl;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -448,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     }
     __syncthreads();
     const uint32_t tile_n = min((uint32_t)kTile, n - tile * kTile);
-    for (uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
+    for (uint32_t p = threadIdx.x; p < tile_n; p += THREADS) {
         const K k = s_keys[p];
         const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
         const uint32_t g = s_base[d] + (p - s_start[d]);
@@ -859,28 +893,6 @@ constexpr int kTopPer = (kTopMax + kUpThreads - 1) / kUpThreads;  // entries per
 static_assert(kTopHash >= kTopMax && (kTopHash & (kTopHash - 1)) == 0, "hash size");
 static_assert(kTopMax < 65536, "entry indices are 16-bit");
 
-template <int THREADS>
-__device__ __forceinline__ uint32_t block_exclusive_scan_t(uint32_t v, uint32_t* total) {
-    __shared__ uint32_t wsum[THREADS / kWave];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, kWave);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-    for (int k = 0; k < THREADS / kWave; ++k) {
-        const uint32_t s = wsum[k];
-        if (k < w) off += s;
-        tot += s;
-    }
-    __syncthreads();
-    *total = tot;
-    return off + x - v;
-}
-
 __device__ __forceinline__ uint32_t top_hash(uint32_t k) { return (k * 0x9E3779B1u) >> 20; }
 
 __global__ __launch_bounds__(kUpThreads) void k_up_top(Node* __restrict__ nodes,
@@ -1191,13 +1203,15 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
     for (int p = 0; p < passes; ++p) {
         uint32_t* st = status + (size_t)p * nt * 256;
         const uint32_t* tp = totals + p * 256;
-#define BH_PASS(K, I)                                                                             \
-    k_radix_pass<K, I><<<nt, kBlock, 0, s>>>((const K*)kin, vin, (K*)kout, vout, n, 8 * p, tp, st, \
-                                             tickets + p, err)
+#define BH_PASS(K, T, I)                                                                          \
+    k_radix_pass<K, T, I><<<nt, T, 0, s>>>((const K*)kin, vin, (K*)kout, vout, n, 8 * p, tp, st,  \
+                                           tickets + p, err)
+        static_assert(kSortItemsLarge * kBlock == 1024 * 4 && kSortItemsSmall * kBlock == 512 * 4,
+                      "pass tiles match the counting tiles");
         if (wide) {
-            if (small) BH_PASS(uint64_t, kSortItemsSmall); else BH_PASS(uint64_t, kSortItemsLarge);
+            if (small) BH_PASS(uint64_t, 512, 4); else BH_PASS(uint64_t, 1024, 4);
         } else {
-            if (small) BH_PASS(uint32_t, kSortItemsSmall); else BH_PASS(uint32_t, kSortItemsLarge);
+            if (small) BH_PASS(uint32_t, 512, 4); else BH_PASS(uint32_t, 1024, 4);
         }
 #undef BH_PASS
         void* tk = kin; kin = kout; kout = tk;
