@@ -316,32 +316,32 @@ constexpr uint32_t kLbSpinLimit = 1u << 22;
 constexpr int kLbWindow = 8;  // look-back loads in flight per digit
 static_assert(kBlock == 256, "one thread per digit");
 
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_radix_counts(const K* __restrict__ keys, uint32_t n,
-                                                         int passes, uint32_t ntiles,
-                                                         uint32_t* __restrict__ counts,
-                                                         uint32_t* __restrict__ status) {
+template <typename K, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_radix_counts(const K* __restrict__ keys, uint32_t n,
+                                                          int passes, uint32_t ntiles,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ status) {
     __shared__ uint32_t h[kSortMaxPasses][256];
-    for (int i = threadIdx.x; i < kSortMaxPasses * 256; i += kBlock) (&h[0][0])[i] = 0u;
-    for (int p = 0; p < passes; ++p)
-        status[((size_t)p * ntiles + blockIdx.x) * 256 + threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < kSortMaxPasses * 256; i += THREADS) (&h[0][0])[i] = 0u;
+    for (int i = threadIdx.x; i < passes * 256; i += THREADS)
+        status[((size_t)(i >> 8) * ntiles + blockIdx.x) * 256 + (i & 255)] = 0u;
     __syncthreads();
-    const uint32_t base = blockIdx.x * sort_tile(ITEMS);
+    const uint32_t base = blockIdx.x * (THREADS * ITEMS);
     K k[ITEMS];  // all loads issued before the first use
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const uint32_t i = base + r * kBlock + threadIdx.x;
+        const uint32_t i = base + r * THREADS + threadIdx.x;
         k[r] = i < n ? keys[i] : (K)0;
     }
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        if (base + r * kBlock + threadIdx.x >= n) continue;
+        if (base + r * THREADS + threadIdx.x >= n) continue;
         for (int p = 0; p < passes; ++p)
             atomicAdd(&h[p][(uint32_t)(k[r] >> (8 * p)) & 0xFFu], 1u);
     }
     __syncthreads();
-    for (int p = 0; p < passes; ++p)
-        counts[((size_t)p * 256 + threadIdx.x) * ntiles + blockIdx.x] = h[p][threadIdx.x];
+    for (int i = threadIdx.x; i < passes * 256; i += THREADS)
+        counts[(size_t)i * ntiles + blockIdx.x] = h[i >> 8][i & 255];
 }
 
 // one wave per (pass, digit) row of counts; block 0 also clears the pass tickets
@@ -1186,12 +1186,12 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
     uint32_t* tickets = totals + kSortMaxPasses * 256;
     uint32_t* err = tickets + kSortMaxPasses;
     const bool small = sort_items(n) == kSortItemsSmall;
-#define BH_COUNTS(K, I)                                                                           \
-    k_radix_counts<K, I><<<nt, kBlock, 0, s>>>((const K*)keys_a, n, passes, nt, counts, status)
+#define BH_COUNTS(K, T, I)                                                                        \
+    k_radix_counts<K, T, I><<<nt, T, 0, s>>>((const K*)keys_a, n, passes, nt, counts, status)
     if (wide) {
-        if (small) BH_COUNTS(uint64_t, kSortItemsSmall); else BH_COUNTS(uint64_t, kSortItemsLarge);
+        if (small) BH_COUNTS(uint64_t, 512, 4); else BH_COUNTS(uint64_t, 1024, 4);
     } else {
-        if (small) BH_COUNTS(uint32_t, kSortItemsSmall); else BH_COUNTS(uint32_t, kSortItemsLarge);
+        if (small) BH_COUNTS(uint32_t, 512, 4); else BH_COUNTS(uint32_t, 1024, 4);
     }
 #undef BH_COUNTS
     k_radix_totals<<<div_up(passes * 256, kWavesPerBlock), kBlock, 0, s>>>(counts, nt, passes,
