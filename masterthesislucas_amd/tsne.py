@@ -211,5 +211,24 @@ class TSNE:
         self.ctx._c(_lib.load().bh_tsne_get_embedding(self.ctx.handle, _ptr(yx), _ptr(yy)))
         return np.stack([yx, yy], 1)
 
+    def save_state(self, path, labels=None):
+        """Write the current embedding and iteration counter (formats.save_state). The previous
+        embedding is not part of the dump: a run resumed from it restarts the momentum term,
+        like a fresh ``TSNE`` started from these positions."""
+        from . import formats
+        formats.save_state(path, self.iteration_counter, self.embeddedPoints, None, labels)
+
+    def load_state(self, path):
+        """Continue from a state dump: embedding and iteration counter (momentum restarts)."""
+        from . import formats
+        s = formats.load_state(path)
+        if s["y"].shape[0] != self.n:
+            raise ValueError(f"state holds {s['y'].shape[0]} points, this run {self.n}")
+        yx = np.ascontiguousarray(s["y"][:, 0])
+        yy = np.ascontiguousarray(s["y"][:, 1])
+        self.ctx._c(_lib.load().bh_tsne_set_embedding(self.ctx.handle, _ptr(yx), _ptr(yy)))
+        self.iteration_counter = s["iteration"]
+        return s
+
     def close(self):
         self.ctx.close()

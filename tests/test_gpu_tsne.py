@@ -164,3 +164,25 @@ def test_direct_sum_between_steps_keeps_p_matrix(gpu):
     finally:
         a.close()
         b.close()
+
+
+def test_state_dump_resumes_the_run(gpu, tmp_path):
+    """A run saved with TSNE.save_state and resumed in a new object continues from the same
+    embedding and iteration (the momentum term restarts, as documented)."""
+    rp, col, val, yx, yy = _problem(2000)
+    a = tsne.TSNE(rp, col, val, (yx, yy), min_theta=0.5, max_theta=0.5)
+    b = tsne.TSNE(rp, col, val, (yx, yy), min_theta=0.5, max_theta=0.5)
+    try:
+        a.run(30)
+        a.save_state(tmp_path / "s.bin")
+        s = b.load_state(tmp_path / "s.bin")
+        assert b.iteration_counter == 30 == s["iteration"]
+        assert np.array_equal(b.embeddedPoints, a.embeddedPoints)
+        # from equal positions with the momentum reset on both, one step agrees
+        a.load_state(tmp_path / "s.bin")
+        da = a.timeStep(return_derivative=True)
+        db = b.timeStep(return_derivative=True)
+        assert np.array_equal(da[0], db[0]) and np.array_equal(a.embeddedPoints, b.embeddedPoints)
+    finally:
+        a.close()
+        b.close()
