@@ -1158,3 +1158,243 @@ void orc_adam(double* p, const double* g, double* mom1, double* mom2, int64_t n,
         p[i] -= lr * mh / (sqrt(vh) + eps);
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Symmetric dual-tree FMM on the fixed-depth Morton tree, t-SNE kernels (NBodySolverFMM_SYM_   */
+/* MORTON: traverse_SYM_NN :154-270, traverse_SYM_PN :272-313, applyForces :482-536, solveNbody */
+/* :68-76; TSNE_FMM_SYM_MORTON_NN/PN/PP_Kernel :622-836), in FP64 over the slots of            */
+/* orc_fixed_tree. Expansions per node: C1 (2), C2 (2x2, [i][j] at 2i+j), C3 (2x2x2, [i][j][k] */
+/* at 4i+2j+k).                                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    const double *xs, *ys;
+    const uint32_t *first_child, *first, *count;
+    const double *bb_len, *cx, *cy, *m0, *m2;
+    double theta;
+    double *c1, *c2, *c3; /* 2, 4, 8 per slot */
+    double *ax, *ay;
+    double total;
+} fmm_ctx;
+
+static void fmm_derivs(double rx, double ry, double* rS, double* D1, double* D2, double* D3) {
+    const double sq = rx * rx + ry * ry;
+    *rS = 1.0 + sq;
+    const double inv = 1.0 / *rS, inv2 = inv * inv, inv3 = inv2 * inv, inv4 = inv3 * inv;
+    *D1 = inv2;
+    *D2 = -4.0 * inv3;
+    *D3 = 24.0 * inv4;
+}
+
+/* C1 of a sink at offset R from a source of mass mb and second moment m2 (xx, xy, yy) */
+static void fmm_c1(double rx, double ry, double D1, double D2, double D3, double mb,
+                   const double* m2, double* out) {
+    const double t00 = m2[0] / mb, t01 = m2[1] / mb, t10 = m2[1] / mb, t11 = m2[2] / mb;
+    const double s1 = t00 + t11;
+    const double s2 = (rx * rx * t00) + (rx * ry * t01) + (ry * rx * t10) + (ry * ry * t11);
+    const double s3x = rx * t00 + ry * t01, s3y = rx * t10 + ry * t11;
+    out[0] = mb * (rx * (D1 + 0.5 * s1 * D2 + 0.5 * s2 * D3) + s3x * D2);
+    out[1] = mb * (ry * (D1 + 0.5 * s1 * D2 + 0.5 * s2 * D3) + s3y * D2);
+}
+
+static void fmm_c2(double rx, double ry, double D1, double D2, double f, double* c2) {
+    c2[0] += f * (D1 + rx * rx * D2);
+    c2[1] += f * (rx * ry * D2);
+    c2[2] += f * (ry * rx * D2);
+    c2[3] += f * (D1 + ry * ry * D2);
+}
+
+static void fmm_c3(double rx, double ry, double D2, double D3, double f, double* c3) {
+    c3[0] += f * ((rx + rx + rx) * D2 + rx * rx * rx * D3); /* 000 */
+    c3[1] += f * ((ry)*D2 + rx * rx * ry * D3);            /* 001 */
+    c3[2] += f * ((ry)*D2 + rx * ry * rx * D3);            /* 010 */
+    c3[3] += f * ((rx)*D2 + rx * ry * ry * D3);            /* 011 */
+    c3[4] += f * ((ry)*D2 + ry * rx * rx * D3);            /* 100 */
+    c3[5] += f * ((rx)*D2 + ry * rx * ry * D3);            /* 101 */
+    c3[6] += f * ((rx)*D2 + ry * ry * rx * D3);            /* 110 */
+    c3[7] += f * ((ry + ry + ry) * D2 + ry * ry * ry * D3); /* 111 */
+}
+
+/* TSNE_FMM_SYM_MORTON_NN_Kernel (:622-723) */
+static void fmm_nn(fmm_ctx* c, int64_t a, int64_t b) {
+    double rx = c->cx[a] - c->cx[b], ry = c->cy[a] - c->cy[b];
+    double rS, D1, D2, D3;
+    fmm_derivs(rx, ry, &rS, &D1, &D2, &D3);
+    double v[2];
+    fmm_c1(rx, ry, D1, D2, D3, c->m0[b], c->m2 + 3 * b, v);
+    c->c1[2 * a] += v[0];
+    c->c1[2 * a + 1] += v[1];
+    double c2[4] = {0, 0, 0, 0}, c3[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    fmm_c2(rx, ry, D1, D2, c->m0[a] * c->m0[b], c2);
+    fmm_c3(rx, ry, D2, D3, c->m0[a] * c->m0[b], c3);
+    for (int k = 0; k < 4; ++k) c->c2[4 * a + k] += c2[k];
+    for (int k = 0; k < 8; ++k) c->c3[8 * a + k] += c3[k];
+    /* second interaction: R -> -R, the other node's moments; C2 added as is, C3 negated */
+    fmm_c1(-rx, -ry, D1, D2, D3, c->m0[a], c->m2 + 3 * a, v);
+    c->c1[2 * b] += v[0];
+    c->c1[2 * b + 1] += v[1];
+    for (int k = 0; k < 4; ++k) c->c2[4 * b + k] += c2[k];
+    for (int k = 0; k < 8; ++k) c->c3[8 * b + k] += -c3[k];
+    c->total += 2.0 * (c->m0[a] * c->m0[b]) / rS;
+}
+
+/* TSNE_FMM_SYM_MORTON_PN_Kernel (:726-816) */
+static void fmm_pn(fmm_ctx* c, int64_t p, int64_t b) {
+    double rx = c->xs[p] - c->cx[b], ry = c->ys[p] - c->cy[b];
+    double rS, D1, D2, D3;
+    fmm_derivs(rx, ry, &rS, &D1, &D2, &D3);
+    double v[2];
+    fmm_c1(rx, ry, D1, D2, D3, c->m0[b], c->m2 + 3 * b, v);
+    c->ax[p] += v[0];
+    c->ay[p] += v[1];
+    rx = -rx;
+    ry = -ry;
+    c->c1[2 * b] += rx * D1;
+    c->c1[2 * b + 1] += ry * D1;
+    fmm_c2(rx, ry, D1, D2, c->m0[b], c->c2 + 4 * b);
+    fmm_c3(rx, ry, D2, D3, c->m0[b], c->c3 + 8 * b);
+    c->total += 2.0 * c->m0[b] / rS;
+}
+
+/* TSNE_FMM_SYM_MORTON_PP_Kernel (:818-836) */
+static void fmm_pp(fmm_ctx* c, int64_t p, int64_t q) {
+    const double rx = c->xs[p] - c->xs[q], ry = c->ys[p] - c->ys[q];
+    const double w = 1.0 / (1.0 + (rx * rx + ry * ry));
+    c->ax[p] += w * w * rx;
+    c->ay[p] += w * w * ry;
+    c->ax[q] += w * w * -rx;
+    c->ay[q] += w * w * -ry;
+    c->total += 2.0 * w;
+}
+
+static void fmm_sym_pn(fmm_ctx* c, int64_t p, int64_t b) {
+    const double rx = c->xs[p] - c->cx[b], ry = c->ys[p] - c->cy[b];
+    const double dist = rx * rx + ry * ry;
+    const double thr = c->bb_len[b] / c->theta;
+    if (dist > thr * thr) {
+        fmm_pn(c, p, b);
+    } else if (c->first_child[b] == 0) {
+        for (int64_t q = c->first[b]; q < (int64_t)c->first[b] + c->count[b]; ++q)
+            if (q != p) fmm_pp(c, p, q);
+    } else {
+        for (int64_t ch = c->first_child[b]; ch < (int64_t)c->first_child[b] + 4; ++ch)
+            if (c->count[ch] != 0) fmm_sym_pn(c, p, ch);
+    }
+}
+
+static void fmm_sym_nn(fmm_ctx* c, int64_t a, int64_t b) {
+    const double rx = c->cx[a] - c->cx[b], ry = c->cy[a] - c->cy[b];
+    const double dist = rx * rx + ry * ry;
+    const double thr = (c->bb_len[a] + c->bb_len[b]) / c->theta;
+    if (dist > thr * thr) {
+        if (c->count[a] != 0 && c->count[b] != 0) fmm_nn(c, a, b);
+    } else if (c->first_child[a] == 0) {
+        if (a == b) {
+            for (int64_t p = c->first[a]; p < (int64_t)c->first[a] + c->count[a]; ++p)
+                if (c->count[b] != 0)
+                    for (int64_t q = p; q < (int64_t)c->first[b] + c->count[b]; ++q)
+                        if (p != q) fmm_pp(c, p, q);
+        } else {
+            for (int64_t p = c->first[a]; p < (int64_t)c->first[a] + c->count[a]; ++p)
+                if (c->count[b] != 0) fmm_sym_pn(c, p, b);
+        }
+    } else if (c->first_child[b] == 0) {
+        if (a == b) {
+            for (int64_t q = c->first[b]; q < (int64_t)c->first[b] + c->count[b]; ++q)
+                if (c->count[a] != 0)
+                    for (int64_t p = q; p < (int64_t)c->first[a] + c->count[a]; ++p)
+                        if (p != q) fmm_pp(c, p, q);
+        } else {
+            for (int64_t q = c->first[b]; q < (int64_t)c->first[b] + c->count[b]; ++q)
+                if (c->count[a] != 0) fmm_sym_pn(c, q, a);
+        }
+    } else if (a == b) {
+        for (int64_t i = c->first_child[a]; i < (int64_t)c->first_child[a] + 4; ++i)
+            for (int64_t j = i; j < (int64_t)c->first_child[a] + 4; ++j)
+                if (c->count[i] != 0 && c->count[j] != 0) fmm_sym_nn(c, i, j);
+    } else {
+        for (int64_t i = c->first_child[a]; i < (int64_t)c->first_child[a] + 4; ++i)
+            for (int64_t j = c->first_child[b]; j < (int64_t)c->first_child[b] + 4; ++j)
+                if (c->count[i] != 0 && c->count[j] != 0) fmm_sym_nn(c, i, j);
+    }
+}
+
+/* v = C1 + d.C2 + 0.5 d.(d.C3) (contractions over the first index, :497-504) */
+static void fmm_translate(const double* c1, const double* c2, const double* c3, double dx,
+                          double dy, double* v1, double* v2) {
+    const double d[2] = {dx, dy};
+    double t[4]; /* t[j][k] = sum_i d_i C3[i][j][k] */
+    for (int j = 0; j < 2; ++j)
+        for (int k = 0; k < 2; ++k) t[2 * j + k] = d[0] * c3[0 * 4 + 2 * j + k] + d[1] * c3[1 * 4 + 2 * j + k];
+    for (int k = 0; k < 2; ++k)
+        v1[k] = c1[k] + (d[0] * c2[0 * 2 + k] + d[1] * c2[1 * 2 + k]) +
+                0.5 * (d[0] * t[0 * 2 + k] + d[1] * t[1 * 2 + k]);
+    if (v2)
+        for (int k = 0; k < 4; ++k) v2[k] = c2[k] + t[k];
+}
+
+static void fmm_apply(fmm_ctx* c, int64_t node) {
+    if (c->first_child[node] != 0) {
+        for (int64_t ch = c->first_child[node]; ch < (int64_t)c->first_child[node] + 4; ++ch) {
+            if (c->count[ch] == 0) continue;
+            const double inv = 1.0 / c->m0[ch];
+            for (int k = 0; k < 4; ++k) c->c2[4 * ch + k] *= inv;
+            for (int k = 0; k < 8; ++k) c->c3[8 * ch + k] *= inv;
+            double v1[2], v2[4];
+            fmm_translate(c->c1 + 2 * node, c->c2 + 4 * node, c->c3 + 8 * node,
+                          c->cx[ch] - c->cx[node], c->cy[ch] - c->cy[node], v1, v2);
+            c->c1[2 * ch] += v1[0];
+            c->c1[2 * ch + 1] += v1[1];
+            for (int k = 0; k < 4; ++k) c->c2[4 * ch + k] += v2[k];
+            for (int k = 0; k < 8; ++k) c->c3[8 * ch + k] += c->c3[8 * node + k];
+            fmm_apply(c, ch);
+        }
+    } else {
+        for (int64_t p = c->first[node]; p < (int64_t)c->first[node] + c->count[node]; ++p) {
+            double v1[2];
+            fmm_translate(c->c1 + 2 * node, c->c2 + 4 * node, c->c3 + 8 * node,
+                          c->xs[p] - c->cx[node], c->ys[p] - c->cy[node], v1, NULL);
+            c->ax[p] += v1[0];
+            c->ay[p] += v1[1];
+        }
+    }
+}
+
+/* solveNbody (:68-76): repulsive derivative per sorted point (accumulated into ax, ay, which
+ * the caller zeroes) and the total Z. Tree arrays as written by orc_fixed_tree. */
+double orc_fmm_sym(const double* xs, const double* ys, int64_t n, int depth,
+                   const uint32_t* first_child, const uint32_t* first_particle,
+                   const uint32_t* count, const double* bb_len, const double* com_x,
+                   const double* com_y, const double* m0, const double* m2, double theta,
+                   double* ax, double* ay) {
+    (void)n;
+    const int64_t slots = orc_fixed_tree_slots(depth);
+    fmm_ctx c;
+    c.xs = xs;
+    c.ys = ys;
+    c.first_child = first_child;
+    c.first = first_particle;
+    c.count = count;
+    c.bb_len = bb_len;
+    c.cx = com_x;
+    c.cy = com_y;
+    c.m0 = m0;
+    c.m2 = m2;
+    c.theta = theta;
+    c.c1 = (double*)calloc((size_t)slots * 2, sizeof(double));
+    c.c2 = (double*)calloc((size_t)slots * 4, sizeof(double));
+    c.c3 = (double*)calloc((size_t)slots * 8, sizeof(double));
+    c.ax = ax;
+    c.ay = ay;
+    c.total = 0.0;
+    if (count[0] != 0) {
+        fmm_sym_nn(&c, 0, 0);
+        const double inv = 1.0 / m0[0];
+        for (int k = 0; k < 4; ++k) c.c2[k] *= inv;
+        for (int k = 0; k < 8; ++k) c.c3[k] *= inv;
+        fmm_apply(&c, 0);
+    }
+    free(c.c1);
+    free(c.c2);
+    free(c.c3);
+    return c.total;
+}

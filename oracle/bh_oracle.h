@@ -20,6 +20,8 @@
  *   orc_direct           closed forms (two-body, symmetric rings), Newton's third law
  *   orc_tsne_*           finite differences of the KL cost; Z invariant
  *   orc_mlp_*            finite differences; Adam closed form
+ *   orc_fmm_sym          theta -> 0 equals the exact t-SNE sum; error vs the exact sum shrinks
+ *                        with theta; two far clusters against the exact pair sum
  */
 #ifndef BH_ORACLE_H
 #define BH_ORACLE_H
@@ -135,6 +137,12 @@ double orc_tsne_step(const double* x, const double* y, const double* px, const d
                      double* nx, double* ny, int nthreads);
 
 /* ---- MLP (10 -> H -> H -> 2, SiLU), FP64 ---- */
+/* symmetric dual-tree FMM, t-SNE kernels, over orc_fixed_tree slots; returns Z */
+double orc_fmm_sym(const double* xs, const double* ys, int64_t n, int depth,
+                   const uint32_t* first_child, const uint32_t* first_particle,
+                   const uint32_t* count, const double* bb_len, const double* com_x,
+                   const double* com_y, const double* m0, const double* m2, double theta,
+                   double* ax, double* ay);
 void orc_mlp_forward(const double* X, int64_t B, int din, int h, int dout, const double* W1,
                      const double* b1, const double* W2, const double* b2, const double* W3,
                      const double* b3, double* Y);

@@ -73,6 +73,7 @@ def lib():
             "orc_mlp_forward": (None, [_vp, I64, I32, I32, I32] + [_vp] * 7),
             "orc_mlp_loss_grad": (D, [_vp, _vp, I64, I32, I32, I32] + [_vp] * 12),
             "orc_adam": (None, [_vp, _vp, _vp, _vp, I64, D, D, D, D, I64]),
+            "orc_fmm_sym": (D, [_vp, _vp, I64, I32] + [_vp] * 8 + [D, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -160,6 +161,19 @@ def fixed_tree(xs, ys, depth, minx, miny, maxx, maxy):
                                                              "com_y", "m0", "m2")])
     out["m2"] = out["m2"].reshape(slots, 3)
     return out
+
+
+def fmm_sym(xs, ys, tree, depth, theta):
+    """Symmetric dual-tree FMM with the t-SNE kernels (nBodySolverFMM_SYM_MORTON.h) over a
+    fixed_tree(): (repulsive derivative x, y per sorted point, total Z), FP64."""
+    xs, ys = _f64(xs), _f64(ys)
+    ax, ay = np.zeros(xs.size), np.zeros(xs.size)
+    m2 = np.ascontiguousarray(tree["m2"], dtype=np.float64).reshape(-1)
+    z = lib().orc_fmm_sym(_p(xs), _p(ys), xs.size, int(depth), _p(tree["first_child"]),
+                          _p(tree["first_particle"]), _p(tree["count"]), _p(tree["bb_len"]),
+                          _p(tree["com_x"]), _p(tree["com_y"]), _p(tree["m0"]), _p(m2),
+                          float(theta), _p(ax), _p(ay))
+    return ax, ay, z
 
 
 def _pre_arrays(cap):
