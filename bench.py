@@ -266,8 +266,74 @@ def tsne_run(args):
            "kl_initial": kl0, "kl_final": kl1, "z_final": z,
            "reference_cpu_s_per_iteration_published": 0.23,
            "speedup_vs_published_reference_repulsion": 0.23 / (rep_ms / 1e3)}
+    y_end = run.embeddedPoints
     run.close()
+    if not args.no_tsne_fmm:
+        out["fmm_sym"] = tsne_fmm_run(args, T, rp, col, val, y0, y_end)
     return out
+
+
+def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
+    """The same t-SNE run with the symmetric dual-tree FMM repulsion (fixed-depth Morton tree,
+    theta 0.5), then both repulsion solvers on the Barnes-Hut run's final embedding: walk time
+    (tree built beforehand) and median relative error against the FP64 direct sum over a seeded
+    sample of sinks."""
+    from masterthesislucas_amd import BHContext
+    from masterthesislucas_amd.workloads import sample_indices
+
+    n = args.tsne_n
+    run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0),
+                 solver="fmm_sym")
+    run.run(5)
+    run.close()
+    run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0),
+                 solver="fmm_sym")
+    run.ctx.profile_enable(True)
+    run.ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    run.run(args.tsne_iters)
+    wall = time.perf_counter() - t0
+    ph = run.ctx.profile_read(reset=True)
+    run.ctx.profile_enable(False)
+    kl1, _ = run.costFunction()
+    run.close()
+    rep_ms = sum(ph[k][0] for k in ("bounds+morton", "sort", "gather", "tree", "upward",
+                                     "traversal")) / args.tsne_iters
+    x = np.ascontiguousarray(y_end[:, 0])
+    y = np.ascontiguousarray(y_end[:, 1])
+    sample = sample_indices(n, args.error_sample)
+    walk, err, ref = {}, {}, None
+    for name, mode in (("bh", "quadtree"), ("fmm_sym", "morton_fixed")):
+        ctx = BHContext(n, kernel="tsne", tree_mode=mode)
+        if name != "bh":
+            ctx.set_solver(name)
+        ctx.set_particles(x, y)
+        ctx.build()
+        ax, ay, _ = ctx.forces(0.5)
+        if ref is None:
+            ref = ctx.direct_fp64(sample)
+        for _ in range(3):
+            ctx.forces_device_only(0.5)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ctx.forces_device_only(0.5)
+        ctx.synchronize()
+        walk[name] = (time.perf_counter() - t0) / 20 * 1e3
+        dx, dy = ref[0], ref[1]
+        e = np.hypot(ax[sample] - dx, ay[sample] - dy) / np.maximum(np.hypot(dx, dy), 1e-300)
+        err[name] = float(np.median(e))
+        ctx.close()
+    log(f"[bench] t-SNE FMM: {1e3 * wall / args.tsne_iters:.3f} ms/iter, walk "
+        f"{walk['fmm_sym']:.3f} ms vs BH {walk['bh']:.3f} ms, error {err['fmm_sym']:.2e} vs "
+        f"{err['bh']:.2e}")
+    return {"workload": "same P and start, symmetric dual-tree FMM repulsion, theta 0.5",
+            "ms_per_iteration": 1e3 * wall / args.tsne_iters,
+            "repulsion_ms_per_iteration": rep_ms,
+            "phase_ms_per_iteration": {k: round(v[0] / args.tsne_iters, 5) for k, v in ph.items()},
+            "kl_final": kl1,
+            "final_embedding_walk_ms": {k: round(v, 4) for k, v in walk.items()},
+            "final_embedding_median_rel_error": err}
 
 
 def main():
@@ -298,6 +364,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--no-tsne", action="store_true",
                     help="skip the t-SNE run (70,000 points, 1,000 iterations)")
+    ap.add_argument("--no-tsne-fmm", action="store_true",
+                    help="skip the t-SNE run with the symmetric dual-tree FMM repulsion")
     ap.add_argument("--tsne-n", type=int, default=70000)
     ap.add_argument("--tsne-iters", type=int, default=1000)
     ap.add_argument("--tsne-split", type=int, default=0,

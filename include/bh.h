@@ -263,6 +263,17 @@ bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
  * Batched contexts accept only 1; bh_forces_learned always walks with split 1. */
 bh_status bh_set_traverse_split(bh_ctx* ctx, int split);
 
+/* ---- force solver (NBodySolverFMM_SYM_MORTON, nBodySolverFMM_SYM_MORTON.h:28-316, 482-536,
+ * registered as "FMM_SYM_MORTON" in tsne.h:121) ----
+ * BH_SOLVER_BH: the Barnes-Hut walk (default). BH_SOLVER_FMM_SYM: the symmetric dual-tree FMM
+ * with the t-SNE kernels (node-node expansions to third order with the source's second moment,
+ * point-node, point-point; downward pass), in FP64, over the fixed-depth Morton tree. It needs a
+ * one-system context with BH_KERNEL_TSNE and BH_TREE_MORTON_FIXED; bh_forces, bh_forces_device
+ * and the bh_tsne_* steps then use it (theta > 0). Interaction counters read 0 for it. */
+#define BH_SOLVER_BH 0
+#define BH_SOLVER_FMM_SYM 1
+bh_status bh_set_solver(bh_ctx* ctx, int solver);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,7 @@ SIGNATURES = {
     "bh_traverse_variant_count": (_i32, []),
     "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
     "bh_set_traverse_split": (_i32, [_vp, C.c_int]),
+    "bh_set_solver": (_i32, [_vp, C.c_int]),
 }
 
 _lib = None

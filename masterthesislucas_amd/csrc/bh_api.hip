@@ -128,6 +128,9 @@ struct bh_ctx {
 
     int variant = 4;  // traversal kernel variant (see launch_traverse)
     uint32_t split = 1;         // waves per sink group in the traversal (one-system contexts)
+    int solver = BH_SOLVER_BH;  // force solver (BH_SOLVER_FMM_SYM: symmetric dual-tree FMM)
+    void* fmm_buf = nullptr;    // FMM workspace (fmm_workspace_bytes: per-slot moments,
+                                // coefficients, interaction lists; per-point shares)
     float* part = nullptr;      // split traversal outputs: x, y, total parts, F n_max each
     uint32_t part_split = 0;    // F the part buffer was sized for
     uint32_t trav_blocks = 0;  // blocks of the last traversal launch
@@ -284,9 +287,25 @@ TraverseBatch batch_of(const bh_ctx* c, int ns_index) {
     return b;
 }
 
+bh_status do_fmm(bh_ctx* c, double theta) {
+    if (!(theta > 0.0)) return fail(c, BH_ERR_ARG, "the FMM solver needs theta > 0");
+    hipStream_t s = c->stream;
+    phase_begin(c, 5);
+    const uint32_t* keys = (const uint32_t*)(c->keys_in_b ? c->keys_b : c->keys_a);
+    launch_tree_slots(s, keys, c->nodes, c->prm, c->cap_nodes, c->depth, c->slot_a, c->slot_b,
+                      c->slot_c, fmm_slots(c->depth));
+    launch_fmm(s, c->pos4, keys, c->slot_b, c->slot_c, c->depth, c->slot_cap, c->n, c->cfg.n_max,
+               c->prm, theta, c->fmm_buf, c->ax, c->ay, c->dsum);
+    phase_end(c, 5);
+    c->trav_blocks = 0;  // interaction counters are those of the tree walk only
+    c->have_acc = true;
+    return launch_status(c, "FMM launch");
+}
+
 bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     if (!c->built) return fail(c, BH_ERR_STATE, "forces requested before a tree was built");
     if (!(theta >= 0.0)) return fail(c, BH_ERR_ARG, "theta must be >= 0");
+    if (c->solver == BH_SOLVER_FMM_SYM) return do_fmm(c, theta);
     hipStream_t s = c->stream;
     const float inv_theta2 = (float)(1.0 / (theta * theta));
     const float eps = c->cfg.softening;
@@ -352,6 +371,7 @@ void bh_destroy(bh_ctx* c) {
     if (c->p_val) (void)hipFree(c->p_val);
     if (c->dpart) (void)hipFree(c->dpart);
     if (c->part) (void)hipFree(c->part);
+    if (c->fmm_buf) (void)hipFree(c->fmm_buf);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -1210,6 +1230,27 @@ bh_status bh_set_traverse_variant(bh_ctx* c, int variant) {
 }
 
 int bh_traverse_variant_count(void) { return kTraverseVariants; }
+
+bh_status bh_set_solver(bh_ctx* c, int solver) {
+    if (!c) return BH_ERR_ARG;
+    if (solver != BH_SOLVER_BH && solver != BH_SOLVER_FMM_SYM)
+        return fail(c, BH_ERR_ARG, "unknown solver %d", solver);
+    if (solver == BH_SOLVER_FMM_SYM) {
+        if (c->wide || c->nsys || c->cfg.kernel != BH_KERNEL_TSNE)
+            return fail(c, BH_ERR_ARG,
+                        "the FMM solver runs the t-SNE kernels on the fixed-depth Morton tree: "
+                        "one system, BH_KERNEL_TSNE, BH_TREE_MORTON_FIXED");
+        if (!c->fmm_buf) {
+            const uint64_t bytes = fmm_workspace_bytes(c->slot_cap, c->cfg.n_max);
+            bh_status st = hip_check(c, hipMalloc(&c->fmm_buf, bytes), "hipMalloc FMM buffers");
+            if (st != BH_OK) return st;
+        }
+    }
+    if (solver != c->solver) c->g_valid = false;
+    c->solver = solver;
+    c->have_acc = false;
+    return BH_OK;
+}
 
 bh_status bh_set_traverse_split(bh_ctx* c, int split) {
     if (!c) return BH_ERR_ARG;

@@ -101,6 +101,16 @@ uint32_t reduce_grid(uint32_t n);
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
                        double* out);
 uint32_t traverse_blocks(uint32_t n);
+// symmetric dual-tree FMM with the t-SNE kernels (FP64) over the fixed-depth slot tree: slot
+// arrays have slot_stride entries (>= fmm_slots(depth)); codes are the sorted Morton codes;
+// workspace holds fmm_workspace_bytes(slot_stride, n_max) bytes. Writes ax, ay (sorted order,
+// FP32) and *total (Z).
+uint64_t fmm_slots(int depth);
+uint64_t fmm_workspace_bytes(uint64_t slot_stride, uint64_t n_max);
+void launch_fmm(hipStream_t s, const float4* pos4, const uint32_t* codes, const uint32_t* first,
+                const uint32_t* count, int depth, uint64_t slot_stride, uint32_t n, uint64_t n_max,
+                const BuildParams* prm, double theta, void* workspace, float* ax, float* ay,
+                double* total);
 // streams the node records and sorted positions into every XCD's L2 when they fit (else no-op)
 void launch_warm_l2(hipStream_t s, const Node* nodes, const BuildParams* prm, const float4* pos4,
                     uint32_t n, uint32_t* sink);

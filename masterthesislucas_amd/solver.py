@@ -224,6 +224,14 @@ class BHContext:
         """Traversal variant in use (the library starts with 4: two sinks per lane)."""
         return getattr(self, "_variant", 4)
 
+    SOLVERS = {"bh": 0, "fmm_sym": 1}
+
+    def set_solver(self, solver):
+        """'bh' (Barnes-Hut walk) or 'fmm_sym' (symmetric dual-tree FMM, t-SNE kernels,
+        morton_fixed tree); see bh_set_solver."""
+        self._c(_lib.load().bh_set_solver(self._h, self.SOLVERS[solver]))
+        self._solver = solver
+
     def set_split(self, split):
         """Waves per sink group in the traversal (1..8); see bh_set_traverse_split."""
         self._c(_lib.load().bh_set_traverse_split(self._h, int(split)))

@@ -144,7 +144,7 @@ class TSNE:
 
     def __init__(self, row_ptr, col, val, y0, min_theta=0.5, max_theta=0.5,
                  exaggeration=(16.0, 4.0), momentum=(0.5, 0.8), switch=250, max_children=16,
-                 learn_rate=None):
+                 learn_rate=None, solver="bh"):
         row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
         col = np.ascontiguousarray(col, np.uint32)
         val = np.ascontiguousarray(val, np.float32)
@@ -154,7 +154,13 @@ class TSNE:
         if row_ptr.size != n + 1:
             raise ValueError("row_ptr must hold n + 1 entries")
         self.n = n
-        self.ctx = BHContext(n, kernel="tsne", max_children=max_children, theta=max_theta)
+        # "bh": Barnes-Hut over the adaptive QuadTree (NBodySolverBH); "fmm_sym": the symmetric
+        # dual-tree FMM over the fixed-depth Morton tree (NBodySolverFMM_SYM_MORTON)
+        mode = "morton_fixed" if solver == "fmm_sym" else "quadtree"
+        self.ctx = BHContext(n, kernel="tsne", max_children=max_children, theta=max_theta,
+                             tree_mode=mode)
+        if solver != "bh":
+            self.ctx.set_solver(solver)
         L = _lib.load()
         p = _lib.TsneParams()
         self.ctx._c(L.bh_tsne_params_default(C.byref(p), n, float(max_theta)))

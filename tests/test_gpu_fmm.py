@@ -1,0 +1,141 @@
+"""Symmetric dual-tree FMM on the GPU (bh_set_solver(BH_SOLVER_FMM_SYM)) against the oracle's
+FP64 restatement of NBodySolverFMM_SYM_MORTON (oracle/bh_oracle.c: orc_fmm_sym), on the same
+fixed-depth Morton tree (codes, order and slots are bit-exact, checked in
+test_gpu_parity.py). Both sides compute in FP64 and differ only in summation order; the GPU
+returns the derivative in FP32, so the bar is FP32 rounding: median 1e-6 and p99 1e-5 relative
+per particle, Z to 1e-10 relative."""
+import numpy as np
+import pytest
+
+from masterthesislucas_amd import BHContext, tsne
+from masterthesislucas_amd._lib import BHError
+from oracle import oracle as O
+from tests.helpers import oracle_tree, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_fmm(x, y, theta):
+    ctx = BHContext(x.size, kernel="tsne", tree_mode="morton_fixed")
+    ctx.set_solver("fmm_sym")
+    ctx.set_particles(x, y)
+    ctx.build()
+    ax, ay, z = ctx.forces(theta)
+    ctx.close()
+    return ax, ay, z
+
+
+def _oracle_fmm(x, y, theta):
+    t = oracle_tree(x, y, None, mode="morton_fixed")
+    minx, miny, maxx, maxy = t["bounds"]
+    tree = O.fixed_tree(t["xs"], t["ys"], t["depth"], minx, miny, maxx, maxy)
+    sx, sy, z = O.fmm_sym(t["xs"], t["ys"], tree, t["depth"], theta)
+    ax, ay = np.empty(x.size), np.empty(x.size)
+    ax[t["perm"]] = sx
+    ay[t["perm"]] = sy
+    return ax, ay, z
+
+
+def _points(n, seed, kind="gauss"):
+    rng = np.random.default_rng(seed)
+    if kind == "gauss":
+        p = rng.normal(size=(n, 2))
+    else:  # clusters of very different density, some exactly coincident points
+        k = rng.integers(0, 4, n)
+        centres = np.array([[0, 0], [6, 0], [0, 9], [30, 30]])
+        scale = np.array([1.0, 0.05, 0.3, 2.0])
+        p = centres[k] + rng.normal(size=(n, 2)) * scale[k][:, None]
+        p[: n // 20] = p[0]
+    return p[:, 0].astype(np.float32), p[:, 1].astype(np.float32)
+
+
+@pytest.mark.parametrize("n,theta,kind", [(1000, 0.5, "gauss"), (20000, 0.3, "gauss"),
+                                          (20000, 0.5, "clusters"), (20000, 1.0, "gauss")])
+def test_fmm_matches_oracle(gpu, n, theta, kind):
+    x, y = _points(n, 70 + n, kind)
+    ax, ay, z = _gpu_fmm(x, y, theta)
+    bx, by, bz = _oracle_fmm(x, y, theta)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= 1e-6 and np.quantile(r, 0.99) <= 1e-5, (np.median(r), r.max())
+    assert abs(z - bz) <= 1e-10 * abs(bz)
+
+
+def test_fmm_small_theta_is_the_exact_sum(gpu):
+    x, y = _points(5000, 71)
+    ax, ay, z = _gpu_fmm(x, y, 1e-6)
+    ex, ey, et = O.direct(x.astype(np.float64), y.astype(np.float64), None, O.KERNEL_TSNE)
+    r = rel_err(ax, ay, ex, ey)
+    assert r.max() <= 1e-5
+    assert abs(z - et.sum()) <= 1e-10 * et.sum()
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 100])
+def test_fmm_small_systems(gpu, n):
+    x, y = _points(n, 72 + n)
+    ax, ay, z = _gpu_fmm(x, y, 0.5)
+    bx, by, bz = _oracle_fmm(x, y, 0.5)
+    assert np.allclose(ax, bx, rtol=1e-5, atol=1e-7) and np.allclose(ay, by, rtol=1e-5, atol=1e-7)
+    assert abs(z - bz) <= 1e-10 * max(abs(bz), 1.0)
+
+
+def test_fmm_is_more_accurate_than_barnes_hut(gpu):
+    """The thesis' point: at the same theta the symmetric FMM (third-order expansions with the
+    source's second moment) errs several times less than the monopole Barnes-Hut walk (2.5x
+    here at theta = 0.5; grapher.ipynb reports 3x on MNIST)."""
+    x, y = _points(20000, 73)
+    ex, ey, _ = O.direct(x.astype(np.float64), y.astype(np.float64), None, O.KERNEL_TSNE)
+    fx, fy, _ = _gpu_fmm(x, y, 0.5)
+    ctx = BHContext(x.size, kernel="tsne")
+    ctx.set_particles(x, y)
+    ctx.build()
+    bx, by, _ = ctx.forces(0.5)
+    ctx.close()
+    e_fmm = np.median(rel_err(fx, fy, ex, ey))
+    e_bh = np.median(rel_err(bx, by, ex, ey))
+    assert e_fmm < 0.5 * e_bh, (e_fmm, e_bh)
+
+
+def test_fmm_solver_rejects_other_setups(gpu):
+    ctx = BHContext(100, kernel="tsne")  # adaptive QuadTree
+    with pytest.raises(BHError):
+        ctx.set_solver("fmm_sym")
+    ctx.close()
+    ctx = BHContext(100, kernel="gravity", tree_mode="morton_fixed")
+    with pytest.raises(BHError):
+        ctx.set_solver("fmm_sym")
+    ctx.close()
+    ctx = BHContext(100, kernel="tsne", tree_mode="morton_fixed")
+    ctx.set_solver("fmm_sym")
+    ctx.set_particles(*_points(100, 74))
+    ctx.build()
+    with pytest.raises(BHError):
+        ctx.forces(0.0)
+    ctx.set_solver("bh")  # back to the Barnes-Hut walk on the same tree
+    ctx.forces(0.5)
+    ctx.close()
+
+
+def test_tsne_step_with_fmm_matches_oracle(gpu):
+    """One t-SNE iteration with the FMM repulsion: d = 4/Z rep + attraction, y' = y + lr d
+    (iteration 0, no momentum), against the oracle's FMM and attraction."""
+    n = 3000
+    X, _ = tsne.gaussian_mixture(n, dim=30, seed=75)
+    rp, col, val = tsne.joint_probabilities(X, perplexity=20.0, k=60)
+    yx, yy = tsne.initial_embedding(n, seed=75, std=1.0)
+    t = tsne.TSNE(rp, col, val, (yx, yy), min_theta=0.5, max_theta=0.5, solver="fmm_sym")
+    try:
+        dx, dy, z = t.timeStep(return_derivative=True)
+        y1 = t.embeddedPoints.astype(np.float64)
+    finally:
+        t.close()
+    rx, ry, oz = _oracle_fmm(yx, yy, 0.5)
+    assert abs(z - oz) <= 1e-10 * oz
+    x0, y0 = yx.astype(np.float64), yy.astype(np.float64)
+    atx, aty = O.tsne_attraction(x0, y0, rp, col, val, 16.0)
+    odx, ody = 4.0 / oz * rx + atx, 4.0 / oz * ry + aty
+    scale = np.hypot(np.abs(4.0 / oz * rx) + np.abs(atx), np.abs(4.0 / oz * ry) + np.abs(aty))
+    e = np.hypot(dx - odx, dy - ody) / scale
+    assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-4
+    lr = n / 15.0
+    e = np.hypot(y1[:, 0] - (x0 + lr * odx), y1[:, 1] - (y0 + lr * ody)) / (lr * scale + 1e-6 * np.hypot(x0, y0))
+    assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-4
