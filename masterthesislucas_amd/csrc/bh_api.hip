@@ -46,6 +46,7 @@ struct bh_ctx {
     uint32_t n = 0;
     int cur = 0;
     bool built = false;
+    bool moments = false;   // node records hold the upward pass of the last build
     bool have_acc = false;  // ax/ay valid for the current order (leapfrog)
     bool wide = true;       // 64-bit codes
     int B = 32;             // code bits per axis
@@ -213,6 +214,8 @@ int effective_depth(const bh_ctx* c) {
     return c->cfg.fixed_depth > 0 ? c->cfg.fixed_depth : reference_depth(c->cfg.max_children, c->n);
 }
 
+void launch_node_moments(bh_ctx* c);
+
 bh_status do_build(bh_ctx* c, const double* bounds) {
     const uint32_t n = c->n;
     if (n == 0) return fail(c, BH_ERR_STATE, "no particles");
@@ -266,15 +269,30 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     if (c->nsys) launch_batch_roots(s, c->sys_off, c->nsys, c->noff, c->roots);
     phase_end(c, 3);
 
-    phase_begin(c, 4);
-    launch_upward(s, c->pos4, c->nodes, c->child4, c->learned ? c->m2 : nullptr, c->prm, c->deferred,
-                  c->cap_nodes);
-    phase_end(c, 4);
+    // the FMM solver computes its own moments: the node records' moments are made when a
+    // consumer of them asks (ensure_moments)
+    c->moments = false;
+    if (c->solver != BH_SOLVER_FMM_SYM) launch_node_moments(c);
     bh_status st = launch_status(c, "build launch");
     if (st != BH_OK) return st;
     c->built = true;
     c->have_acc = false;
     return BH_OK;
+}
+
+// upward pass over the node records (mass, centre of mass, second moments when learned)
+void launch_node_moments(bh_ctx* c) {
+    phase_begin(c, 4);
+    launch_upward(c->stream, c->pos4, c->nodes, c->child4, c->learned ? c->m2 : nullptr, c->prm,
+                  c->deferred, c->cap_nodes);
+    phase_end(c, 4);
+    c->moments = true;
+}
+
+bh_status ensure_moments(bh_ctx* c) {
+    if (c->moments) return BH_OK;
+    launch_node_moments(c);
+    return launch_status(c, "upward launch");
 }
 
 TraverseBatch batch_of(const bh_ctx* c, int ns_index) {
@@ -306,6 +324,8 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     if (!c->built) return fail(c, BH_ERR_STATE, "forces requested before a tree was built");
     if (!(theta >= 0.0)) return fail(c, BH_ERR_ARG, "theta must be >= 0");
     if (c->solver == BH_SOLVER_FMM_SYM) return do_fmm(c, theta);
+    bh_status mst = ensure_moments(c);
+    if (mst != BH_OK) return mst;
     hipStream_t s = c->stream;
     const float inv_theta2 = (float)(1.0 / (theta * theta));
     const float eps = c->cfg.softening;
@@ -1136,6 +1156,7 @@ bh_status bh_get_tree(bh_ctx* c, int32_t* level, uint32_t* first, uint32_t* coun
                       uint8_t* flags, float* cx, float* cy, float* mass) {
     uint32_t nn = 0;
     bh_status st = bh_tree_info(c, &nn, nullptr, nullptr);
+    if (st == BH_OK) st = ensure_moments(c);
     if (st != BH_OK) return st;
     std::vector<Node> h(nn);
     if (nn) {
@@ -1631,6 +1652,7 @@ bh_status bh_learned_collect(bh_ctx* c, bh_mlp* m, double theta, double band,
     if (!c || !m || (!sinks && ns) || !(theta > 0.0) || band < 0.0 || max_pairs_per_sink == 0)
         return BH_ERR_ARG;
     if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
+    if (ensure_moments(c) != BH_OK) return BH_ERR_HIP;
     if (ns == 0) return BH_OK;
     for (uint32_t k = 0; k < ns; ++k)
         if (sinks[k] >= c->n) return fail(c, BH_ERR_ARG, "sink index %u out of range", sinks[k]);
@@ -1684,6 +1706,7 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
                             float* ay, double* total) {
     if (!c || !m || !(theta > 0.0) || band < 0.0) return BH_ERR_ARG;
     if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
+    if (ensure_moments(c) != BH_OK) return BH_ERR_HIP;
     hipStream_t s = c->stream;
     const float inv_t2 = (float)(1.0 / (theta * theta));
     const float inv_b2 = band > 0.0 ? (float)(1.0 / (band * band)) : 0.f;

@@ -1100,6 +1100,18 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
     if (i < n) p[i] = v;
 }
 
+// the three slot arrays cleared in one launch
+__global__ __launch_bounds__(kBlock) void k_clear_slots(uint32_t* __restrict__ a,
+                                                        uint32_t* __restrict__ b,
+                                                        uint32_t* __restrict__ c, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        a[i] = 0u;
+        b[i] = 0u;
+        c[i] = 0u;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ p, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = i;
@@ -1309,9 +1321,8 @@ void launch_iota(hipStream_t s, uint32_t* p, uint32_t n) {
 void launch_tree_slots(hipStream_t s, const uint32_t* codes, const Node* nodes,
                        const BuildParams* prm, uint32_t cap_nodes, int depth, uint32_t* first_child,
                        uint32_t* first_particle, uint32_t* count, uint64_t slots) {
-    k_fill_u32<<<div_up(slots, kBlock), kBlock, 0, s>>>(first_child, slots, 0u);
-    k_fill_u32<<<div_up(slots, kBlock), kBlock, 0, s>>>(first_particle, slots, 0u);
-    k_fill_u32<<<div_up(slots, kBlock), kBlock, 0, s>>>(count, slots, 0u);
+    k_clear_slots<<<div_up(slots, kBlock), kBlock, 0, s>>>(first_child, first_particle, count,
+                                                           slots);
     k_tree_slots<<<div_up(cap_nodes, kBlock), kBlock, 0, s>>>(codes, nodes, prm, cap_nodes, depth,
                                                               first_child, first_particle, count);
 }

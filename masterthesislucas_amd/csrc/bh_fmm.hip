@@ -197,55 +197,107 @@ __global__ __launch_bounds__(kBlock) void k_fmm_leaf_moments(const float4* __res
 
 // bottomUpNodeConstruction (:381-470): mass-weighted COM (children taken from the last), then
 // M2 = sum over children of (com_c - com)(com_c - com)^T + M2_c (not weighted by M0_c)
-__device__ __forceinline__ void fmm_up_node(const uint32_t* __restrict__ count, int l, uint32_t i,
-                                            uint32_t S, double* __restrict__ mom) {
-    FmmMoments M = fmm_moments(mom, S);
-    const uint32_t s = fmm_level_index(l) + i, pfc = fmm_level_index(l + 1) + 4 * i;
-    double m0 = 0.0, cx = 0.0, cy = 0.0, xx = 0.0, xy = 0.0, yy = 0.0;
+struct FmmMom {
+    double m0, cx, cy, xx, xy, yy;
+};
+
+__device__ __forceinline__ FmmMom fmm_up_combine(const FmmMom* ch) {
+    FmmMom r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = 3; j >= 0; --j) {
-        const uint32_t c = pfc + j;
-        if (count[c] == 0u) continue;
-        cx += M.m0[c] * M.cx[c];
-        cy += M.m0[c] * M.cy[c];
-        m0 += M.m0[c];
+        if (ch[j].m0 == 0.0) continue;
+        r.cx += ch[j].m0 * ch[j].cx;
+        r.cy += ch[j].m0 * ch[j].cy;
+        r.m0 += ch[j].m0;
     }
-    if (count[s] != 0u) {
-        cx /= m0;
-        cy /= m0;
-        for (uint32_t c = pfc; c < pfc + 4; ++c) {
-            if (M.m0[c] == 0.0) continue;
-            const double rx = M.cx[c] - cx, ry = M.cy[c] - cy;
-            xx += rx * rx;
-            xy += rx * ry;
-            yy += ry * ry;
-            xx += M.xx[c];
-            xy += M.xy[c];
-            yy += M.yy[c];
+    if (r.m0 != 0.0) {
+        r.cx /= r.m0;
+        r.cy /= r.m0;
+        for (int j = 0; j < 4; ++j) {
+            if (ch[j].m0 == 0.0) continue;
+            const double rx = ch[j].cx - r.cx, ry = ch[j].cy - r.cy;
+            r.xx += rx * rx;
+            r.xy += rx * ry;
+            r.yy += ry * ry;
+            r.xx += ch[j].xx;
+            r.xy += ch[j].xy;
+            r.yy += ch[j].yy;
         }
     }
-    M.m0[s] = m0;
-    M.cx[s] = cx;
-    M.cy[s] = cy;
-    M.xx[s] = xx;
-    M.xy[s] = xy;
-    M.yy[s] = yy;
+    return r;
+}
+
+__device__ __forceinline__ FmmMom fmm_up_node(int l, uint32_t i, uint32_t S,
+                                              double* __restrict__ mom) {
+    FmmMoments M = fmm_moments(mom, S);
+    const uint32_t s = fmm_level_index(l) + i, pfc = fmm_level_index(l + 1) + 4 * i;
+    FmmMom ch[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        ch[j] = FmmMom{M.m0[pfc + j], M.cx[pfc + j], M.cy[pfc + j], M.xx[pfc + j], M.xy[pfc + j],
+                       M.yy[pfc + j]};
+    const FmmMom r = fmm_up_combine(ch);
+    M.m0[s] = r.m0;
+    M.cx[s] = r.cx;
+    M.cy[s] = r.cy;
+    M.xx[s] = r.xx;
+    M.xy[s] = r.xy;
+    M.yy[s] = r.yy;
+    return r;
 }
 
 // levels with at most kFmmTopNodes nodes are done by one block, level by level
 constexpr uint32_t kFmmTopNodes = 1024;
 constexpr int kFmmTopLevel = 5;  // 4^5 = kFmmTopNodes
 
-__global__ __launch_bounds__(kBlock) void k_fmm_up(const uint32_t* __restrict__ count, int l,
-                                                   uint32_t S, double* __restrict__ mom) {
+__global__ __launch_bounds__(kBlock) void k_fmm_up(int l, uint32_t S, double* __restrict__ mom) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < (1u << (2 * l))) fmm_up_node(count, l, i, S, mom);
+    if (i < (1u << (2 * l))) (void)fmm_up_node(l, i, S, mom);
 }
 
-__global__ __launch_bounds__(kFmmTopNodes) void k_fmm_up_top(const uint32_t* __restrict__ count,
-                                                             int top, uint32_t S,
+// levels top..0 in one block: level top from memory, the levels above it from LDS
+__global__ __launch_bounds__(kFmmTopNodes) void k_fmm_up_top(int top, uint32_t S,
                                                              double* __restrict__ mom) {
-    for (int l = top; l >= 0; --l) {
-        if (threadIdx.x < (1u << (2 * l))) fmm_up_node(count, l, threadIdx.x, S, mom);
+    __shared__ double sm[6][kFmmTopNodes];
+    const uint32_t i = threadIdx.x;
+    FmmMoments M = fmm_moments(mom, S);
+    if (i < (1u << (2 * top))) {
+        const FmmMom r = fmm_up_node(top, i, S, mom);
+        sm[0][i] = r.m0;
+        sm[1][i] = r.cx;
+        sm[2][i] = r.cy;
+        sm[3][i] = r.xx;
+        sm[4][i] = r.xy;
+        sm[5][i] = r.yy;
+    }
+    __syncthreads();
+    for (int l = top - 1; l >= 0; --l) {
+        const bool act = i < (1u << (2 * l));
+        FmmMom r{};
+        if (act) {
+            FmmMom ch[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t c = 4 * i + j;
+                ch[j] = FmmMom{sm[0][c], sm[1][c], sm[2][c], sm[3][c], sm[4][c], sm[5][c]};
+            }
+            r = fmm_up_combine(ch);
+        }
+        __syncthreads();
+        if (act) {
+            sm[0][i] = r.m0;
+            sm[1][i] = r.cx;
+            sm[2][i] = r.cy;
+            sm[3][i] = r.xx;
+            sm[4][i] = r.xy;
+            sm[5][i] = r.yy;
+            const uint32_t s = fmm_level_index(l) + i;
+            M.m0[s] = r.m0;
+            M.cx[s] = r.cx;
+            M.cy[s] = r.cy;
+            M.xx[s] = r.xx;
+            M.xy[s] = r.xy;
+            M.yy[s] = r.yy;
+        }
         __syncthreads();
     }
 }
@@ -301,7 +353,9 @@ constexpr uint32_t kFmmOverflow = 0xFFFFFFFFu;
 // the embedding in a fixed-depth leaf) is spread over many waves. F_X grows with the number of
 // points X's chunks meet (its leaf partners, itself included).
 constexpr uint32_t kFmmMaxSplit = 16;
-constexpr uint32_t kFmmUnitSources = 512;   // partner points per unit aimed at
+constexpr uint32_t kFmmUnitSources = 448;   // partner points per unit aimed at
+constexpr uint32_t kFmmStage = 512;         // sources a wave stages in LDS (>= kFmmUnitSources + 64)
+constexpr uint32_t kFmmExpSources = 2048;   // lower-partner points per expansion unit aimed at
 constexpr uint32_t kFmmScanThreads = 1024;
 constexpr uint32_t kFmmUnitBlocks = 4096;   // blocks of the persistent leaf-unit kernel
 
@@ -330,6 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
                                                       uint32_t* __restrict__ lists,
                                                       uint32_t* __restrict__ nlist,
                                                       uint32_t* __restrict__ units,
+                                                      uint32_t unit_stride,
                                                       uint32_t* __restrict__ split) {
     const int lane = threadIdx.x & 63;
     const uint32_t ix = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -341,6 +396,8 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
             tot[s] = 0.0;
             if (l == depth) {
                 units[ix] = 0;
+                units[unit_stride + ix] = 0;
+                units[2 * unit_stride + ix] = 0;
                 split[ix] = 1;
             }
         }
@@ -355,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
     uint32_t* out = lists + (size_t)s * kFmmListCap;
     FmmCoef c = {};
     double t = 0.0;
-    uint32_t meet = 0, n_out = 0;
+    uint32_t meet = 0, lower = 0, n_out = 0;  // lower: points of partners of lower slot
     const uint32_t par = l > 0 ? fmm_level_index(l - 1) + (ix >> 2) : 0u;
     const uint32_t np = l > 0 ? nlist[par] : 1u;
     if (np != kFmmOverflow) {
@@ -379,10 +436,12 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
                 const uint32_t at = n_out + (uint32_t)__popcll(kb & lane_mask_lt());
                 if (at < kFmmListCap) out[at] = y;
                 meet += count[y];
+                if (y < s) lower += count[y];
             }
             n_out += (uint32_t)__popcll(kb);
         }
         meet = (uint32_t)wave_sum_u64(meet);
+        lower = (uint32_t)wave_sum_u64(lower);
     } else {
         // the parent's list overflowed: walk from the root (uniform), lanes take turns with the
         // accepted pairs
@@ -396,6 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
             if (n_out < kFmmListCap && lane == 0) out[n_out] = y;
             ++n_out;
             meet += count[y];
+            if (y < s) lower += count[y];
         });
     }
 #pragma unroll
@@ -413,90 +473,157 @@ __global__ __launch_bounds__(kBlock) void k_fmm_level(const uint32_t* __restrict
             const uint32_t chunks = (count[s] + kWave - 1) / kWave;
             uint32_t f = (meet + kFmmUnitSources - 1) / kFmmUnitSources;
             f = f < 1u ? 1u : (f > kFmmMaxSplit ? kFmmMaxSplit : f);
-            units[ix] = chunks * f;
+            // light: at most 64 partners whose shares fit the stage buffer (sum over partners of
+            // ceil(n_Y / f) <= (meet + n_out f) / f)
+            const bool light = n_out <= (uint32_t)kWave && meet + n_out * f <= kFmmStage * f;
+            units[ix] = light ? chunks * f : 0u;
+            units[unit_stride + ix] = light ? 0u : chunks * f;
             split[ix] = f;
+            // expansion units: shares of the lower-slot partners' points, at most 1 + n_X/4
+            // per leaf (which bounds their partial sums by 14 (leaves + n/4) doubles)
+            uint32_t e = (lower + kFmmExpSources - 1) / kFmmExpSources;
+            const uint32_t ecap = 1u + count[s] / 4u;
+            e = e > ecap ? ecap : e;
+            units[2 * unit_stride + ix] = e > kFmmMaxSplit ? kFmmMaxSplit : e;
         }
     }
 }
 
-// every leaf partner Y of leaf X (slot X, level-local index ix), from the list or by the walk
+__device__ __forceinline__ double fmm_readlane_d(double v, uint32_t k) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// every leaf partner Y of leaf X with its point range [fY, end) and centre: the list entries
+// are fetched 64 at a time (one per lane, with their ranges and centres) and handed out with
+// readlane, so stepping from one partner to the next waits on no load
 template <typename Fn>
-__device__ __forceinline__ void fmm_leaf_partners(const uint32_t* __restrict__ count,
+__device__ __forceinline__ void fmm_partners_meta(const uint32_t* __restrict__ first,
+                                                  const uint32_t* __restrict__ count,
                                                   FmmMoments M, double side, double theta,
                                                   int depth, uint32_t X, uint32_t ix,
                                                   const uint32_t* __restrict__ lists,
                                                   const uint32_t* __restrict__ nlist, Fn&& fn) {
+    const int lane = threadIdx.x & 63;
     const uint32_t np = nlist[X];
     if (np != kFmmOverflow) {
         const uint32_t* in = lists + (size_t)X * kFmmListCap;
-        for (uint32_t j = 0; j < np; ++j) fn(in[j]);
+        for (uint32_t b0 = 0; b0 < np; b0 += kWave) {
+            const uint32_t j = b0 + (uint32_t)lane;
+            const uint32_t yl = j < np ? in[j] : X;
+            const uint32_t fl = first[yl], nl = count[yl];
+            const double cxl = M.cx[yl], cyl = M.cy[yl];
+            const uint32_t nb = np - b0 < (uint32_t)kWave ? np - b0 : (uint32_t)kWave;
+            for (uint32_t k = 0; k < nb; ++k) {
+                const uint32_t Y = (uint32_t)__builtin_amdgcn_readlane((int)yl, (int)k);
+                const uint32_t fY = (uint32_t)__builtin_amdgcn_readlane((int)fl, (int)k);
+                const uint32_t nY = (uint32_t)__builtin_amdgcn_readlane((int)nl, (int)k);
+                fn(Y, fY, fY + nY, fmm_readlane_d(cxl, k), fmm_readlane_d(cyl, k));
+            }
+        }
     } else {
         fmm_walk(count, M, side, theta, depth, ix,
                  [&](int k, uint32_t y, bool acc, double, double) {
-                     if (k == depth && !acc) fn(y);
+                     if (k == depth && !acc) fn(y, first[y], first[y] + count[y], M.cx[y], M.cy[y]);
                  });
     }
 }
 
-// exclusive scan of the per-leaf unit counts in place; units[L] = the total (one block).
-// Tiles of kFmmScanThreads entries, kFmmScanBatch tiles loaded together, scanned in order.
-constexpr int kFmmScanBatch = 16;
+// exclusive scans of the per-leaf unit counts of the three unit classes (light, heavy,
+// expansion: units + c * unit_stride, unit_stride a multiple of 4) in place; entry L of each =
+// its total (one block). Each thread sums a contiguous run of entries (a multiple of 16, read
+// as uint4, four loads in flight), the run totals are scanned over the block, and the runs are
+// rewritten with their offsets.
+constexpr int kFmmScanArrays = 3;
 
-__global__ __launch_bounds__(kFmmScanThreads) void k_fmm_scan(uint32_t* __restrict__ units,
-                                                              uint32_t L) {
-    __shared__ uint32_t wsum[kFmmScanBatch][kFmmScanThreads / kWave];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    constexpr int kW = kFmmScanThreads / kWave;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < L; base += kFmmScanThreads * kFmmScanBatch) {
-        uint32_t v[kFmmScanBatch], x[kFmmScanBatch];
-#pragma unroll
-        for (int k = 0; k < kFmmScanBatch; ++k) {
-            const uint32_t i = base + k * kFmmScanThreads + threadIdx.x;
-            v[k] = i < L ? units[i] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kFmmScanBatch; ++k) {
-            x[k] = v[k];
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                const uint32_t y = __shfl_up(x[k], d, kWave);
-                if (lane >= d) x[k] += y;
-            }
-            if (lane == kWave - 1) wsum[k][w] = x[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kFmmScanBatch; ++k) {
-            uint32_t off = carry, all = 0;
-            for (int j = 0; j < kW; ++j) {
-                const uint32_t s = wsum[k][j];
-                off += j < w ? s : 0u;
-                all += s;
-            }
-            const uint32_t i = base + k * kFmmScanThreads + threadIdx.x;
-            if (i < L) units[i] = off + x[k] - v[k];
-            carry += all;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) units[L] = carry;
+__device__ __forceinline__ uint4 fmm_load4(const uint32_t* __restrict__ u, uint32_t i, uint32_t L) {
+    if (i + 3 < L) return *reinterpret_cast<const uint4*>(u + i);
+    return make_uint4(i < L ? u[i] : 0u, i + 1 < L ? u[i + 1] : 0u, i + 2 < L ? u[i + 2] : 0u,
+                      i + 3 < L ? u[i + 3] : 0u);
 }
 
-// 1 / d for d >= 1: hardware reciprocal estimate refined by two Newton-Raphson steps
-// (r <- r + r (1 - d r)), which leaves it within an ulp or two of the FP64 quotient
+__global__ __launch_bounds__(kFmmScanThreads) void k_fmm_scan(uint32_t* __restrict__ units,
+                                                              uint32_t unit_stride, uint32_t L) {
+    constexpr int kW = kFmmScanThreads / kWave;
+    __shared__ uint32_t wsum[kFmmScanArrays][kW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t per = ((L + kFmmScanThreads - 1) / kFmmScanThreads + 15) & ~15u;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per;  // entries >= L read as 0 and are not written
+    uint32_t own[kFmmScanArrays], x[kFmmScanArrays];
+#pragma unroll
+    for (int a = 0; a < kFmmScanArrays; ++a) {
+        const uint32_t* u = units + a * unit_stride;
+        uint32_t s = 0;
+        for (uint32_t i = b; i < e && i < L; i += 16) {
+            const uint4 v0 = fmm_load4(u, i, L), v1 = fmm_load4(u, i + 4, L),
+                        v2 = fmm_load4(u, i + 8, L), v3 = fmm_load4(u, i + 12, L);
+            s += (v0.x + v0.y + v0.z + v0.w) + (v1.x + v1.y + v1.z + v1.w) +
+                 (v2.x + v2.y + v2.z + v2.w) + (v3.x + v3.y + v3.z + v3.w);
+        }
+        own[a] = s;
+    }
+#pragma unroll
+    for (int a = 0; a < kFmmScanArrays; ++a) {
+        x[a] = own[a];
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(x[a], d, kWave);
+            if (lane >= d) x[a] += y;
+        }
+        if (lane == kWave - 1) wsum[a][w] = x[a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < kFmmScanArrays; ++a) {
+        uint32_t off = 0, all = 0;
+        for (int j = 0; j < kW; ++j) {
+            const uint32_t s = wsum[a][j];
+            off += j < w ? s : 0u;
+            all += s;
+        }
+        uint32_t* u = units + a * unit_stride;
+        uint32_t run = off + x[a] - own[a];
+        for (uint32_t i = b; i < e && i < L; i += 16) {
+            uint4 v[4] = {fmm_load4(u, i, L), fmm_load4(u, i + 4, L), fmm_load4(u, i + 8, L),
+                          fmm_load4(u, i + 12, L)};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint4 c = v[k];
+                uint4 o;
+                o.x = run;
+                o.y = o.x + c.x;
+                o.z = o.y + c.y;
+                o.w = o.z + c.z;
+                run = o.w + c.w;
+                const uint32_t j = i + 4 * k;
+                if (j + 3 < L) {
+                    *reinterpret_cast<uint4*>(u + j) = o;
+                } else {
+                    if (j < L) u[j] = o.x;
+                    if (j + 1 < L) u[j + 1] = o.y;
+                    if (j + 2 < L) u[j + 2] = o.z;
+                }
+            }
+        }
+        if (threadIdx.x == 0) u[L] = all;
+    }
+}
+
+// 1 / d for d >= 1: hardware reciprocal estimate (relative error below 1e-7) refined by one
+// Newton-Raphson step r <- r + r (1 - d r), which squares the error (below 1e-14)
 __device__ __forceinline__ double fmm_inv(double d) {
-    double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
-    return __builtin_fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(d);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
 }
 
 // Point-to-node expansions into leaf X (the reference's point-node kernel applied to X as the
 // second leaf of a pair): for every leaf partner Y of lower slot, each point q of Y farther than
-// l_X/theta from X's centre adds its C1 = -R D1 and C2, C3 (x M0_X) to X. One wave per leaf,
-// lanes over the partner's points, the 14 coefficients reduced over the wave.
+// l_X/theta from X's centre adds its C1 = -R D1 and C2, C3 (x M0_X) to X. Work units (X, e of
+// E_X) take the points e, e + E_X, ... of those partners over the lanes; each writes its 14
+// coefficients, reduced over the wave, to epart (14 per unit), summed into X by k_fmm_exp_add.
 __global__ __launch_bounds__(kBlock) void k_fmm_exp(const double2* __restrict__ pd,
                                                     const uint32_t* __restrict__ first,
                                                     const uint32_t* __restrict__ count, int depth,
@@ -504,52 +631,82 @@ __global__ __launch_bounds__(kBlock) void k_fmm_exp(const double2* __restrict__ 
                                                     double theta, double* __restrict__ mom,
                                                     const uint32_t* __restrict__ lists,
                                                     const uint32_t* __restrict__ nlist,
-                                                    double* __restrict__ C) {
+                                                    const uint32_t* __restrict__ unit_off,
+                                                    double* __restrict__ epart) {
     const int lane = threadIdx.x & 63;
-    const uint32_t ix = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (ix >= (1u << (2 * depth))) return;
-    const uint32_t X = fmm_level_index(depth) + ix;
-    if (count[X] == 0u) return;
+    const uint32_t L = 1u << (2 * depth);
+    const uint32_t total = unit_off[L];
+    const uint32_t waves = gridDim.x * kWavesPerBlock;
     FmmMoments M = fmm_moments(mom, S);
     double side = __longlong_as_double((long long)prm->side64);
     if (!(side > 0.0)) side = 1.0;
     const double thr = ldexp(side, -depth) / theta;
-    const double cxX = M.cx[X], cyX = M.cy[X], m0X = M.m0[X];
-    FmmCoef cX = {};
-    bool any = false;
-    fmm_leaf_partners(count, M, side, theta, depth, X, ix, lists, nlist, [&](uint32_t Y) {
-        if (Y >= X) return;
-        any = true;
-        const uint32_t fY = first[Y], nY = count[Y];
-        for (uint32_t q = fY + lane; q < fY + nY; q += kWave) {
-            const double2 qq = pd[q];
-            const double rx = qq.x - cxX, ry = qq.y - cyX;
-            if (rx * rx + ry * ry > thr * thr) {
-                double rS, D1, D2, D3;
-                fmm_derivs(rx, ry, rS, D1, D2, D3);
-                cX.c1[0] += -rx * D1;
-                cX.c1[1] += -ry * D1;
-                fmm_add_c2c3(-rx, -ry, D1, D2, D3, m0X, cX);
-            }
+    const uint32_t base = fmm_level_index(depth);
+    for (uint32_t u = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+         u < total; u += waves) {
+        uint32_t lo = 0, hi = L;  // unit_off[lo] <= u < unit_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (unit_off[mid] <= u) lo = mid;
+            else hi = mid;
         }
-    });
-    if (!any) return;
+        const uint32_t ix = lo, X = base + ix;
+        const uint32_t E = unit_off[ix + 1] - unit_off[ix], e = u - unit_off[ix];
+        const double cxX = M.cx[X], cyX = M.cy[X], m0X = M.m0[X];
+        FmmCoef cX = {};
+        const uint32_t start = e + E * (uint32_t)lane, step = E * (uint32_t)kWave;
+        fmm_partners_meta(first, count, M, side, theta, depth, X, ix, lists, nlist,
+                          [&](uint32_t Y, uint32_t fY, uint32_t end, double, double) {
+            if (Y >= X) return;
+            for (uint32_t q = fY + start; q < end; q += step) {
+                const double2 qq = pd[q];
+                const double rx = qq.x - cxX, ry = qq.y - cyX;
+                if (rx * rx + ry * ry > thr * thr) {
+                    double rS, D1, D2, D3;
+                    fmm_derivs(rx, ry, rS, D1, D2, D3);
+                    cX.c1[0] += -rx * D1;
+                    cX.c1[1] += -ry * D1;
+                    fmm_add_c2c3(-rx, -ry, D1, D2, D3, m0X, cX);
+                }
+            }
+        });
+        double v[14];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) cX.c1[k] = wave_sum_d(cX.c1[k]);
+        for (int k = 0; k < 2; ++k) v[k] = wave_sum_d(cX.c1[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) cX.c2[k] = wave_sum_d(cX.c2[k]);
+        for (int k = 0; k < 4; ++k) v[2 + k] = wave_sum_d(cX.c2[k]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) cX.c3[k] = wave_sum_d(cX.c3[k]);
-    if (lane == 0) {
-        FmmCoef c = fmm_load(C, S, X);
+        for (int k = 0; k < 8; ++k) v[6 + k] = wave_sum_d(cX.c3[k]);
+        if (lane < 14) {
+            double mine = v[0];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) c.c1[k] += cX.c1[k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) c.c2[k] += cX.c2[k];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c.c3[k] += cX.c3[k];
-        fmm_store(C, S, X, c);
+            for (int k = 1; k < 14; ++k) mine = lane == k ? v[k] : mine;
+            epart[(size_t)u * 14 + lane] = mine;
+        }
     }
+}
+
+// adds each leaf's expansion-unit partials (in unit order) to its coefficients; thread per leaf
+__global__ __launch_bounds__(kBlock) void k_fmm_exp_add(const uint32_t* __restrict__ unit_off,
+                                                        int depth, uint32_t S,
+                                                        const double* __restrict__ epart,
+                                                        double* __restrict__ C) {
+    const uint32_t ix = blockIdx.x * kBlock + threadIdx.x;
+    if (ix >= (1u << (2 * depth))) return;
+    const uint32_t b = unit_off[ix], e = unit_off[ix + 1];
+    if (b == e) return;
+    const uint32_t X = fmm_level_index(depth) + ix;
+    FmmCoef c = fmm_load(C, S, X);
+    for (uint32_t u = b; u < e; ++u) {
+        const double* p = epart + (size_t)u * 14;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) c.c1[k] += p[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c.c2[k] += p[2 + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c.c3[k] += p[6 + k];
+    }
+    fmm_store(C, S, X, c);
 }
 
 // point-point term of source q on the lane's sink (p.x, p.y)
@@ -565,29 +722,218 @@ __device__ __forceinline__ void fmm_pp1(double2 q, double px, double py, double&
 
 // point-point sums of one chunk against the share q, q + F, ... < end of a partner's points
 // (four sources per step, their loads issued together)
-template <bool SELF>
 __device__ __forceinline__ void fmm_pp_share(const double2* __restrict__ pd, uint32_t q,
-                                             uint32_t end, uint32_t F, uint32_t p, double px,
-                                             double py, double& ax, double& ay, double& t) {
+                                             uint32_t end, uint32_t F, double px, double py,
+                                             double& ax, double& ay, double& t) {
     for (; q + 3 * F < end; q += 4 * F) {
         const double2 a = pd[q], b = pd[q + F], c = pd[q + 2 * F], d = pd[q + 3 * F];
-        fmm_pp1(a, px, py, ax, ay, t, !SELF || q != p);
-        fmm_pp1(b, px, py, ax, ay, t, !SELF || q + F != p);
-        fmm_pp1(c, px, py, ax, ay, t, !SELF || q + 2 * F != p);
-        fmm_pp1(d, px, py, ax, ay, t, !SELF || q + 3 * F != p);
+        fmm_pp1(a, px, py, ax, ay, t, true);
+        fmm_pp1(b, px, py, ax, ay, t, true);
+        fmm_pp1(c, px, py, ax, ay, t, true);
+        fmm_pp1(d, px, py, ax, ay, t, true);
     }
-    for (; q < end; q += F) fmm_pp1(pd[q], px, py, ax, ay, t, !SELF || q != p);
+    for (; q < end; q += F) fmm_pp1(pd[q], px, py, ax, ay, t, true);
 }
 
-// The leaf phase of the pair recursion for one unit: its 64 sinks of leaf X against every leaf
-// partner Y (X itself included). The reference orients a leaf pair (A, B) with A the lower
-// slot; only A's points are tested against B's box (l_B/theta):
+// One unit of the leaf phase: chunk c0 of leaf X, share f of F. GROUPED (leaves of at most 32
+// points): G groups of Wc lanes hold the same sinks and take G different source shares
+// (share f G + g of F G), summed over the groups at the end.
+template <bool GROUPED>
+__device__ __forceinline__ void fmm_pp_unit(const double2* __restrict__ pd,
+                                            const uint32_t* __restrict__ first,
+                                            const uint32_t* __restrict__ count, int depth,
+                                            FmmMoments M, double side, double theta, double thr,
+                                            const uint32_t* __restrict__ lists,
+                                            const uint32_t* __restrict__ nlist, uint32_t X,
+                                            uint32_t ix, uint32_t F, uint32_t c0, uint32_t f,
+                                            uint64_t part_stride, double* __restrict__ part,
+                                            double& t_out, bool& mine_out) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t fX = first[X], nX = count[X];
+    const uint32_t Wc = GROUPED ? (nX > 16u ? 32u : (nX > 8u ? 16u : 8u)) : (uint32_t)kWave;
+    const uint32_t G = GROUPED ? (uint32_t)kWave / Wc : 1u;
+    const uint32_t g = GROUPED ? (uint32_t)lane / Wc : 0u;
+    const uint32_t sl = GROUPED ? ((uint32_t)lane & (Wc - 1)) : (uint32_t)lane;
+    const uint32_t FG = F * G, share = f * G + g;
+    const bool mine = c0 + sl < nX;
+    const uint32_t p = fX + c0 + (mine ? sl : 0);
+    const double2 pp = pd[p];
+    const double px = pp.x, py = pp.y;
+    const double cxX = M.cx[X], cyX = M.cy[X];
+    double ax = 0.0, ay = 0.0, t = 0.0;
+    fmm_partners_meta(first, count, M, side, theta, depth, X, ix, lists, nlist,
+                      [&](uint32_t Y, uint32_t fY, uint32_t end, double cxY, double cyY) {
+        if (Y == X) {
+            fmm_pp_share(pd, fY + share, end, FG, px, py, ax, ay, t);
+            // the sink itself is in its own share once (w = 1, no force): take it out
+            if ((p - fX) % FG == share) t -= 1.0;
+        } else if (X < Y) {
+            const double rx = px - cxY, ry = py - cyY;
+            if (rx * rx + ry * ry > thr * thr) {
+                if (share == 0) {
+                    double rS, D1, D2, D3;
+                    fmm_derivs(rx, ry, rS, D1, D2, D3);
+                    double vx, vy;
+                    fmm_c1(rx, ry, D1, D2, D3, M.m0[Y], M.xx[Y], M.xy[Y], M.yy[Y], vx, vy);
+                    ax += vx;
+                    ay += vy;
+                    t += 2.0 * M.m0[Y] / rS;
+                }
+            } else {
+                fmm_pp_share(pd, fY + share, end, FG, px, py, ax, ay, t);
+            }
+        } else {
+            for (uint32_t q = fY + share; q < end; q += FG) {
+                const double2 qq = pd[q];
+                const double bx = qq.x - cxX, by = qq.y - cyX;
+                if (bx * bx + by * by > thr * thr) continue;
+                fmm_pp1(qq, px, py, ax, ay, t, true);
+            }
+        }
+    });
+    if (GROUPED) {
+        for (uint32_t o = Wc; o < (uint32_t)kWave; o <<= 1) {
+            ax += __shfl_xor(ax, (int)o, kWave);
+            ay += __shfl_xor(ay, (int)o, kWave);
+        }
+    }
+    if (mine && g == 0) {
+        part[(size_t)(2 * f) * part_stride + p] = ax;
+        part[(size_t)(2 * f + 1) * part_stride + p] = ay;
+    }
+    t_out = t;
+    mine_out = mine;
+}
+
+// The same unit with its sources staged in LDS first (units of at most 64 partners whose
+// shares hold at most kFmmStage points): lane k copies the share of partner k into its segment
+// of the wave's stage buffer, dropping the points of a lower-slot partner that X's box accepts,
+// so every load of the unit is in flight at once; the sums then read the sources from LDS.
+// Returns false (nothing done) when the unit does not qualify.
+template <bool GROUPED>
+__device__ __forceinline__ bool fmm_pp_unit_staged(const double2* __restrict__ pd,
+                                                   const uint32_t* __restrict__ first,
+                                                   const uint32_t* __restrict__ count,
+                                                   FmmMoments M, double thr,
+                                                   const uint32_t* __restrict__ lists,
+                                                   const uint32_t* __restrict__ nlist, uint32_t X,
+                                                   uint32_t F, uint32_t c0, uint32_t f,
+                                                   double2* __restrict__ stage,
+                                                   uint64_t part_stride, double* __restrict__ part,
+                                                   double& t_out, bool& mine_out) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t np = nlist[X];
+    if (np == kFmmOverflow || np > (uint32_t)kWave) return false;
+    const uint32_t fX = first[X], nX = count[X];
+    const double cxX = M.cx[X], cyX = M.cy[X];
+    // partner k in lane k
+    const bool has = (uint32_t)lane < np;
+    const uint32_t Y = has ? lists[(size_t)X * kFmmListCap + lane] : X;
+    const uint32_t fY = first[Y], nY = has ? count[Y] : 0u;
+    const uint32_t m = nY > f ? (nY - 1 - f) / F + 1 : 0u;  // points of share f
+    uint32_t off = m;  // exclusive prefix over the lanes
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(off, d, kWave);
+        if (lane >= d) off += y;
+    }
+    const uint32_t all = (uint32_t)__builtin_amdgcn_readlane((int)off, kWave - 1);
+    if (all > kFmmStage) return false;
+    off -= m;
+    const bool second = Y < X;
+    uint32_t kept = 0;
+    for (uint32_t i = 0; i < m; i += 4) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = pd[fY + f + F * (i + u < m ? i + u : i)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i + u >= m) break;
+            if (second) {
+                const double bx = v[u].x - cxX, by = v[u].y - cyX;
+                if (bx * bx + by * by > thr * thr) continue;
+            }
+            stage[off + kept] = v[u];
+            ++kept;
+        }
+    }
+    const double cxl = M.cx[Y], cyl = M.cy[Y];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const uint32_t Wc = GROUPED ? (nX > 16u ? 32u : (nX > 8u ? 16u : 8u)) : (uint32_t)kWave;
+    const uint32_t G = GROUPED ? (uint32_t)kWave / Wc : 1u;
+    const uint32_t g = GROUPED ? (uint32_t)lane / Wc : 0u;
+    const uint32_t sl = GROUPED ? ((uint32_t)lane & (Wc - 1)) : (uint32_t)lane;
+    const bool mine = c0 + sl < nX;
+    const uint32_t o = c0 + (mine ? sl : 0);  // sink offset in the leaf
+    const uint32_t p = fX + o;
+    const double2 pp = pd[p];
+    const double px = pp.x, py = pp.y;
+    double ax = 0.0, ay = 0.0, t = 0.0;
+    for (uint32_t k = 0; k < np; ++k) {
+        const uint32_t Yk = (uint32_t)__builtin_amdgcn_readlane((int)Y, (int)k);
+        const uint32_t sb = (uint32_t)__builtin_amdgcn_readlane((int)off, (int)k);
+        const uint32_t se = sb + (uint32_t)__builtin_amdgcn_readlane((int)kept, (int)k);
+        bool run = true;
+        if (X < Yk) {
+            const double rx = px - fmm_readlane_d(cxl, k), ry = py - fmm_readlane_d(cyl, k);
+            if (rx * rx + ry * ry > thr * thr) {
+                run = false;
+                if (f == 0 && g == 0) {
+                    double rS, D1, D2, D3;
+                    fmm_derivs(rx, ry, rS, D1, D2, D3);
+                    double vx, vy;
+                    fmm_c1(rx, ry, D1, D2, D3, M.m0[Yk], M.xx[Yk], M.xy[Yk], M.yy[Yk], vx, vy);
+                    ax += vx;
+                    ay += vy;
+                    t += 2.0 * M.m0[Yk] / rS;
+                }
+            }
+        }
+        if (!run) continue;
+        uint32_t j = sb + g;
+        for (; j + 3 * G < se; j += 4 * G) {
+            const double2 a = stage[j], b = stage[j + G], c = stage[j + 2 * G], d = stage[j + 3 * G];
+            fmm_pp1(a, px, py, ax, ay, t, true);
+            fmm_pp1(b, px, py, ax, ay, t, true);
+            fmm_pp1(c, px, py, ax, ay, t, true);
+            fmm_pp1(d, px, py, ax, ay, t, true);
+        }
+        for (; j < se; j += G) fmm_pp1(stage[j], px, py, ax, ay, t, true);
+        // the leaf's own share holds the sink itself once (w = 1, no force): take it out
+        if (Yk == X && o % F == f && (o / F) % G == g) t -= 1.0;
+    }
+    if (GROUPED) {
+        for (uint32_t w = Wc; w < (uint32_t)kWave; w <<= 1) {
+            ax += __shfl_xor(ax, (int)w, kWave);
+            ay += __shfl_xor(ay, (int)w, kWave);
+        }
+    }
+    if (mine && g == 0) {
+        part[(size_t)(2 * f) * part_stride + p] = ax;
+        part[(size_t)(2 * f + 1) * part_stride + p] = ay;
+    }
+    t_out = t;
+    mine_out = mine;
+    // the next unit's staging must not overwrite sources still being read
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return true;
+}
+
+// The leaf phase of the pair recursion, one unit per wave at a time: its sinks of leaf X against
+// every leaf partner Y (X itself included). The reference orients a leaf pair (A, B) with A
+// the lower slot; only A's points are tested against B's box (l_B/theta):
 //  - Y == X: point-point over the leaf's other points;
 //  - X < Y (X first): a sink far enough from Y's centre takes Y's expansion (C1 of the
-//    point-node kernel, unit f = 0 only), the others sum over Y's points;
+//    point-node kernel, share 0 only), the others sum over Y's points;
 //  - Y < X (X second): Y's points not accepted against X's box act point by point on the sinks
 //    (the accepted ones went into X's expansion in k_fmm_exp).
-// Unit f takes partner points f, f + F, ...; the F shares are summed per point afterwards.
+// Unit f takes partner points f, f + F, ... (split further over lane groups for small leaves);
+// the F shares are summed per point afterwards.
+template <bool LIGHT>
 __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ pd,
                                                    const uint32_t* __restrict__ first,
                                                    const uint32_t* __restrict__ count, int depth,
@@ -609,6 +955,7 @@ __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ p
     if (!(side > 0.0)) side = 1.0;
     const double thr = ldexp(side, -depth) / theta;
     const uint32_t base = fmm_level_index(depth);
+    __shared__ double2 stage_all[LIGHT ? kWavesPerBlock : 1][LIGHT ? kFmmStage : 1];
     for (uint32_t u = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
          u < total; u += waves) {
         uint32_t lo = 0, hi = L;  // unit_off[lo] <= u < unit_off[hi]
@@ -620,44 +967,24 @@ __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ p
         const uint32_t ix = lo, X = base + ix;
         const uint32_t F = split[ix], local = u - unit_off[ix];
         const uint32_t c0 = (local / F) * kWave, f = local % F;
-        const uint32_t fX = first[X], nX = count[X];
-        const bool mine = c0 + lane < nX;
-        const uint32_t p = fX + c0 + (mine ? lane : 0);
-        const double2 pp = pd[p];
-        const double px = pp.x, py = pp.y;
-        const double cxX = M.cx[X], cyX = M.cy[X];
-        double ax = 0.0, ay = 0.0, t = 0.0;
-        fmm_leaf_partners(count, M, side, theta, depth, X, ix, lists, nlist, [&](uint32_t Y) {
-            const uint32_t fY = first[Y], end = fY + count[Y];
-            if (Y == X) {
-                fmm_pp_share<true>(pd, fY + f, end, F, p, px, py, ax, ay, t);
-            } else if (X < Y) {
-                const double rx = px - M.cx[Y], ry = py - M.cy[Y];
-                if (rx * rx + ry * ry > thr * thr) {
-                    if (f == 0) {
-                        double rS, D1, D2, D3;
-                        fmm_derivs(rx, ry, rS, D1, D2, D3);
-                        double vx, vy;
-                        fmm_c1(rx, ry, D1, D2, D3, M.m0[Y], M.xx[Y], M.xy[Y], M.yy[Y], vx, vy);
-                        ax += vx;
-                        ay += vy;
-                        t += 2.0 * M.m0[Y] / rS;
-                    }
-                } else {
-                    fmm_pp_share<false>(pd, fY + f, end, F, p, px, py, ax, ay, t);
-                }
-            } else {
-                for (uint32_t q = fY + f; q < end; q += F) {
-                    const double2 qq = pd[q];
-                    const double bx = qq.x - cxX, by = qq.y - cyX;
-                    if (bx * bx + by * by > thr * thr) continue;
-                    fmm_pp1(qq, px, py, ax, ay, t, true);
-                }
-            }
-        });
-        if (mine) {
-            part[(size_t)(2 * f) * part_stride + p] = ax;
-            part[(size_t)(2 * f + 1) * part_stride + p] = ay;
+        double t;
+        bool mine;
+        const bool grouped = count[X] <= 32u;
+        bool done = false;
+        if (LIGHT) {
+            double2* stage = stage_all[LIGHT ? (threadIdx.x >> 6) : 0];
+            done = grouped ? fmm_pp_unit_staged<true>(pd, first, count, M, thr, lists, nlist, X, F,
+                                                      c0, f, stage, part_stride, part, t, mine)
+                           : fmm_pp_unit_staged<false>(pd, first, count, M, thr, lists, nlist, X, F,
+                                                       c0, f, stage, part_stride, part, t, mine);
+        }
+        if (!done) {
+            if (grouped)
+                fmm_pp_unit<true>(pd, first, count, depth, M, side, theta, thr, lists, nlist, X, ix,
+                                  F, c0, f, part_stride, part, t, mine);
+            else
+                fmm_pp_unit<false>(pd, first, count, depth, M, side, theta, thr, lists, nlist, X, ix,
+                                   F, c0, f, part_stride, part, t, mine);
         }
         t = wave_sum_d(mine ? t : 0.0);
         if (lane == 0) tot_unit[u] = t;
@@ -740,17 +1067,21 @@ __global__ __launch_bounds__(kBlock) void k_fmm_l2p(const float4* __restrict__ p
     ay[p] = (float)(dy + vy);
 }
 
-// Z: sum of the node-node totals (S slots) and the leaf-unit totals (unit_off[L] units) in a
-// fixed order (block partials, then one block)
+// Z: sum of the node-node totals (S slots) and the light and heavy leaf-unit totals in a fixed
+// order (block partials, then one block)
 __global__ __launch_bounds__(kBlock) void k_fmm_sum_partial(const double* __restrict__ a,
                                                             uint32_t na,
                                                             const double* __restrict__ b,
                                                             const uint32_t* __restrict__ nb_ptr,
+                                                            const double* __restrict__ c,
+                                                            const uint32_t* __restrict__ nc_ptr,
                                                             double* __restrict__ partials) {
-    const uint32_t nb = *nb_ptr;
+    const uint32_t nb = *nb_ptr, nc = *nc_ptr;
+    const uint32_t step = gridDim.x * kBlock;
     double s = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < na; i += gridDim.x * kBlock) s += a[i];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += gridDim.x * kBlock) s += b[i];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < na; i += step) s += a[i];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += step) s += b[i];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nc; i += step) s += c[i];
     s = wave_sum_d(s);
     __shared__ double red[kWavesPerBlock];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -778,11 +1109,14 @@ namespace {
 constexpr uint32_t kFmmSumBlocks = 64;
 
 // workspace layout (each part 256-byte aligned): moments 6 x S, coefficients 14 x S, node
-// totals S, unit totals max_units, point-point shares 2 x kFmmMaxSplit x n_max, FP64 positions
-// n_max, Z partials, unit offsets S + 1, splits S, list lengths S, lists kFmmListCap x S (uint32)
+// totals S, unit totals 2 x max_units (light, heavy), point-point shares 2 x kFmmMaxSplit x n_max,
+// FP64 positions n_max, Z partials, unit offsets 3 x (S + 1 rounded up to 4) (light, heavy,
+// expansion), splits S, list lengths S, lists kFmmListCap x S (uint32). The expansion partials (14 per expansion
+// unit, at most 14 (leaves + n/4) doubles) live in the point-point shares' space, which is
+// written only after they are summed.
 struct FmmLayout {
     uint64_t mom, coef, tot_nn, tot_unit, part, pd, partials, units, split, nlist, lists, bytes,
-        max_units;
+        max_units, unit_stride;
 };
 
 FmmLayout fmm_layout(uint64_t S, uint64_t n_max) {
@@ -794,11 +1128,12 @@ FmmLayout fmm_layout(uint64_t S, uint64_t n_max) {
     L.mom = o;       o += up(6 * S * sizeof(double));
     L.coef = o;      o += up(14 * S * sizeof(double));
     L.tot_nn = o;    o += up(S * sizeof(double));
-    L.tot_unit = o;  o += up(L.max_units * sizeof(double));
+    L.tot_unit = o;  o += up(2 * L.max_units * sizeof(double));
     L.part = o;      o += up(2ull * kFmmMaxSplit * n_max * sizeof(double));
     L.pd = o;        o += up(n_max * sizeof(double2));
     L.partials = o;  o += up(kFmmSumBlocks * sizeof(double));
-    L.units = o;     o += up((S + 1) * sizeof(uint32_t));
+    L.unit_stride = (S + 1 + 3) & ~(uint64_t)3;
+    L.units = o;     o += up(3 * L.unit_stride * sizeof(uint32_t));
     L.split = o;     o += up(S * sizeof(uint32_t));
     L.nlist = o;     o += up(S * sizeof(uint32_t));
     L.lists = o;     o += up((uint64_t)kFmmListCap * S * sizeof(uint32_t));
@@ -827,6 +1162,7 @@ void launch_fmm(hipStream_t s, const float4* pos4, const uint32_t* codes, const 
     uint32_t* units = (uint32_t*)(w + Lw.units);
     uint32_t* split = (uint32_t*)(w + Lw.split);
     uint32_t* nlist = (uint32_t*)(w + Lw.nlist);
+    const uint32_t US = (uint32_t)Lw.unit_stride;
     uint32_t* lists = (uint32_t*)(w + Lw.lists);
     const uint32_t leaves = 1u << (2 * depth);
     const uint32_t leaf_waves = fmm_div_up(leaves, kWavesPerBlock);
@@ -834,22 +1170,33 @@ void launch_fmm(hipStream_t s, const float4* pos4, const uint32_t* codes, const 
     k_fmm_leaf_moments<<<leaf_waves, kBlock, 0, s>>>(pos4, first, count, depth, S, mom, pd);
     const int top = depth - 1 < kFmmTopLevel ? depth - 1 : kFmmTopLevel;
     for (int l = depth - 1; l > top; --l)
-        k_fmm_up<<<fmm_div_up(1u << (2 * l), kBlock), kBlock, 0, s>>>(count, l, S, mom);
-    if (top >= 0) k_fmm_up_top<<<1, kFmmTopNodes, 0, s>>>(count, top, S, mom);
+        k_fmm_up<<<fmm_div_up(1u << (2 * l), kBlock), kBlock, 0, s>>>(l, S, mom);
+    if (top >= 0) k_fmm_up_top<<<1, kFmmTopNodes, 0, s>>>(top, S, mom);
     for (int l = 0; l <= depth; ++l)
         k_fmm_level<<<fmm_div_up(1u << (2 * l), kWavesPerBlock), kBlock, 0, s>>>(
-            count, l, depth, S, prm, theta, mom, coef, tot_nn, lists, nlist, units, split);
-    k_fmm_scan<<<1, kFmmScanThreads, 0, s>>>(units, leaves);
-    k_fmm_exp<<<leaf_waves, kBlock, 0, s>>>(pd, first, count, depth, S, prm, theta, mom, lists,
-                                            nlist, coef);
+            count, l, depth, S, prm, theta, mom, coef, tot_nn, lists, nlist, units, US, split);
+    k_fmm_scan<<<1, kFmmScanThreads, 0, s>>>(units, US, leaves);
+    {
+        const uint64_t ebound = (uint64_t)(leaves < n ? leaves : n) + n / 4;
+        uint32_t eblocks = fmm_div_up(ebound, kWavesPerBlock);
+        if (eblocks > kFmmUnitBlocks) eblocks = kFmmUnitBlocks;
+        if (eblocks < 1) eblocks = 1;
+        k_fmm_exp<<<eblocks, kBlock, 0, s>>>(pd, first, count, depth, S, prm, theta, mom, lists,
+                                             nlist, units + 2 * US, part);
+        k_fmm_exp_add<<<fmm_div_up(leaves, kBlock), kBlock, 0, s>>>(units + 2 * US, depth, S,
+                                                                    part, coef);
+    }
     // the unit count stays on the device: a fixed grid of waves strides over the units
     const uint64_t unit_bound = (uint64_t)kFmmMaxSplit * ((n + kWave - 1) / kWave +
                                                           (leaves < n ? leaves : n));
     uint32_t blocks = fmm_div_up(unit_bound, kWavesPerBlock);
     if (blocks > kFmmUnitBlocks) blocks = kFmmUnitBlocks;
     if (blocks < 1) blocks = 1;
-    k_fmm_pp<<<blocks, kBlock, 0, s>>>(pd, first, count, depth, S, prm, theta, mom, lists, nlist,
-                                       units, split, n_max, part, tot_unit);
+    k_fmm_pp<true><<<blocks, kBlock, 0, s>>>(pd, first, count, depth, S, prm, theta, mom, lists,
+                                             nlist, units, split, n_max, part, tot_unit);
+    k_fmm_pp<false><<<blocks, kBlock, 0, s>>>(pd, first, count, depth, S, prm, theta, mom, lists,
+                                              nlist, units + US, split, n_max, part,
+                                              tot_unit + Lw.max_units);
     const int dtop = depth < kFmmTopLevel ? depth : kFmmTopLevel;
     k_fmm_down_top<<<1, kFmmTopNodes, 0, s>>>(count, dtop, S, mom, coef);
     for (int l = dtop + 1; l <= depth; ++l)
@@ -858,7 +1205,8 @@ void launch_fmm(hipStream_t s, const float4* pos4, const uint32_t* codes, const 
                                                        n_max, part, ax, ay);
     const uint32_t used = (uint32_t)fmm_slots(depth);
     k_fmm_sum_partial<<<kFmmSumBlocks, kBlock, 0, s>>>(tot_nn, used, tot_unit, units + leaves,
-                                                       partials);
+                                                       tot_unit + Lw.max_units,
+                                                       units + US + leaves, partials);
     k_fmm_sum_final<<<1, 64, 0, s>>>(partials, kFmmSumBlocks, total);
 }
 
