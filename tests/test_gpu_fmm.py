@@ -60,6 +60,35 @@ def test_fmm_matches_oracle(gpu, n, theta, kind):
     assert abs(z - bz) <= 1e-10 * abs(bz)
 
 
+def test_fmm_large_theta_matches_oracle(gpu):
+    """theta = 2: most pairs are accepted high in the tree, leaf lists are short."""
+    x, y = _points(20000, 76, "clusters")
+    ax, ay, z = _gpu_fmm(x, y, 2.0)
+    bx, by, bz = _oracle_fmm(x, y, 2.0)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= 1e-6 and np.quantile(r, 0.99) <= 1e-5, (np.median(r), r.max())
+    assert abs(z - bz) <= 1e-10 * abs(bz)
+
+
+def test_fmm_is_deterministic(gpu):
+    """Fixed summation order (no atomics): repeated evaluations and a fresh context agree bit
+    for bit."""
+    x, y = _points(30000, 77, "clusters")
+    ctx = BHContext(x.size, kernel="tsne", tree_mode="morton_fixed")
+    ctx.set_solver("fmm_sym")
+    ctx.set_particles(x, y)
+    ctx.build()
+    a1 = ctx.forces(0.5)
+    a2 = ctx.forces(0.5)
+    ctx.build()
+    a3 = ctx.forces(0.5)
+    ctx.close()
+    b = _gpu_fmm(x, y, 0.5)
+    for other in (a2, a3, b):
+        assert np.array_equal(a1[0], other[0]) and np.array_equal(a1[1], other[1])
+        assert a1[2] == other[2]
+
+
 def test_fmm_small_theta_is_the_exact_sum(gpu):
     x, y = _points(5000, 71)
     ax, ay, z = _gpu_fmm(x, y, 1e-6)
