@@ -276,8 +276,9 @@ def tsne_run(args):
 def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
     """The same t-SNE run with the symmetric dual-tree FMM repulsion (fixed-depth Morton tree,
     theta 0.5), then both repulsion solvers on the Barnes-Hut run's final embedding: walk time
-    (tree built beforehand) and median relative error against the FP64 direct sum over a seeded
-    sample of sinks."""
+    (tree built beforehand), median relative error against the FP64 direct sum over a seeded
+    sample of sinks, and the harness' NMAE over every point (each side scaled by 4/Z of its own
+    solver, tsneTests.h:143-165)."""
     from masterthesislucas_amd import BHContext
     from masterthesislucas_amd.workloads import sample_indices
 
@@ -302,13 +303,14 @@ def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
     x = np.ascontiguousarray(y_end[:, 0])
     y = np.ascontiguousarray(y_end[:, 1])
     sample = sample_indices(n, args.error_sample)
-    walk, err, ref = {}, {}, None
+    walk, err, nmae, ref = {}, {}, {}, None
     for name, mode in (("bh", "quadtree"), ("fmm_sym", "morton_fixed")):
         ctx = BHContext(n, kernel="tsne", tree_mode=mode)
         if name != "bh":
             ctx.set_solver(name)
         ctx.set_particles(x, y)
         ctx.build()
+        nmae[name] = T.repulsion_nmae(ctx, 0.5)[0]
         ax, ay, _ = ctx.forces(0.5)
         if ref is None:
             ref = ctx.direct_fp64(sample)
@@ -326,14 +328,15 @@ def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
         ctx.close()
     log(f"[bench] t-SNE FMM: {1e3 * wall / args.tsne_iters:.3f} ms/iter, walk "
         f"{walk['fmm_sym']:.3f} ms vs BH {walk['bh']:.3f} ms, error {err['fmm_sym']:.2e} vs "
-        f"{err['bh']:.2e}")
+        f"{err['bh']:.2e}, NMAE {nmae['fmm_sym']:.2e} vs {nmae['bh']:.2e}")
     return {"workload": "same P and start, symmetric dual-tree FMM repulsion, theta 0.5",
             "ms_per_iteration": 1e3 * wall / args.tsne_iters,
             "repulsion_ms_per_iteration": rep_ms,
             "phase_ms_per_iteration": {k: round(v[0] / args.tsne_iters, 5) for k, v in ph.items()},
             "kl_final": kl1,
             "final_embedding_walk_ms": {k: round(v, 4) for k, v in walk.items()},
-            "final_embedding_median_rel_error": err}
+            "final_embedding_median_rel_error": err,
+            "final_embedding_nmae": nmae}
 
 
 def main():

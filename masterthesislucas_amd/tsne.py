@@ -238,3 +238,36 @@ class TSNE:
 
     def close(self):
         self.ctx.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# accuracy of the repulsion
+# ---------------------------------------------------------------------------------------------
+def nmae(d_naive, d_approx):
+    """Mean over the points of |d_naive - d_approx| / |d_naive| (L2 norms per point), the error
+    measure of the reference's harness (TsneTest::getNMAE, tsneTests.h:1366-1386). Arrays of
+    shape [n, 2]; FP64 accumulation."""
+    a = np.asarray(d_naive, np.float64).reshape(-1, 2)
+    b = np.asarray(d_approx, np.float64).reshape(-1, 2)
+    if a.shape != b.shape or a.shape[0] == 0:
+        raise ValueError("two non-empty [n, 2] arrays of the same shape")
+    return float(np.mean(np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1]) /
+                         np.hypot(a[:, 0], a[:, 1])))
+
+
+def repulsion_nmae(ctx, theta, sinks=None):
+    """NMAE of the context's repulsion against the exact sum, as the harness measures it
+    (tsneTests.h:143-165): each derivative is the repulsion scaled by 4/Z of its own solver
+    (TSNE::updateRepulsive, tsne.h:287-301). The approximate side is the context's solver (the
+    Barnes-Hut walk or the symmetric FMM) on its current tree; the exact side is the FP64 direct
+    sum on the GPU over every point (Z_naive = the sum of all pairs), its derivative compared at
+    `sinks` (all points by default). Returns (nmae, Z_approx, Z_naive)."""
+    n = ctx.n
+    ax, ay, z_approx = ctx.forces(theta)
+    every = np.arange(n, dtype=np.uint32)
+    dx, dy, tot = ctx.direct_fp64(every)
+    z_naive = float(np.sum(tot))
+    sel = every if sinks is None else np.asarray(sinks, np.int64)
+    naive = np.stack([dx[sel], dy[sel]], 1) * (4.0 / z_naive)
+    approx = np.stack([ax[sel], ay[sel]], 1).astype(np.float64) * (4.0 / z_approx)
+    return nmae(naive, approx), z_approx, z_naive

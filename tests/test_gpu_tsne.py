@@ -186,3 +186,43 @@ def test_state_dump_resumes_the_run(gpu, tmp_path):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("solver", ["bh", "fmm_sym"])
+def test_repulsion_nmae_matches_oracle(gpu, solver):
+    """The harness' error measure (tsneTests.h:143-165, getNMAE :1366-1386) on the GPU: both
+    derivatives scaled by 4/Z of their own solver, compared over every point. Against the same
+    measure from the oracle's solver (same tree and decisions) and its FP64 direct sum."""
+    from masterthesislucas_amd import BHContext
+    from tests.helpers import oracle_tree
+
+    rng = np.random.default_rng(81)
+    k = rng.integers(0, 5, 6000)
+    p = rng.normal(size=(6000, 2)) * 0.4 + rng.normal(size=(5, 2))[k] * 6.0
+    x, y = p[:, 0].astype(np.float32), p[:, 1].astype(np.float32)
+    mode = "quadtree" if solver == "bh" else "morton_fixed"
+    ctx = BHContext(x.size, kernel="tsne", tree_mode=mode)
+    if solver != "bh":
+        ctx.set_solver(solver)
+    ctx.set_particles(x, y)
+    ctx.build()
+    e_gpu, z_gpu, z_naive = tsne.repulsion_nmae(ctx, 0.5)
+    e_zero = tsne.repulsion_nmae(ctx, 1e-6)[0]
+    ctx.close()
+    ex, ey, et = orc.direct(x.astype(np.float64), y.astype(np.float64), None, orc.KERNEL_TSNE)
+    assert z_naive == pytest.approx(et.sum(), rel=1e-11)
+    t = oracle_tree(x, y, None, mode=mode)
+    if solver == "bh":
+        qt = orc.QuadTree(x.astype(np.float64), y.astype(np.float64), 16)
+        bx, by, bz = qt.forces(orc.KERNEL_TSNE, 1.0, 1.0, 0.0, 0.5, nthreads=8)[:3]
+    else:
+        minx, miny, maxx, maxy = t["bounds"]
+        tree = orc.fixed_tree(t["xs"], t["ys"], t["depth"], minx, miny, maxx, maxy)
+        sx, sy, bz = orc.fmm_sym(t["xs"], t["ys"], tree, t["depth"], 0.5)
+        bx, by = np.empty(x.size), np.empty(x.size)
+        bx[t["perm"]] = sx
+        by[t["perm"]] = sy
+    e_orc = tsne.nmae(np.stack([ex, ey], 1) * (4.0 / et.sum()), np.stack([bx, by], 1) * (4.0 / bz))
+    assert abs(e_gpu - e_orc) <= 1e-3 * e_orc + 1e-6, (e_gpu, e_orc)
+    assert z_gpu == pytest.approx(bz, rel=1e-5)
+    assert e_zero <= 1e-5  # theta -> 0: every pair direct, the exact sum up to FP32 output

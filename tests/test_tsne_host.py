@@ -99,3 +99,15 @@ def test_theta_schedule_matches_oracle():
     for t in (1, 100, 333, 499):
         assert orc.tsne_theta(0.3, 1.2, t) + orc.tsne_theta(0.3, 1.2, 999 - t) == \
             pytest.approx(1.5, abs=1e-14)
+
+
+def test_nmae_closed_form():
+    """TsneTest::getNMAE: mean over the points of |naive - approx| / |naive|."""
+    naive = np.array([[1.0, 0.0], [0.0, 2.0], [3.0, 4.0]])
+    approx = np.array([[1.0, 0.1], [0.0, 1.0], [3.0, 4.0]])
+    assert tsne.nmae(naive, approx) == pytest.approx((0.1 / 1.0 + 1.0 / 2.0 + 0.0) / 3.0, abs=1e-15)
+    assert tsne.nmae(naive, naive) == 0.0
+    # scale invariance of each term, and the mean (not the max) over the points
+    assert tsne.nmae(7.0 * naive, 7.0 * approx) == pytest.approx(tsne.nmae(naive, approx), abs=1e-15)
+    with pytest.raises(ValueError):
+        tsne.nmae(naive, approx[:2])
