@@ -50,7 +50,16 @@ typedef enum bh_tree_mode {
     /* NBodySolverFMM_SYM_MORTON (nBodySolverFMM_SYM_MORTON.h:79-151, 272-313, 328-561):
      * 32-bit Morton codes, fixed depth getDepth(max_children, N), point-node traversal
      * accepting d^2 > (l/theta)^2, particle-particle at the leaves. */
-    BH_TREE_MORTON_FIXED = 1
+    BH_TREE_MORTON_FIXED = 1,
+    /* The tree of the reference's GPU step (NBodySolverGpuBH, nBodySolverGpuBH.cu:78-240):
+     * complete to a fixed depth (config fixed_depth, 6 when 0, as tsneGpu.h:145 builds it),
+     * root centred on the bounding box with side the larger extent (FP32), every cell split at
+     * its centre with a coordinate equal to the centre going low, particle-particle at the
+     * leaves. The acceptance is d^2 > (l/theta)^2 (the reference evaluates l/d < 1 with theta
+     * fixed at 1: the two differ only when d and l agree to the last bit), and a leaf excludes
+     * the sink itself by index (the reference compares positions, which also drops points
+     * coincident with the sink). */
+    BH_TREE_GPU_REF = 2
 } bh_tree_mode;
 
 typedef struct bh_config {
@@ -145,10 +154,16 @@ typedef struct bh_tsne_params {
     int32_t exag_switch;     /* 250 */
     int32_t schedule_iters;  /* 999: theta(t) = min + (max-min)(1/2 + cos(pi min(t/999, 1))/2) */
     int32_t pad;
+    double repulsion_scale;  /* tsne.h:291: derivative = (repulsion_scale / Z) rep + attraction;
+                                4 (0 is read as 4) */
 } bh_tsne_params;
 
 /* reference constants for n points, constant theta (theta_min = theta_max = theta) */
 bh_status bh_tsne_params_default(bh_tsne_params* p, uint32_t n, double theta);
+/* the constants of the reference's GPU step (NBodySolverGpuBH::timeStep, tsneGpu.h:145,
+ * nBodySolverGpuBH.cu:424-460): learning rate 1000, momentum 0.2, no exaggeration, theta 1,
+ * derivative = (1/Z) rep + attraction. Use with a BH_TREE_GPU_REF context. */
+bh_status bh_tsne_params_gpu_reference(bh_tsne_params* p);
 /* theta used at iteration t: theta_max at t = 0, the cosine schedule at t - 1 afterwards */
 double bh_tsne_theta(const bh_tsne_params* p, int32_t iteration);
 /* P matrix in CSR, caller order, rows sorted by (row, col) as setupEfficientPmatrix builds it

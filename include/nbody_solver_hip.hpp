@@ -52,6 +52,7 @@ public:
         cfg.softening = softening;
         cfg.theta = initTheta;
         code_bits_ = mode == BH_TREE_QUADTREE ? cfg.code_bits : 16;
+        mode_ = mode;
         check(bh_create(&ctx_, &cfg), "bh_create");
     }
 
@@ -106,6 +107,12 @@ public:
         check(bh_get_morton(ctx_, codes.data(), nullptr), "bh_get_morton");
         double b[4], side = 0.0;
         check(bh_get_bounds(ctx_, b, &side), "bh_get_bounds");
+        if (mode_ == BH_TREE_GPU_REF) {  // root centred on the bounding box (FP32 midpoint)
+            b[0] = static_cast<double>((static_cast<float>(b[0]) + static_cast<float>(b[2])) * 0.5f) -
+                   0.5 * side;
+            b[1] = static_cast<double>((static_cast<float>(b[1]) + static_cast<float>(b[3])) * 0.5f) -
+                   0.5 * side;
+        }
         static const std::array<glm::vec3, 7> colors = {
             glm::vec3(1, 1, 1), glm::vec3(1, 0, 0), glm::vec3(0, 1, 0), glm::vec3(0, 0, 1),
             glm::vec3(1, 1, 0), glm::vec3(0, 1, 1), glm::vec3(1, 0, 1)};
@@ -142,6 +149,7 @@ private:
 
     bh_ctx* ctx_ = nullptr;
     int code_bits_ = 32;
+    bh_tree_mode mode_ = BH_TREE_QUADTREE;
     size_t n_ = 0;
     std::vector<float> x_, y_, m_, ax_, ay_;
 };

@@ -18,6 +18,7 @@ KERNEL_GRAVITY = 0
 KERNEL_TSNE = 1
 TREE_QUADTREE = 0
 TREE_MORTON_FIXED = 1
+TREE_GPU_REF = 2
 NUM_PHASES = 7
 PHASE_NAMES = ("bounds+morton", "sort", "gather", "tree", "upward", "traversal", "integrate")
 
@@ -34,7 +35,8 @@ class TsneParams(C.Structure):
                 ("momentum_late", C.c_double), ("exag_early", C.c_double),
                 ("exag_late", C.c_double), ("theta_min", C.c_double), ("theta_max", C.c_double),
                 ("momentum_switch", C.c_int32), ("exag_switch", C.c_int32),
-                ("schedule_iters", C.c_int32), ("pad", C.c_int32)]
+                ("schedule_iters", C.c_int32), ("pad", C.c_int32),
+                ("repulsion_scale", C.c_double)]
 
 
 class BHError(RuntimeError):
@@ -61,6 +63,7 @@ SIGNATURES = {
     "bh_leapfrog": (_i32, [_vp, C.c_int, _f, _d]),
     "bh_direct_fp64": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),
     "bh_tsne_params_default": (_i32, [C.POINTER(TsneParams), _u32, _d]),
+    "bh_tsne_params_gpu_reference": (_i32, [C.POINTER(TsneParams)]),
     "bh_tsne_theta": (_d, [C.POINTER(TsneParams), _i32]),
     "bh_tsne_set_p": (_i32, [_vp, _u32, _vp, _vp, _vp, _u32]),
     "bh_tsne_set_embedding": (_i32, [_vp, _vp, _vp]),

@@ -210,8 +210,13 @@ bh_status launch_status(bh_ctx* c, const char* what) {
     return hip_check(c, hipGetLastError(), what);
 }
 
+// the reference GPU step's tree depth (NBodySolverGpuBH constructed with 6, tsneGpu.h:145)
+constexpr int kGpuRefDepth = 6;
+
 int effective_depth(const bh_ctx* c) {
-    return c->cfg.fixed_depth > 0 ? c->cfg.fixed_depth : reference_depth(c->cfg.max_children, c->n);
+    if (c->cfg.fixed_depth > 0) return c->cfg.fixed_depth;
+    if (c->cfg.tree_mode == BH_TREE_GPU_REF) return kGpuRefDepth;
+    return reference_depth(c->cfg.max_children, c->n);
 }
 
 void launch_node_moments(bh_ctx* c);
@@ -238,9 +243,17 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
         launch_morton_batched(s, c->x[cur], c->y[cur], n, c->sys_off, c->nsys, c->sys_prm,
                               c->qbits, (uint64_t*)c->keys_a, c->vals_a);
     } else {
-        launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
-                      quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
-        launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a, c->wide);
+        if (c->cfg.tree_mode == BH_TREE_GPU_REF) {
+            // the reference GPU step takes the bounds of the points themselves
+            launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm, kTreeGpuRef, nullptr);
+            launch_morton_gpuref(s, c->x[cur], c->y[cur], n, c->prm, fixed, (uint32_t*)c->keys_a,
+                                 c->vals_a);
+        } else {
+            launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
+                          quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
+            launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a,
+                          c->wide);
+        }
     }
     phase_end(c, 0);
 
@@ -416,7 +429,8 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     *out = nullptr;
     if (cfg->n_max == 0 || cfg->n_max > (1u << 30)) return BH_ERR_ARG;
     if (cfg->kernel != BH_KERNEL_GRAVITY && cfg->kernel != BH_KERNEL_TSNE) return BH_ERR_ARG;
-    if (cfg->tree_mode != BH_TREE_QUADTREE && cfg->tree_mode != BH_TREE_MORTON_FIXED)
+    if (cfg->tree_mode != BH_TREE_QUADTREE && cfg->tree_mode != BH_TREE_MORTON_FIXED &&
+        cfg->tree_mode != BH_TREE_GPU_REF)
         return BH_ERR_ARG;
     if (cfg->max_children < 1 || cfg->max_children > kMaxLeafCap) return BH_ERR_ARG;
     if (cfg->fixed_depth < 0 || cfg->fixed_depth > 16) return BH_ERR_ARG;
@@ -453,7 +467,9 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
         }
         c->cap_nodes = (uint32_t)capn;
     } else {
-        const int dcap = std::max(cfg->fixed_depth, reference_depth(cfg->max_children, N));
+        const int dcap = cfg->tree_mode == BH_TREE_GPU_REF
+                             ? (cfg->fixed_depth > 0 ? cfg->fixed_depth : kGpuRefDepth)
+                             : std::max(cfg->fixed_depth, reference_depth(cfg->max_children, N));
         c->max_fixed_depth = std::max(dcap, 1);
         const uint64_t slots = ((1ull << (2 * (c->max_fixed_depth + 1))) - 1) / 3;
         const uint64_t bound = (uint64_t)(c->max_fixed_depth + 1) * N + 1;
@@ -905,6 +921,24 @@ bh_status bh_tsne_params_default(bh_tsne_params* p, uint32_t n, double theta) {
     p->momentum_switch = 250;
     p->exag_switch = 250;
     p->schedule_iters = 999;
+    p->repulsion_scale = 4.0;
+    return BH_OK;
+}
+
+bh_status bh_tsne_params_gpu_reference(bh_tsne_params* p) {
+    if (!p) return BH_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    p->learning_rate = 1000.0;  // tsneGpu.h:145
+    p->momentum_early = 0.2;
+    p->momentum_late = 0.2;
+    p->exag_early = 1.0;        // P as loaded, no exaggeration (nBodySolverGpuBH.cu:430-441)
+    p->exag_late = 1.0;
+    p->theta_min = 1.0;         // the acceptance l / d < 1 (nBodySolverGpuBH.cu:283)
+    p->theta_max = 1.0;
+    p->momentum_switch = 0;
+    p->exag_switch = 0;
+    p->schedule_iters = 999;
+    p->repulsion_scale = 1.0;   // derivative *= -1/Z (nBodySolverGpuBH.cu:424)
     return BH_OK;
 }
 
@@ -1000,7 +1034,8 @@ static bh_status tsne_iteration(bh_ctx* c, const bh_tsne_params* p, int32_t t, b
     const float mu = (float)(t < p->momentum_switch ? p->momentum_early : p->momentum_late);
     // ty_xy holds the current embedding: tsne_stage_build staged it for this iteration
     launch_tsne_update(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[cur], c->ty_y[cur], c->ty_xy,
-                       c->ty_x[prev], c->ty_y[prev], c->rep_x, c->rep_y, c->dsum, exag,
+                       c->ty_x[prev], c->ty_y[prev], c->rep_x, c->rep_y, c->dsum,
+                       p->repulsion_scale > 0.0 ? p->repulsion_scale : 4.0, exag,
                        (float)p->learning_rate, mu, c->ty_x[next], c->ty_y[next],
                        keep_deriv ? c->tder_x : nullptr, keep_deriv ? c->tder_y : nullptr);
     // rotate: next becomes current, current becomes previous
@@ -1257,7 +1292,7 @@ bh_status bh_set_solver(bh_ctx* c, int solver) {
     if (solver != BH_SOLVER_BH && solver != BH_SOLVER_FMM_SYM)
         return fail(c, BH_ERR_ARG, "unknown solver %d", solver);
     if (solver == BH_SOLVER_FMM_SYM) {
-        if (c->wide || c->nsys || c->cfg.kernel != BH_KERNEL_TSNE)
+        if (c->cfg.tree_mode != BH_TREE_MORTON_FIXED || c->nsys || c->cfg.kernel != BH_KERNEL_TSNE)
             return fail(c, BH_ERR_ARG,
                         "the FMM solver runs the t-SNE kernels on the fixed-depth Morton tree: "
                         "one system, BH_KERNEL_TSNE, BH_TREE_MORTON_FIXED");

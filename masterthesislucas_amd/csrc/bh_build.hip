@@ -118,8 +118,13 @@ __global__ __launch_bounds__(kBlock) void k_bounds_seg_final(const float4* __res
 
 __device__ void fill_params(BuildParams* prm, float4 r, uint64_t mnx64, uint64_t mny64,
                             uint64_t mxx64, uint64_t mxy64, uint32_t tree_mode, uint32_t n) {
-    uint64_t side = (tree_mode == kTreeQuadTree) ? bh_sfp_quadtree_side(mnx64, mny64, mxx64, mxy64)
-                                                 : bh_sfp_morton_extent(mnx64, mny64, mxx64, mxy64);
+    uint64_t side;
+    if (tree_mode == kTreeQuadTree)
+        side = bh_sfp_quadtree_side(mnx64, mny64, mxx64, mxy64);
+    else if (tree_mode == kTreeGpuRef)  // createRoot: the larger FP32 extent (.cu:78-94)
+        side = bh_sfp_from_f32(fmaxf(r.z - r.x, r.w - r.y));
+    else
+        side = bh_sfp_morton_extent(mnx64, mny64, mxx64, mxy64);
     if ((side & ~BH_SFP_SIGN) == 0) side = 0x3FF0000000000000ull;  // all points coincide: side 1
     prm->minx = r.x;
     prm->miny = r.y;
@@ -159,6 +164,39 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ xs,
     else
         code = (K)((bh_spread32(qy) << 1) | bh_spread32(qx));
     keys[i] = code;
+    vals[i] = i;
+}
+
+// The tree of the reference's GPU step (NBodySolverGpuBH: createRoot, createChildren,
+// nBodySolverGpuBH.cu:78-200): root centred on the bounding box (FP32 midpoint) with side the
+// larger FP32 extent; every cell splits at its centre, a point equal to the centre coordinate
+// going low; child centres c +- 0.25 l and side 0.5 l in FP32 as there. Each point descends
+// `depth` levels by those comparisons; the code holds its path two bits per level, x on the
+// even bit as in the Morton codes (the reference's child order A, B, C, D is (x, y) =
+// (low, low), (low, high), (high, low), (high, high); only the storage order differs), in the
+// top 2 depth bits of the 32-bit key.
+__global__ __launch_bounds__(kBlock) void k_morton_gpuref(const float* __restrict__ xs,
+                                                          const float* __restrict__ ys,
+                                                          uint32_t n,
+                                                          const BuildParams* __restrict__ prm,
+                                                          int depth, uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float mnx = prm->minx, mny = prm->miny, mxx = prm->maxx, mxy = prm->maxy;
+    float len = fmaxf(mxx - mnx, mxy - mny);
+    float cx = (mnx + mxx) * 0.5f, cy = (mny + mxy) * 0.5f;
+    const float x = xs[i], y = ys[i];
+    uint32_t code = 0;
+    for (int l = 0; l < depth; ++l) {
+        const bool xh = !(x <= cx), yh = !(y <= cy);
+        code = (code << 2) | ((uint32_t)yh << 1) | (uint32_t)xh;
+        const float q = 0.25f * len;
+        cx = cx + (xh ? q : -q);
+        cy = cy + (yh ? q : -q);
+        len = 0.5f * len;
+    }
+    keys[i] = depth > 0 ? code << (32 - 2 * depth) : 0u;
     vals[i] = i;
 }
 
@@ -1155,6 +1193,11 @@ void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, 
     k_bounds_final<<<1, kBlock, 0, s>>>(partials, g, n, prm, tree_mode, caller ? 1 : 0,
                                         caller ? caller[0] : 0.0, caller ? caller[1] : 0.0,
                                         caller ? caller[2] : 0.0, caller ? caller[3] : 0.0);
+}
+
+void launch_morton_gpuref(hipStream_t s, const float* xs, const float* ys, uint32_t n,
+                          const BuildParams* prm, int depth, uint32_t* keys, uint32_t* vals) {
+    k_morton_gpuref<<<div_up(n, kBlock), kBlock, 0, s>>>(xs, ys, n, prm, depth, keys, vals);
 }
 
 void launch_morton(hipStream_t s, const float* xs, const float* ys, uint32_t n,

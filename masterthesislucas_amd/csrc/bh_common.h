@@ -33,6 +33,7 @@ constexpr int kTopMax = 2560;                  // deferred nodes the top pass ke
 enum : uint32_t {
     kTreeQuadTree = 0,     // adaptive, leaf capacity, QuadTree root and near-field rules
     kTreeMortonFixed = 1,  // fixed depth, reference 32-bit Morton codes
+    kTreeGpuRef = 2,       // fixed depth, centred root, split at the cell centre (<= goes low)
 };
 
 // Per-build parameters, produced on the device by the bounds pass and the tree build.

@@ -18,6 +18,9 @@ constexpr int kKernelTsne = 1;
 uint32_t bounds_grid(uint32_t n);
 void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, float4* partials,
                    BuildParams* prm, uint32_t tree_mode, const double* caller);
+// keys of the reference GPU step's tree (32-bit, path in the top 2 depth bits)
+void launch_morton_gpuref(hipStream_t s, const float* xs, const float* ys, uint32_t n,
+                          const BuildParams* prm, int depth, uint32_t* keys, uint32_t* vals);
 void launch_morton(hipStream_t s, const float* xs, const float* ys, uint32_t n,
                    const BuildParams* prm, int B, void* keys, uint32_t* vals, bool wide);
 uint32_t scan_partials_size(uint64_t n);
@@ -134,8 +137,9 @@ void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t
 void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
                         const float* val, uint32_t n, const float* yx, const float* yy,
                         const float2* xy, const float* pyx, const float* pyy,
-                        const float* rep_x, const float* rep_y, const double* z, float exag,
-                        float lr, float mu, float* nyx, float* nyy, float* dx, float* dy);
+                        const float* rep_x, const float* rep_y, const double* z,
+                        double rep_scale, float exag, float lr, float mu, float* nyx, float* nyy,
+                        float* dx, float* dy);
 uint32_t tsne_kl_blocks(uint32_t n);
 void launch_tsne_kl(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col, const float* val,
                     uint32_t n, const float* yx, const float* yy, const double* z,

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(kBlock) void k_tsne_stage(const float* __restrict__
 }
 
 // One wave per row i: attraction exag * sum_j P_ij w_ij (y_j - y_i) with w = 1/(1+|y_j-y_i|^2)
-// (updateAttractive, tsne.h:356-381), derivative d_i = (4/Z) rep_i + attraction
-// (updateRepulsive scaling, tsne.h:291-294), then y' = y + lr d + mu (y - y_prev)
+// (updateAttractive, tsne.h:356-381), derivative d_i = (s/Z) rep_i + attraction with s = 4
+// (updateRepulsive scaling, tsne.h:291-294; 1 in the reference's GPU step,
+// nBodySolverGpuBH.cu:424), then y' = y + lr d + mu (y - y_prev)
 // (updatePoints, tsne.h:179-206). The neighbours' positions are gathered from the interleaved
 // copy xy (one 8-byte load per non-zero instead of two 4-byte ones: the gathers, not the bytes,
 // bound this kernel).
@@ -44,8 +45,8 @@ __global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restri
                                                         const float* __restrict__ pyy,
                                                         const float* __restrict__ rep_x,
                                                         const float* __restrict__ rep_y,
-                                                        const double* __restrict__ z, float exag,
-                                                        float lr, float mu,
+                                                        const double* __restrict__ z, double rep_scale,
+                                                        float exag, float lr, float mu,
                                                         float* __restrict__ nyx,
                                                         float* __restrict__ nyy,
                                                         float* __restrict__ dx_out,
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restri
     sx = wave_sum_f(sx);
     sy = wave_sum_f(sy);
     if (lane != 0) return;
-    const float scale = (float)(4.0 / *z);
+    const float scale = (float)(rep_scale / *z);
     const float dx = fmaf(scale, rep_x[r], sx);
     const float dy = fmaf(scale, rep_y[r], sy);
     if (dx_out) {
@@ -139,11 +140,12 @@ void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t
 void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
                         const float* val, uint32_t n, const float* yx, const float* yy,
                         const float2* xy, const float* pyx, const float* pyy,
-                        const float* rep_x, const float* rep_y, const double* z, float exag,
-                        float lr, float mu, float* nyx, float* nyy, float* dx, float* dy) {
+                        const float* rep_x, const float* rep_y, const double* z,
+                        double rep_scale, float exag, float lr, float mu, float* nyx, float* nyy,
+                        float* dx, float* dy) {
     k_tsne_update<<<div_up(n, kWavesPerBlock), kBlock, 0, s>>>(row_ptr, col, val, n, yx, yy, xy,
-                                                               pyx, pyy, rep_x, rep_y, z, exag, lr,
-                                                               mu, nyx, nyy, dx, dy);
+                                                               pyx, pyy, rep_x, rep_y, z, rep_scale,
+                                                               exag, lr, mu, nyx, nyy, dx, dy);
 }
 
 uint32_t tsne_kl_blocks(uint32_t n) {

@@ -46,7 +46,8 @@ class BHContext:
         cfg.n_max = int(n_max)
         cfg.kernel = {"gravity": _lib.KERNEL_GRAVITY, "tsne": _lib.KERNEL_TSNE}[kernel]
         cfg.tree_mode = {"quadtree": _lib.TREE_QUADTREE,
-                         "morton_fixed": _lib.TREE_MORTON_FIXED}[tree_mode]
+                         "morton_fixed": _lib.TREE_MORTON_FIXED,
+                         "gpu_ref": _lib.TREE_GPU_REF}[tree_mode]
         cfg.max_children = int(max_children)
         cfg.fixed_depth = int(fixed_depth)
         cfg.code_bits = int(code_bits)
@@ -318,7 +319,7 @@ class NBodySolverHipBH:
         t = self._ctx.tree()
         codes, _ = self._ctx.morton()
         b, side = self._ctx.bounds()
-        B = 16 if self._kw["tree_mode"] == "morton_fixed" else int(self._kw["code_bits"])
+        B = int(self._kw["code_bits"]) if self._kw["tree_mode"] == "quadtree" else 16
         sel = np.ones(t["level"].size, bool) if level == -1 else (t["level"] == level)
         lv = t["level"][sel].astype(np.int64)
         c = codes[t["first"][sel]]
@@ -328,8 +329,12 @@ class NBodySolverHipBH:
             ix |= ((c >> np.uint64(2 * bit)) & np.uint64(1)).astype(np.int64) << bit
             iy |= ((c >> np.uint64(2 * bit + 1)) & np.uint64(1)).astype(np.int64) << bit
         cell = side / np.exp2(lv.astype(np.float64))
-        lx = b[0] + (ix >> (B - lv)) * cell
-        ly = b[1] + (iy >> (B - lv)) * cell
+        x0, y0 = b[0], b[1]
+        if self._kw["tree_mode"] == "gpu_ref":  # root centred on the bounding box
+            x0 = float(np.float32(np.float32(b[0]) + np.float32(b[2])) * np.float32(0.5)) - 0.5 * side
+            y0 = float(np.float32(np.float32(b[1]) + np.float32(b[3])) * np.float32(0.5)) - 0.5 * side
+        lx = x0 + (ix >> (B - lv)) * cell
+        ly = y0 + (iy >> (B - lv)) * cell
         hx, hy = lx + cell, ly + cell
         v = np.stack([lx, ly, hx, ly, lx, ly, lx, hy, lx, hy, hx, hy, hx, ly, hx, hy], 1)
         col = colors[min(level + 1, 6)]

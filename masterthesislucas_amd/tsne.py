@@ -144,7 +144,7 @@ class TSNE:
 
     def __init__(self, row_ptr, col, val, y0, min_theta=0.5, max_theta=0.5,
                  exaggeration=(16.0, 4.0), momentum=(0.5, 0.8), switch=250, max_children=16,
-                 learn_rate=None, solver="bh"):
+                 learn_rate=None, solver="bh", _gpu_reference_depth=None):
         row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
         col = np.ascontiguousarray(col, np.uint32)
         val = np.ascontiguousarray(val, np.float32)
@@ -157,17 +157,23 @@ class TSNE:
         # "bh": Barnes-Hut over the adaptive QuadTree (NBodySolverBH); "fmm_sym": the symmetric
         # dual-tree FMM over the fixed-depth Morton tree (NBodySolverFMM_SYM_MORTON)
         mode = "morton_fixed" if solver == "fmm_sym" else "quadtree"
+        depth = 0
+        if _gpu_reference_depth is not None:
+            mode, depth = "gpu_ref", int(_gpu_reference_depth)
         self.ctx = BHContext(n, kernel="tsne", max_children=max_children, theta=max_theta,
-                             tree_mode=mode)
+                             tree_mode=mode, fixed_depth=depth)
         if solver != "bh":
             self.ctx.set_solver(solver)
         L = _lib.load()
         p = _lib.TsneParams()
-        self.ctx._c(L.bh_tsne_params_default(C.byref(p), n, float(max_theta)))
-        p.theta_min, p.theta_max = float(min_theta), float(max_theta)
-        p.exag_early, p.exag_late = map(float, exaggeration)
-        p.momentum_early, p.momentum_late = map(float, momentum)
-        p.momentum_switch = p.exag_switch = int(switch)
+        if _gpu_reference_depth is not None:
+            self.ctx._c(L.bh_tsne_params_gpu_reference(C.byref(p)))
+        else:
+            self.ctx._c(L.bh_tsne_params_default(C.byref(p), n, float(max_theta)))
+            p.theta_min, p.theta_max = float(min_theta), float(max_theta)
+            p.exag_early, p.exag_late = map(float, exaggeration)
+            p.momentum_early, p.momentum_late = map(float, momentum)
+            p.momentum_switch = p.exag_switch = int(switch)
         if learn_rate is not None:
             p.learning_rate = float(learn_rate)
         self.params = p
@@ -238,6 +244,19 @@ class TSNE:
 
     def close(self):
         self.ctx.close()
+
+
+class TsneGpu(TSNE):
+    """The reference's GPU t-SNE (TsneGpu, nBodyInstances/tsneGpu.h, with NBodySolverGpuBH): the
+    complete fixed-depth tree with the centred root and centre splits (depth 6 by default,
+    tsneGpu.h:145), theta 1, learning rate 1000, momentum 0.2, no exaggeration, derivative
+    (1/Z) rep + attraction (nBodySolverGpuBH.cu:643-758). Same methods as `TSNE`; the
+    per-point work runs in this library's kernels (include/bh.h: BH_TREE_GPU_REF,
+    bh_tsne_params_gpu_reference)."""
+
+    def __init__(self, row_ptr, col, val, y0, depth=6, learn_rate=None):
+        super().__init__(row_ptr, col, val, y0, learn_rate=learn_rate,
+                         _gpu_reference_depth=depth)
 
 
 # ---------------------------------------------------------------------------------------------

@@ -56,6 +56,85 @@ void orc_morton32(const double* x, const double* y, int64_t n, double minx, doub
     }
 }
 
+/* createRoot (nBodySolverGpuBH.cu:78-94): centre = FP32 midpoint of the bounding box, side =
+ * the larger FP32 extent. createChildren (:96-200): a cell's points are split by x against the
+ * centre (<= goes low), each half by y; the four children, in the order (x, y) = (low, low),
+ * (low, high), (high, low), (high, high), get centres c + (+-0.25 l, +-0.25 l) and side 0.5 l in
+ * FP32. Restated as the reference does it, cell by cell over index lists, level by level. */
+void orc_gpuref_codes32(const float* x, const float* y, int64_t n, int depth, uint32_t* code) {
+    if (n <= 0) return;
+    float mnx = x[0], mny = y[0], mxx = x[0], mxy = y[0];
+    for (int64_t i = 1; i < n; ++i) {
+        if (x[i] < mnx) mnx = x[i];
+        if (y[i] < mny) mny = y[i];
+        if (x[i] > mxx) mxx = x[i];
+        if (y[i] > mxy) mxy = y[i];
+    }
+    const float lx = mxx - mnx, ly = mxy - mny;
+    /* cells of the current level: point lists (as index ranges of `idx`), centres, sides */
+    int64_t ncell = 1;
+    int64_t* idx = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    int64_t* tmp = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    int64_t* beg = (int64_t*)malloc(sizeof(int64_t) * 2);
+    float* cx = (float*)malloc(sizeof(float));
+    float* cy = (float*)malloc(sizeof(float));
+    float* len = (float*)malloc(sizeof(float));
+    for (int64_t i = 0; i < n; ++i) {
+        idx[i] = i;
+        code[i] = 0;
+    }
+    beg[0] = 0;
+    beg[1] = n;
+    cx[0] = (mnx + mxx) * 0.5f;
+    cy[0] = (mny + mxy) * 0.5f;
+    len[0] = lx > ly ? lx : ly;
+    for (int l = 0; l < depth; ++l) {
+        const int64_t nc = ncell * 4;
+        int64_t* nbeg = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nc + 1));
+        float* ncx = (float*)malloc(sizeof(float) * (size_t)nc);
+        float* ncy = (float*)malloc(sizeof(float) * (size_t)nc);
+        float* nlen = (float*)malloc(sizeof(float) * (size_t)nc);
+        int64_t w = 0;
+        for (int64_t c = 0; c < ncell; ++c) {
+            for (int q = 0; q < 4; ++q) { /* A, B, C, D */
+                const int xh = q >> 1, yh = q & 1;
+                nbeg[4 * c + q] = w;
+                for (int64_t k = beg[c]; k < beg[c + 1]; ++k) {
+                    const int64_t i = idx[k];
+                    const int px = !(x[i] <= cx[c]), py = !(y[i] <= cy[c]);
+                    if (px == xh && py == yh) {
+                        tmp[w++] = i;
+                        code[i] |= (uint32_t)((yh << 1) | xh) << (30 - 2 * l);
+                    }
+                }
+                const float o = 0.25f * len[c];
+                ncx[4 * c + q] = cx[c] + (xh ? o : -o);
+                ncy[4 * c + q] = cy[c] + (yh ? o : -o);
+                nlen[4 * c + q] = 0.5f * len[c];
+            }
+        }
+        nbeg[nc] = w;
+        int64_t* s = idx;
+        idx = tmp;
+        tmp = s;
+        free(beg);
+        free(cx);
+        free(cy);
+        free(len);
+        beg = nbeg;
+        cx = ncx;
+        cy = ncy;
+        len = nlen;
+        ncell = nc;
+    }
+    free(idx);
+    free(tmp);
+    free(beg);
+    free(cx);
+    free(cy);
+    free(len);
+}
+
 void orc_quantize_axis(const double* v, int64_t n, double vmin, double L, int bits, uint64_t* q) {
     const double nm = -vmin;
     const double scale = ldexp(1.0, bits);

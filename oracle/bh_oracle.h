@@ -22,6 +22,8 @@
  *   orc_mlp_*            finite differences; Adam closed form
  *   orc_fmm_sym          theta -> 0 equals the exact t-SNE sum; error vs the exact sum shrinks
  *                        with theta; two far clusters against the exact pair sum
+ *   orc_gpuref_codes32   hand cases (centre and boundary points go low, the maximum goes high)
+ *                        and a per-point descent by the same comparisons in tests/test_oracle.py
  */
 #ifndef BH_ORACLE_H
 #define BH_ORACLE_H
@@ -38,6 +40,12 @@ enum {
     ORC_SEM_QUADTREE = 0, /* nBodySolverBH.h:56-89: l/|d| < theta; children<=1 -> PP over all */
     ORC_SEM_MORTON = 1    /* nBodySolverFMM_SYM_MORTON.h:272-313: d^2 > (l/theta)^2; PP at leaves */
 };
+
+/* ---- the reference GPU step's tree: nBodySolverGpuBH.cu:78-200 ----
+ * The cells' recursive partition (createRoot, createChildren) of the FP32 positions x, y to
+ * `depth` levels; out: per point its path, two bits per level (x on the even bit: child
+ * (x high) | (y high) << 1), in the top 2 depth bits of a 32-bit code. */
+void orc_gpuref_codes32(const float* x, const float* y, int64_t n, int depth, uint32_t* code);
 
 /* ---- Morton codes: nBodySolverFMM_SYM_MORTON.h:538-561 ---- */
 void orc_morton32(const double* x, const double* y, int64_t n, double minx, double miny, double L,

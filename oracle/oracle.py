@@ -44,6 +44,7 @@ def lib():
         I64, I32, D = C.c_int64, C.c_int, C.c_double
         sig = {
             "orc_morton32": (None, [_vp, _vp, I64, D, D, D, _vp]),
+            "orc_gpuref_codes32": (None, [_vp, _vp, I64, I32, _vp]),
             "orc_morton64": (None, [_vp, _vp, I64, D, D, D, I32, _vp]),
             "orc_quantize_axis": (None, [_vp, I64, D, D, I32, _vp]),
             "orc_stable_sort_u64": (None, [_vp, I64, _vp]),
@@ -113,6 +114,15 @@ def morton32(x, y, minx, miny, L):
     x, y = _f64(x), _f64(y)
     out = np.empty(x.size, np.uint32)
     lib().orc_morton32(_p(x), _p(y), x.size, float(minx), float(miny), float(L), _p(out))
+    return out
+
+
+def gpuref_codes32(x32, y32, depth):
+    """Paths of the reference GPU step's partition (nBodySolverGpuBH.cu:78-200) as 32-bit codes."""
+    x = np.ascontiguousarray(x32, np.float32)
+    y = np.ascontiguousarray(y32, np.float32)
+    out = np.empty(x.size, np.uint32)
+    lib().orc_gpuref_codes32(_p(x), _p(y), x.size, int(depth), _p(out))
     return out
 
 

@@ -15,6 +15,12 @@ def oracle_tree(x32, y32, m32=None, mode="quadtree", cap=16, bits=32, depth=None
     if mode == "quadtree":
         side = O.quadtree_side(minx, miny, maxx, maxy)
         codes = O.morton64(x, y, minx, miny, side, bits)
+    elif mode == "gpu_ref":  # the reference GPU step's tree, depth 6 unless given
+        if depth is None:
+            depth = 6
+        x32, y32 = np.asarray(x32, np.float32), np.asarray(y32, np.float32)
+        side = float(max(np.float32(x32.max() - x32.min()), np.float32(y32.max() - y32.min())))
+        codes = O.gpuref_codes32(x32, y32, depth).astype(np.uint64)
     else:
         side = max(maxx + (-minx), maxy + (-miny))
         codes = O.morton32(x, y, minx, miny, side).astype(np.uint64)

@@ -1,0 +1,114 @@
+"""The reference GPU step's tree and iteration (NBodySolverGpuBH + TsneGpu,
+nBodySolverGpuBH.cu:78-460, tsneGpu.h:145) as BH_TREE_GPU_REF and TsneGpu, against the oracle:
+partition paths (orc_gpuref_codes32, pinned in test_oracle.py by a per-point descent), sort
+order and topology bit-exact; forces on the same tree within the FP32 bar of the other modes
+(median 1e-5, p99 1e-4 relative); one TsneGpu iteration against the oracle's repulsion on that
+tree scaled by 1/Z, the attraction without exaggeration and the update with learning rate 1000
+and momentum 0.2."""
+import numpy as np
+import pytest
+
+from masterthesislucas_amd import BHContext, tsne
+from masterthesislucas_amd.workloads import plummer, uniform_disk
+from oracle import oracle as O
+from tests.helpers import oracle_forces, oracle_tree, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _clusters(n, seed):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 6, n)
+    c = rng.normal(size=(6, 2)) * 5.0
+    p = c[k] + rng.normal(size=(n, 2)) * 0.3
+    p[: n // 50] = p[0]  # coincident points
+    return p[:, 0].astype(np.float32), p[:, 1].astype(np.float32)
+
+
+@pytest.mark.parametrize("depth", [0, 6, 9])
+def test_gpuref_codes_order_topology_bitexact(gpu, depth):
+    x, y = _clusters(30000, 90 + depth)
+    ctx = BHContext(x.size, kernel="tsne", tree_mode="gpu_ref", fixed_depth=depth)
+    ctx.set_particles(x, y)
+    ctx.build()
+    codes, perm = ctx.morton()
+    d = depth if depth > 0 else 6
+    t = oracle_tree(x, y, None, mode="gpu_ref", depth=d)
+    assert np.array_equal(codes, t["codes_sorted"])
+    assert np.array_equal(perm, t["perm"])
+    _, side = ctx.bounds()
+    assert side == t["side"]
+    g = ctx.tree()
+    r = t["tree"]
+    for k in ("level", "first", "count", "skip"):
+        assert np.array_equal(g[k].astype(np.int64), r[k].astype(np.int64)), k
+    ctx.close()
+
+
+@pytest.mark.parametrize("kernel,theta", [("tsne", 1.0), ("tsne", 0.5), ("gravity", 0.7)])
+def test_gpuref_forces_match_oracle(gpu, kernel, theta):
+    grav = kernel == "gravity"
+    p = plummer(40000, a=0.2, seed=91) if grav else None
+    x, y = (p.x, p.y) if grav else _clusters(40000, 92)
+    m = p.m if grav else None
+    eps = float(np.float32(1e-3)) if grav else 0.0
+    ctx = BHContext(x.size, kernel=kernel, tree_mode="gpu_ref", softening=eps)
+    ctx.set_particles(x, y, m)
+    ctx.build()
+    ax, ay, tot = ctx.forces(theta)
+    ctx.close()
+    t = oracle_tree(x, y, m, mode="gpu_ref")
+    bx, by, bt, _ = oracle_forces(t, O.KERNEL_GRAVITY if grav else O.KERNEL_TSNE, theta,
+                                  mode="gpu_ref", eps=eps)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= 1e-5 and np.quantile(r, 0.99) <= 1e-4, (np.median(r), r.max())
+    assert abs(tot - bt.sum()) <= 1e-5 * abs(bt.sum())
+
+
+def test_tsne_gpu_step_matches_oracle(gpu):
+    n = 4000
+    X, _ = tsne.gaussian_mixture(n, dim=30, seed=93)
+    rp, col, val = tsne.joint_probabilities(X, perplexity=20.0, k=60)
+    yx, yy = tsne.initial_embedding(n, seed=93, std=1.0)
+    run = tsne.TsneGpu(rp, col, val, (yx, yy))
+    assert (run.params.learning_rate, run.params.momentum_early, run.params.exag_early,
+            run.params.theta_max, run.params.repulsion_scale) == (1000.0, 0.2, 1.0, 1.0, 1.0)
+    try:
+        dx, dy, z = run.timeStep(return_derivative=True)
+        y1 = run.embeddedPoints.astype(np.float64)
+    finally:
+        run.close()
+    t = oracle_tree(yx, yy, None, mode="gpu_ref")
+    rx, ry, rt, _ = oracle_forces(t, O.KERNEL_TSNE, 1.0, mode="gpu_ref")
+    oz = rt.sum()
+    assert abs(z - oz) <= 1e-5 * oz
+    x0, y0 = yx.astype(np.float64), yy.astype(np.float64)
+    atx, aty = O.tsne_attraction(x0, y0, rp, col, val, 1.0)
+    odx, ody = rx / oz + atx, ry / oz + aty
+    scale = np.hypot(np.abs(rx / oz) + np.abs(atx), np.abs(ry / oz) + np.abs(aty))
+    e = np.hypot(dx - odx, dy - ody) / scale
+    assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-4
+    # iteration 0: previous = current, so y' = y + 1000 d
+    e = np.hypot(y1[:, 0] - (x0 + 1000.0 * odx), y1[:, 1] - (y0 + 1000.0 * ody))
+    e /= 1000.0 * scale + 1e-6 * np.hypot(x0, y0)
+    assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-4
+
+
+def test_gpuref_rejects_fmm_and_exports_boxes(gpu):
+    p = uniform_disk(5000, seed=94)
+    ctx = BHContext(p.n, kernel="tsne", tree_mode="gpu_ref")
+    from masterthesislucas_amd._lib import BHError
+    with pytest.raises(BHError):
+        ctx.set_solver("fmm_sym")
+    ctx.close()
+    from masterthesislucas_amd import NBodySolverHipBH, Points
+    s = NBodySolverHipBH(kernel="tsne", tree_mode="gpu_ref", n_max=p.n)
+    pts = Points(np.stack([p.x, p.y], 1).astype(np.float64))
+    s.updateTree(pts)
+    pos, _ = s.getNodesBufferData(0)
+    # the root box is centred on the points' bounding box with side the larger extent
+    lo, hi = pos.min(0), pos.max(0)
+    cx, cy = (p.x.min() + p.x.max()) / 2, (p.y.min() + p.y.max()) / 2
+    side = max(p.x.max() - p.x.min(), p.y.max() - p.y.min())
+    assert np.allclose((lo + hi) / 2, (cx, cy), atol=1e-5)
+    assert np.allclose(hi - lo, side, rtol=1e-5)

@@ -513,3 +513,57 @@ def test_fmm_sym_expansion_order_on_two_far_clusters():
     for (f0, z0), (f1, z1) in zip(errs, errs[1:]):
         assert f0 / f1 > 30, errs
         assert z0 / z1 > 8, errs
+
+
+# ---------------------------------------------------------------------------------------------
+# the reference GPU step's partition (nBodySolverGpuBH.cu:78-200)
+def _gpuref_descent(x, y, depth):
+    """Each point walks down by itself: compare with the cell centre (<= goes low), move to the
+    child's centre c +- l/4 and side l/2 in float32 (an independent route to the paths the
+    oracle computes cell by cell over point lists)."""
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    f = np.float32
+    len0 = max(f(x.max() - x.min()), f(y.max() - y.min()))
+    cx = np.full(x.size, f((x.min() + x.max()) * f(0.5)), np.float32)
+    cy = np.full(x.size, f((y.min() + y.max()) * f(0.5)), np.float32)
+    ln = np.full(x.size, len0, np.float32)
+    code = np.zeros(x.size, np.uint64)
+    for l in range(depth):
+        xh = ~(x <= cx)
+        yh = ~(y <= cy)
+        code |= ((yh.astype(np.uint64) << np.uint64(1)) | xh.astype(np.uint64)) << np.uint64(30 - 2 * l)
+        q = (f(0.25) * ln).astype(np.float32)
+        cx = np.where(xh, cx + q, cx - q).astype(np.float32)
+        cy = np.where(yh, cy + q, cy - q).astype(np.float32)
+        ln = (f(0.5) * ln).astype(np.float32)
+    return code.astype(np.uint32)
+
+
+def test_gpuref_partition_hand_cases():
+    # root (0,0)-(1,1): centre (0.5, 0.5); a coordinate equal to a centre goes low
+    x = np.array([0.0, 1.0, 0.5, 0.25, 0.5, 1.0], np.float32)
+    y = np.array([0.0, 1.0, 0.5, 0.75, 1.0, 0.5], np.float32)
+    c = O.gpuref_codes32(x, y, 2)
+    assert c[0] == 0x00000000            # low, low twice
+    assert c[1] == 0xF0000000            # the maximum: high, high twice
+    assert c[2] == 0x30000000            # the centre goes low, then high in (0.25, 0.25)'s cell
+    assert c[3] == 0x80000000            # (x low, y high), then on the child centre: low, low
+    assert c[4] == (0b10 << 30) | (0b11 << 28)  # x on the centre: low; y = max: high, high
+    assert c[5] == (0b01 << 30) | (0b11 << 28)  # y on the centre: low; then (0.75, 0.25): high, high
+    # all points equal: zero extent, everything goes low at every level
+    z = O.gpuref_codes32(np.full(5, 3.0, np.float32), np.full(5, -2.0, np.float32), 6)
+    assert (z == 0).all()
+
+
+@pytest.mark.parametrize("depth", [1, 3, 6, 10])
+def test_gpuref_partition_equals_per_point_descent(depth):
+    rng = np.random.default_rng(40 + depth)
+    n = 4000
+    x = rng.normal(size=n).astype(np.float32)
+    y = (rng.normal(size=n) * 3.0 + 1.0).astype(np.float32)
+    # points on cell centres of several levels and exact duplicates
+    x[:200] = np.float32(x.min() + (x.max() - x.min()) * (np.arange(200) % 8) / 8.0)
+    x[200:220] = x[0]
+    y[200:220] = y[0]
+    assert np.array_equal(O.gpuref_codes32(x, y, depth), _gpuref_descent(x, y, depth))
