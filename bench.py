@@ -270,7 +270,32 @@ def tsne_run(args):
     run.close()
     if not args.no_tsne_fmm:
         out["fmm_sym"] = tsne_fmm_run(args, T, rp, col, val, y0, y_end)
+    out["gpu_reference_step"] = tsne_gpu_reference_run(args, T, rp, col, val, y0)
     return out
+
+
+def tsne_gpu_reference_run(args, T, rp, col, val, y0):
+    """The reference's GPU t-SNE step (TsneGpu + NBodySolverGpuBH: complete depth-6 tree with
+    the centred root, theta 1, learning rate 1000, momentum 0.2, 1/Z, no exaggeration) on the
+    same P and start, args.tsne_iters iterations."""
+    run = T.TsneGpu(rp, col, val, y0)
+    run.run(5)
+    run.close()
+    run = T.TsneGpu(rp, col, val, y0)
+    run.ctx.profile_enable(True)
+    run.ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    run.run(args.tsne_iters)
+    wall = time.perf_counter() - t0
+    ph = run.ctx.profile_read(reset=True)
+    run.ctx.profile_enable(False)
+    kl1, _ = run.costFunction()
+    run.close()
+    log(f"[bench] t-SNE (reference GPU step): {1e3 * wall / args.tsne_iters:.3f} ms/iter")
+    return {"workload": "same P and start, depth-6 centred tree, theta 1, lr 1000, momentum 0.2",
+            "ms_per_iteration": 1e3 * wall / args.tsne_iters,
+            "phase_ms_per_iteration": {k: round(v[0] / args.tsne_iters, 5) for k, v in ph.items()},
+            "kl_final": kl1}
 
 
 def tsne_fmm_run(args, T, rp, col, val, y0, y_end):

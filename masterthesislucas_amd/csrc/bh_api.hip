@@ -297,7 +297,7 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
 void launch_node_moments(bh_ctx* c) {
     phase_begin(c, 4);
     launch_upward(c->stream, c->pos4, c->nodes, c->child4, c->learned ? c->m2 : nullptr, c->prm,
-                  c->deferred, c->cap_nodes);
+                  c->deferred, c->cap_nodes, !c->wide && !c->nsys);
     phase_end(c, 4);
     c->moments = true;
 }

@@ -775,6 +775,39 @@ __device__ __forceinline__ Moments leaf_moments(const float4* __restrict__ pos4,
     return r;
 }
 
+// Fixed-depth trees can hold thousands of points in one leaf (a tight cluster): such leaves
+// (more than kBigLeaf points) get their mass and centre of mass from one wave each before the
+// chunked pass, which then takes them from the node record.
+constexpr uint32_t kBigLeaf = 64;
+
+__global__ __launch_bounds__(kBlock) void k_big_leaf_moments(const float4* __restrict__ pos4,
+                                                             Node* __restrict__ nodes,
+                                                             const BuildParams* __restrict__ prm,
+                                                             uint32_t cap_nodes) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (k >= cap_nodes || k >= prm->n_nodes) return;
+    const uint32_t meta = nodes[k].meta, f = nodes[k].first, c = nodes[k].count;
+    if (!(meta & kMetaLeaf) || c <= kBigLeaf) return;
+    const float4 p0 = pos4[f];
+    float M = 0.f, sx = 0.f, sy = 0.f;
+    for (uint32_t j = f + lane; j < f + c; j += kWave) {
+        const float4 q = pos4[j];
+        M += q.z;
+        sx = fmaf(q.z, q.x - p0.x, sx);
+        sy = fmaf(q.z, q.y - p0.y, sy);
+    }
+    M = wave_sum_f(M);
+    sx = wave_sum_f(sx);
+    sy = wave_sum_f(sy);
+    if (lane == 0) {
+        const float inv = (M != 0.f) ? 1.0f / M : 0.f;
+        nodes[k].cx = p0.x + sx * inv;
+        nodes[k].cy = p0.y + sy * inv;
+        nodes[k].mass = M;
+    }
+}
+
 // combine up to four children (absent ones have m = 0 and are flagged by has[t] = false)
 __device__ __forceinline__ Moments combine(const Moments* ch, const bool* has, bool with_m2) {
     // origin: the first child present (selected with constant indices: no private memory)
@@ -852,7 +885,9 @@ __global__ __launch_bounds__(kUpThreads) void k_up_chunk(const float4* __restric
         const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
         if (k >= c1 || !(nd[r].meta & kMetaLeaf)) continue;
         const uint32_t s = k - c0;
-        const Moments mo = leaf_moments(pos4, nd[r].first, nd[r].count, with_m2);
+        const Moments mo = (!with_m2 && nd[r].count > kBigLeaf)
+                               ? Moments{nd[r].mass, nd[r].cx, nd[r].cy, 0.f, 0.f, 0.f}
+                               : leaf_moments(pos4, nd[r].first, nd[r].count, with_m2);
         s_m[s] = mo.m; s_x[s] = mo.x; s_y[s] = mo.y;
         s_xx[s] = mo.xx; s_xy[s] = mo.xy; s_yy[s] = mo.yy;
     }
@@ -1330,8 +1365,12 @@ void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
 uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }
 
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
-                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes) {
+                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes,
+                   bool big_leaves) {
     const uint32_t nch = up_chunks(cap_nodes);
+    if (big_leaves && !m2)
+        k_big_leaf_moments<<<div_up(cap_nodes, kWavesPerBlock), kBlock, 0, s>>>(pos4, nodes, prm,
+                                                                                cap_nodes);
     k_up_chunk<<<nch, kUpThreads, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
                                       cap_nodes);
     k_up_top<<<1, kUpThreads, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, nch,
