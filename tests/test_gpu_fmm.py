@@ -89,6 +89,27 @@ def test_fmm_is_deterministic(gpu):
         assert a1[2] == other[2]
 
 
+@pytest.mark.parametrize("n,theta,kind", [(3, 0.5, "gauss"), (33, 0.2, "clusters"),
+                                          (500, 0.7, "line"), (4097, 0.2, "gauss"),
+                                          (50000, 0.2, "clusters"), (50000, 0.7, "line")])
+def test_fmm_sweep_matches_oracle(gpu, n, theta, kind):
+    """Sizes around the chunk and unit boundaries; theta = 0.2 overflows the leaf interaction
+    lists (more than 256 partners), so the level, expansion and point-point kernels take their
+    walk-from-the-root path; points on a line stack many leaves' worth of points per column."""
+    if kind == "line":
+        rng = np.random.default_rng(78 + n)
+        t = rng.random(n)
+        x = (t * 10.0).astype(np.float32)
+        y = (0.3 * t + 1e-3 * rng.normal(size=n)).astype(np.float32)
+    else:
+        x, y = _points(n, 79 + n, kind)
+    ax, ay, z = _gpu_fmm(x, y, theta)
+    bx, by, bz = _oracle_fmm(x, y, theta)
+    r = rel_err(ax, ay, bx, by)
+    assert np.median(r) <= 1e-6 and np.quantile(r, 0.99) <= 1e-5, (np.median(r), r.max())
+    assert abs(z - bz) <= 1e-10 * max(abs(bz), 1.0)
+
+
 def test_fmm_small_theta_is_the_exact_sum(gpu):
     x, y = _points(5000, 71)
     ax, ay, z = _gpu_fmm(x, y, 1e-6)
