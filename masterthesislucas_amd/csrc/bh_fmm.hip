@@ -746,8 +746,9 @@ __device__ __forceinline__ void fmm_pp_unit(const double2* __restrict__ pd,
                                             const uint32_t* __restrict__ lists,
                                             const uint32_t* __restrict__ nlist, uint32_t X,
                                             uint32_t ix, uint32_t F, uint32_t c0, uint32_t f,
-                                            uint64_t part_stride, double* __restrict__ part,
-                                            double& t_out, bool& mine_out) {
+                                            double2* __restrict__ stage, uint64_t part_stride,
+                                            double* __restrict__ part, double& t_out,
+                                            bool& mine_out) {
     const int lane = threadIdx.x & 63;
     const uint32_t fX = first[X], nX = count[X];
     const uint32_t Wc = GROUPED ? (nX > 16u ? 32u : (nX > 8u ? 16u : 8u)) : (uint32_t)kWave;
@@ -781,6 +782,37 @@ __device__ __forceinline__ void fmm_pp_unit(const double2* __restrict__ pd,
                 }
             } else {
                 fmm_pp_share(pd, fY + share, end, FG, px, py, ax, ay, t);
+            }
+        } else if (!GROUPED) {
+            // Y's points tested against X's box 64 at a time (one per lane), the ones kept
+            // compacted into LDS and read back by the whole wave
+            for (uint32_t b0 = fY + share; b0 < end; b0 += FG * (uint32_t)kWave) {
+                const uint32_t q = b0 + FG * (uint32_t)lane;
+                bool keep = false;
+                double2 qq = make_double2(0.0, 0.0);
+                if (q < end) {
+                    qq = pd[q];
+                    const double bx = qq.x - cxX, by = qq.y - cyX;
+                    keep = !(bx * bx + by * by > thr * thr);
+                }
+                const uint64_t kb = __ballot(keep);
+                if (keep) stage[__popcll(kb & lane_mask_lt())] = qq;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t nk = (uint32_t)__popcll(kb);
+                uint32_t j = 0;
+                for (; j + 3 < nk; j += 4) {
+                    const double2 a = stage[j], b = stage[j + 1], c = stage[j + 2],
+                                  d = stage[j + 3];
+                    fmm_pp1(a, px, py, ax, ay, t, true);
+                    fmm_pp1(b, px, py, ax, ay, t, true);
+                    fmm_pp1(c, px, py, ax, ay, t, true);
+                    fmm_pp1(d, px, py, ax, ay, t, true);
+                }
+                for (; j < nk; ++j) fmm_pp1(stage[j], px, py, ax, ay, t, true);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
         } else {
             for (uint32_t q = fY + share; q < end; q += FG) {
@@ -955,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ p
     if (!(side > 0.0)) side = 1.0;
     const double thr = ldexp(side, -depth) / theta;
     const uint32_t base = fmm_level_index(depth);
-    __shared__ double2 stage_all[LIGHT ? kWavesPerBlock : 1][LIGHT ? kFmmStage : 1];
+    __shared__ double2 stage_all[kWavesPerBlock][LIGHT ? kFmmStage : kWave];
     for (uint32_t u = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
          u < total; u += waves) {
         uint32_t lo = 0, hi = L;  // unit_off[lo] <= u < unit_off[hi]
@@ -971,8 +1003,8 @@ __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ p
         bool mine;
         const bool grouped = count[X] <= 32u;
         bool done = false;
+        double2* stage = stage_all[threadIdx.x >> 6];
         if (LIGHT) {
-            double2* stage = stage_all[LIGHT ? (threadIdx.x >> 6) : 0];
             done = grouped ? fmm_pp_unit_staged<true>(pd, first, count, M, thr, lists, nlist, X, F,
                                                       c0, f, stage, part_stride, part, t, mine)
                            : fmm_pp_unit_staged<false>(pd, first, count, M, thr, lists, nlist, X, F,
@@ -981,10 +1013,10 @@ __global__ __launch_bounds__(kBlock) void k_fmm_pp(const double2* __restrict__ p
         if (!done) {
             if (grouped)
                 fmm_pp_unit<true>(pd, first, count, depth, M, side, theta, thr, lists, nlist, X, ix,
-                                  F, c0, f, part_stride, part, t, mine);
+                                  F, c0, f, stage, part_stride, part, t, mine);
             else
                 fmm_pp_unit<false>(pd, first, count, depth, M, side, theta, thr, lists, nlist, X, ix,
-                                   F, c0, f, part_stride, part, t, mine);
+                                   F, c0, f, stage, part_stride, part, t, mine);
         }
         t = wave_sum_d(mine ? t : 0.0);
         if (lane == 0) tot_unit[u] = t;
