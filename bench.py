@@ -59,8 +59,14 @@ def cpu_baseline(p, theta, eps, n):
     qt = O.QuadTree(x, y, 16)
     qt.forces(O.KERNEL_GRAVITY, G=1.0, m=1.0 / n, eps2=eps * eps, theta=theta, nthreads=cores)
     secs_all = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n / secs, "unit": "force evals/s", "cores": 1, "kind": kind,
-            "seconds": secs,
+            "seconds": secs, "host_cpu": model, "host_nproc": os.cpu_count(),
             "sample": f"one updateTree + solveNbody step (QuadTree, leaf capacity 16, "
                       f"theta={theta}) over all {n} particles of the benchmark's initial state, "
                       f"FP64, single thread",
