@@ -357,6 +357,10 @@ def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
         e = np.hypot(ax[sample] - dx, ay[sample] - dy) / np.maximum(np.hypot(dx, dy), 1e-300)
         err[name] = float(np.median(e))
         ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = fmm_cpu_baseline(x, y)
+        log(f"[bench] FMM CPU restatement: {cpu['seconds']:.2f} s per evaluation")
     log(f"[bench] t-SNE FMM: {1e3 * wall / args.tsne_iters:.3f} ms/iter, walk "
         f"{walk['fmm_sym']:.3f} ms vs BH {walk['bh']:.3f} ms, error {err['fmm_sym']:.2e} vs "
         f"{err['bh']:.2e}, NMAE {nmae['fmm_sym']:.2e} vs {nmae['bh']:.2e}")
@@ -367,7 +371,31 @@ def tsne_fmm_run(args, T, rp, col, val, y0, y_end):
             "kl_final": kl1,
             "final_embedding_walk_ms": {k: round(v, 4) for k, v in walk.items()},
             "final_embedding_median_rel_error": err,
-            "final_embedding_nmae": nmae}
+            "final_embedding_nmae": nmae,
+            "cpu_baseline": cpu}
+
+
+def fmm_cpu_baseline(x32, y32, theta=0.5):
+    """The oracle's FP64 restatement of NBodySolverFMM_SYM_MORTON (the reference's Morton solvers
+    need boost and cannot be compiled here) timed on this host, one thread: Morton codes, sort,
+    fixed-depth tree and the FMM evaluation on the given points."""
+    from oracle import oracle as O
+
+    x = np.asarray(x32, np.float32).astype(np.float64)
+    y = np.asarray(y32, np.float32).astype(np.float64)
+    t0 = time.perf_counter()
+    minx, miny, maxx, maxy = x.min(), y.min(), x.max(), y.max()
+    side = max(maxx + (-minx), maxy + (-miny))
+    perm = O.stable_sort(O.morton32(x, y, minx, miny, side).astype(np.uint64))
+    xs, ys = x[perm], y[perm]
+    depth = O.get_depth(16, x.size)
+    tree = O.fixed_tree(xs, ys, depth, minx, miny, maxx, maxy)
+    O.fmm_sym(xs, ys, tree, depth, theta)
+    secs = time.perf_counter() - t0
+    return {"value": x.size / secs, "unit": "force evals/s", "cores": 1, "kind": "port",
+            "seconds": secs,
+            "sample": f"one FMM evaluation (theta={theta}) on the {x.size}-point final embedding, "
+                      f"FP64, single thread"}
 
 
 def main():
