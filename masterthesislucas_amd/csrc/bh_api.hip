@@ -97,6 +97,9 @@ struct bh_ctx {
     int ty_cur = 0;  // ty[ty_cur] current, ty[(ty_cur+2)%3] previous
     bool ty_set = false;
     float *rep_x = nullptr, *rep_y = nullptr, *tder_x = nullptr, *tder_y = nullptr;
+    float *att_x = nullptr, *att_y = nullptr;   // attraction sums (caller order)
+    hipStream_t att_stream = nullptr;           // the attraction's stream (default priority)
+    hipEvent_t ev_staged = nullptr, ev_att = nullptr;
     double* tz = nullptr;
     double* kl_part = nullptr;
     double* kl_out = nullptr;
@@ -396,6 +399,12 @@ static void graph_reset(bh_ctx* c);
 void bh_destroy(bh_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->att_stream) {
+        (void)hipStreamSynchronize(c->att_stream);
+        (void)hipStreamDestroy(c->att_stream);
+    }
+    if (c->ev_staged) (void)hipEventDestroy(c->ev_staged);
+    if (c->ev_att) (void)hipEventDestroy(c->ev_att);
     graph_reset(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->dsinks) (void)hipFree(c->dsinks);
@@ -532,6 +541,14 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
         TRY(dalloc(c, &c->rep_y, N));
         TRY(dalloc(c, &c->tder_x, N));
         TRY(dalloc(c, &c->tder_y, N));
+        TRY(dalloc(c, &c->att_x, N));
+        TRY(dalloc(c, &c->att_y, N));
+        TRY(hip_check(c, hipStreamCreateWithFlags(&c->att_stream, hipStreamNonBlocking),
+                      "hipStreamCreate"));
+        TRY(hip_check(c, hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming),
+                      "hipEventCreate"));
+        TRY(hip_check(c, hipEventCreateWithFlags(&c->ev_att, hipEventDisableTiming),
+                      "hipEventCreate"));
         TRY(dalloc(c, &c->tz, 2));
         TRY(dalloc(c, &c->kl_part, kMaxReduceBlocks));
         TRY(dalloc(c, &c->kl_out, 2));
@@ -1011,33 +1028,49 @@ bh_status bh_tsne_get_embedding(bh_ctx* c, float* yx, float* yy) {
     return hip_check(c, hipStreamSynchronize(c->stream), "get embedding");
 }
 
-// stage the current embedding as the solver's particles and build the tree
-static bh_status tsne_stage_build(bh_ctx* c) {
+// stage the current embedding as the solver's particles (and its interleaved copy ty_xy)
+static void tsne_stage(bh_ctx* c) {
     const uint32_t n = c->p_n;
     c->n = n;
     c->cur = 0;
     launch_tsne_stage(c->stream, c->ty_x[c->ty_cur], c->ty_y[c->ty_cur], n, c->x[0], c->y[0],
                       c->m[0], c->id[0], c->ty_xy);
+}
+
+// stage the current embedding and build the tree
+static bh_status tsne_stage_build(bh_ctx* c) {
+    tsne_stage(c);
     return do_build(c, nullptr);
 }
 
+// The attraction needs only the staged embedding: it runs on att_stream from the stage on, while
+// the tree is built and walked on the context's stream; the update waits for both.
 static bh_status tsne_iteration(bh_ctx* c, const bh_tsne_params* p, int32_t t, bool keep_deriv) {
-    bh_status st = tsne_stage_build(c);
+    hipStream_t s = c->stream;
+    const uint32_t n = c->p_n;
+    const float exag = (float)(t < p->exag_switch ? p->exag_early : p->exag_late);
+    const float mu = (float)(t < p->momentum_switch ? p->momentum_early : p->momentum_late);
+    tsne_stage(c);
+    bh_status st = hip_check(c, hipEventRecord(c->ev_staged, s), "event");
+    if (st == BH_OK) st = hip_check(c, hipStreamWaitEvent(c->att_stream, c->ev_staged, 0), "wait");
+    if (st != BH_OK) return st;
+    launch_tsne_attract(c->att_stream, c->p_row, c->p_col, c->p_val, n, c->ty_xy, exag, c->att_x,
+                        c->att_y);
+    st = hip_check(c, hipEventRecord(c->ev_att, c->att_stream), "event");
+    if (st != BH_OK) return st;
+    st = do_build(c, nullptr);
     if (st != BH_OK) return st;
     st = do_forces(c, tsne_theta(p, t), true);  // repulsion + per-point Z terms, Z -> dsum
     if (st != BH_OK) return st;
-    hipStream_t s = c->stream;
-    const uint32_t n = c->p_n;
     launch_unsort2(s, c->id[c->cur], n, c->ax, c->ay, c->rep_x, c->rep_y);
+    st = hip_check(c, hipStreamWaitEvent(s, c->ev_att, 0), "wait");
+    if (st != BH_OK) return st;
     const int cur = c->ty_cur, prev = (cur + 2) % 3, next = (cur + 1) % 3;
-    const float exag = (float)(t < p->exag_switch ? p->exag_early : p->exag_late);
-    const float mu = (float)(t < p->momentum_switch ? p->momentum_early : p->momentum_late);
-    // ty_xy holds the current embedding: tsne_stage_build staged it for this iteration
-    launch_tsne_update(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[cur], c->ty_y[cur], c->ty_xy,
-                       c->ty_x[prev], c->ty_y[prev], c->rep_x, c->rep_y, c->dsum,
-                       p->repulsion_scale > 0.0 ? p->repulsion_scale : 4.0, exag,
-                       (float)p->learning_rate, mu, c->ty_x[next], c->ty_y[next],
-                       keep_deriv ? c->tder_x : nullptr, keep_deriv ? c->tder_y : nullptr);
+    launch_tsne_combine(s, n, c->ty_x[cur], c->ty_y[cur], c->ty_x[prev], c->ty_y[prev], c->rep_x,
+                        c->rep_y, c->att_x, c->att_y, c->dsum,
+                        p->repulsion_scale > 0.0 ? p->repulsion_scale : 4.0,
+                        (float)p->learning_rate, mu, c->ty_x[next], c->ty_y[next],
+                        keep_deriv ? c->tder_x : nullptr, keep_deriv ? c->tder_y : nullptr);
     // rotate: next becomes current, current becomes previous
     c->ty_cur = next;
     return launch_status(c, "t-SNE step");

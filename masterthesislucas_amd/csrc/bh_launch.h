@@ -135,12 +135,14 @@ void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint
 // stage also writes the interleaved copy xy that the update's neighbour gathers read
 void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
                        float* y, float* m, uint32_t* id, float2* xy);
-void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
-                        const float* val, uint32_t n, const float* yx, const float* yy,
-                        const float2* xy, const float* pyx, const float* pyy,
-                        const float* rep_x, const float* rep_y, const double* z,
-                        double rep_scale, float exag, float lr, float mu, float* nyx, float* nyy,
-                        float* dx, float* dy);
+void launch_tsne_attract(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
+                         const float* val, uint32_t n, const float2* xy, float exag, float* att_x,
+                         float* att_y);
+void launch_tsne_combine(hipStream_t s, uint32_t n, const float* yx, const float* yy,
+                         const float* pyx, const float* pyy, const float* rep_x,
+                         const float* rep_y, const float* att_x, const float* att_y,
+                         const double* z, double rep_scale, float lr, float mu, float* nyx,
+                         float* nyy, float* dx, float* dy);
 uint32_t tsne_kl_blocks(uint32_t n);
 void launch_tsne_kl(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col, const float* val,
                     uint32_t n, const float* yx, const float* yy, const double* z,

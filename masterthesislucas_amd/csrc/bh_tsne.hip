@@ -29,37 +29,25 @@ __global__ __launch_bounds__(kBlock) void k_tsne_stage(const float* __restrict__
 }
 
 // One wave per row i: attraction exag * sum_j P_ij w_ij (y_j - y_i) with w = 1/(1+|y_j-y_i|^2)
-// (updateAttractive, tsne.h:356-381), derivative d_i = (s/Z) rep_i + attraction with s = 4
-// (updateRepulsive scaling, tsne.h:291-294; 1 in the reference's GPU step,
-// nBodySolverGpuBH.cu:424), then y' = y + lr d + mu (y - y_prev)
-// (updatePoints, tsne.h:179-206). The neighbours' positions are gathered from the interleaved
-// copy xy (one 8-byte load per non-zero instead of two 4-byte ones: the gathers, not the bytes,
-// bound this kernel).
-__global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ col,
-                                                        const float* __restrict__ val, uint32_t n,
-                                                        const float* __restrict__ yx,
-                                                        const float* __restrict__ yy,
-                                                        const float2* __restrict__ xy,
-                                                        const float* __restrict__ pyx,
-                                                        const float* __restrict__ pyy,
-                                                        const float* __restrict__ rep_x,
-                                                        const float* __restrict__ rep_y,
-                                                        const double* __restrict__ z, double rep_scale,
-                                                        float exag, float lr, float mu,
-                                                        float* __restrict__ nyx,
-                                                        float* __restrict__ nyy,
-                                                        float* __restrict__ dx_out,
-                                                        float* __restrict__ dy_out) {
+// (updateAttractive, tsne.h:356-381). It needs only the current embedding, so it runs on a
+// stream of its own while the tree is built and walked. The neighbours' positions are gathered
+// from the interleaved copy xy (one 8-byte load per non-zero instead of two 4-byte ones: the
+// gathers, not the bytes, bound this kernel).
+__global__ __launch_bounds__(kBlock) void k_tsne_attract(const uint32_t* __restrict__ row_ptr,
+                                                         const uint32_t* __restrict__ col,
+                                                         const float* __restrict__ val, uint32_t n,
+                                                         const float2* __restrict__ xy, float exag,
+                                                         float* __restrict__ att_x,
+                                                         float* __restrict__ att_y) {
     const int lane = threadIdx.x & 63;
     const uint32_t r = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (r >= n) return;
-    const float xi = yx[r], yi = yy[r];
+    const float2 yi = xy[r];
     float sx = 0.f, sy = 0.f;
     const uint32_t e0 = row_ptr[r], e1 = row_ptr[r + 1];
     for (uint32_t e = e0 + lane; e < e1; e += kWave) {
         const float2 yc = xy[col[e]];
-        const float ddx = yc.x - xi, ddy = yc.y - yi;
+        const float ddx = yc.x - yi.x, ddy = yc.y - yi.y;
         const float q = __builtin_amdgcn_rcpf(1.0f + fmaf(ddx, ddx, ddy * ddy));
         const float s = exag * val[e] * q;
         sx = fmaf(s, ddx, sx);
@@ -68,14 +56,38 @@ __global__ __launch_bounds__(kBlock) void k_tsne_update(const uint32_t* __restri
     sx = wave_sum_f(sx);
     sy = wave_sum_f(sy);
     if (lane != 0) return;
+    att_x[r] = sx;
+    att_y[r] = sy;
+}
+
+// derivative d_i = (s/Z) rep_i + attraction with s = 4 (updateRepulsive scaling,
+// tsne.h:291-294; 1 in the reference's GPU step, nBodySolverGpuBH.cu:424), then
+// y' = y + lr d + mu (y - y_prev) (updatePoints, tsne.h:179-206); one thread per point
+__global__ __launch_bounds__(kBlock) void k_tsne_combine(uint32_t n, const float* __restrict__ yx,
+                                                         const float* __restrict__ yy,
+                                                         const float* __restrict__ pyx,
+                                                         const float* __restrict__ pyy,
+                                                         const float* __restrict__ rep_x,
+                                                         const float* __restrict__ rep_y,
+                                                         const float* __restrict__ att_x,
+                                                         const float* __restrict__ att_y,
+                                                         const double* __restrict__ z,
+                                                         double rep_scale, float lr, float mu,
+                                                         float* __restrict__ nyx,
+                                                         float* __restrict__ nyy,
+                                                         float* __restrict__ dx_out,
+                                                         float* __restrict__ dy_out) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
     const float scale = (float)(rep_scale / *z);
-    const float dx = fmaf(scale, rep_x[r], sx);
-    const float dy = fmaf(scale, rep_y[r], sy);
+    const float dx = fmaf(scale, rep_x[r], att_x[r]);
+    const float dy = fmaf(scale, rep_y[r], att_y[r]);
     if (dx_out) {
         dx_out[r] = dx;
         dy_out[r] = dy;
     }
     if (nyx) {
+        const float xi = yx[r], yi = yy[r];
         nyx[r] = fmaf(lr, dx, xi) + mu * (xi - pyx[r]);
         nyy[r] = fmaf(lr, dy, yi) + mu * (yi - pyy[r]);
     }
@@ -137,15 +149,21 @@ void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t
     k_tsne_stage<<<div_up(n, kBlock), kBlock, 0, s>>>(yx, yy, n, x, y, m, id, xy);
 }
 
-void launch_tsne_update(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
-                        const float* val, uint32_t n, const float* yx, const float* yy,
-                        const float2* xy, const float* pyx, const float* pyy,
-                        const float* rep_x, const float* rep_y, const double* z,
-                        double rep_scale, float exag, float lr, float mu, float* nyx, float* nyy,
-                        float* dx, float* dy) {
-    k_tsne_update<<<div_up(n, kWavesPerBlock), kBlock, 0, s>>>(row_ptr, col, val, n, yx, yy, xy,
-                                                               pyx, pyy, rep_x, rep_y, z, rep_scale,
-                                                               exag, lr, mu, nyx, nyy, dx, dy);
+void launch_tsne_attract(hipStream_t s, const uint32_t* row_ptr, const uint32_t* col,
+                         const float* val, uint32_t n, const float2* xy, float exag, float* att_x,
+                         float* att_y) {
+    k_tsne_attract<<<div_up(n, kWavesPerBlock), kBlock, 0, s>>>(row_ptr, col, val, n, xy, exag,
+                                                                att_x, att_y);
+}
+
+void launch_tsne_combine(hipStream_t s, uint32_t n, const float* yx, const float* yy,
+                         const float* pyx, const float* pyy, const float* rep_x,
+                         const float* rep_y, const float* att_x, const float* att_y,
+                         const double* z, double rep_scale, float lr, float mu, float* nyx,
+                         float* nyy, float* dx, float* dy) {
+    k_tsne_combine<<<div_up(n, kBlock), kBlock, 0, s>>>(n, yx, yy, pyx, pyy, rep_x, rep_y, att_x,
+                                                        att_y, z, rep_scale, lr, mu, nyx, nyy, dx,
+                                                        dy);
 }
 
 uint32_t tsne_kl_blocks(uint32_t n) {
