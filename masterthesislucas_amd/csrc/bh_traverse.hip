@@ -142,7 +142,10 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     // wave-uniform (the compiler cannot tell that threadIdx.x / 64 is): keeps the stack pointer,
     // the stack reads and the loop control in scalar registers
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
+    // blocks in launch order: the hardware deals them round-robin over the 8 XCDs, so every XCD
+    // gets a share of the dense and of the sparse regions (measured 3% faster than giving each
+    // XCD a contiguous range of sinks: the balance matters more than L2 locality here)
+    const uint32_t wb = blockIdx.x;
     uint32_t base, count = 0, k0 = 0, end0 = 0;
     uint32_t part = 0, pstart = 0, pend = 0;
     if constexpr (BATCHED) {
