@@ -8,7 +8,7 @@ step = bounds, Morton codes, radix sort, gather, tree build, upward pass, traver
 leapfrog update, with the particle state resident in HBM. Metric: force evaluations per second
 (particles x steps / wall time).
 
-Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
+Prints one JSON line. --gpus above 1 is not supported: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -439,7 +439,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs the Barnes-Hut step on one GPU only; --gpus {args.gpus} "
-              f"(WORLD_SIZE={world}) is not supported by this build", file=sys.stderr)
+              f"(WORLD_SIZE={world}) is not supported", file=sys.stderr)
         sys.exit(2)
 
     import torch

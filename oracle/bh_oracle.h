@@ -2,8 +2,8 @@
  * CPU oracle for the Barnes-Hut force step — TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of the reference's CPU algorithm (LucasFurer/MasterThesisLucas), used as
- * the checker of the HIP path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it, and only as the checker or the timed CPU baseline, never as the product path.
+ * the reference for checking the HIP path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only to check results or as the timed CPU baseline, never as the product path.
  * Each function names the reference file:line it follows. Arithmetic is FP64 like the reference.
  *
  * Pinning (what checks each function beyond retyping it):

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so) and of the compiled reference (_ref/libref_bh.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-leg as the checker / CPU baseline. The product path never imports this module.
+leg, to check results / as the CPU baseline. The product path never imports this module.
 """
 from __future__ import annotations
 

@@ -1,4 +1,4 @@
-"""Pins of the CPU oracle (oracle/bh_oracle.c) before it is trusted as the checker.
+"""Pins of the CPU oracle (oracle/bh_oracle.c) before results are checked against it.
 
 Each oracle function is checked against something that does not restate its own formula:
 the compiled reference (oracle/_ref, built from the reference's own QuadTree/NBodySolverBH
