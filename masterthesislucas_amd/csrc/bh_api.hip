@@ -1224,7 +1224,12 @@ bh_status bh_get_tree(bh_ctx* c, int32_t* level, uint32_t* first, uint32_t* coun
                       uint8_t* flags, float* cx, float* cy, float* mass) {
     uint32_t nn = 0;
     bh_status st = bh_tree_info(c, &nn, nullptr, nullptr);
-    if (st == BH_OK) st = ensure_moments(c);
+    if (st == BH_OK && !c->moments) {
+        // the node records' moments are made on the context's stream, which the copy below
+        // (on the null stream) does not wait for
+        st = ensure_moments(c);
+        if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "upward");
+    }
     if (st != BH_OK) return st;
     std::vector<Node> h(nn);
     if (nn) {

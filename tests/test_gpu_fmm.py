@@ -165,6 +165,31 @@ def test_fmm_solver_rejects_other_setups(gpu):
     ctx.close()
 
 
+def test_node_records_made_on_demand_under_fmm(gpu):
+    """With the FMM solver a build skips the Barnes-Hut upward pass; exporting the tree or a
+    Barnes-Hut walk on that build makes it first, with the same node records as a build under
+    the Barnes-Hut solver."""
+    x, y = _points(20000, 80, "clusters")
+    plain = BHContext(x.size, kernel="tsne", tree_mode="morton_fixed")
+    plain.set_particles(x, y)
+    plain.build()
+    want = plain.tree()
+    wx, wy, wz = plain.forces(0.5)
+    plain.close()
+    ctx = BHContext(x.size, kernel="tsne", tree_mode="morton_fixed")
+    ctx.set_solver("fmm_sym")
+    ctx.set_particles(x, y)
+    ctx.build()
+    got = ctx.tree()
+    for k in ("level", "first", "count", "skip", "cx", "cy", "mass"):
+        assert np.array_equal(got[k], want[k]), k
+    ctx.build()
+    ctx.set_solver("bh")
+    ax, ay, z = ctx.forces(0.5)
+    ctx.close()
+    assert np.array_equal(ax, wx) and np.array_equal(ay, wy) and z == wz
+
+
 def test_tsne_step_with_fmm_matches_oracle(gpu):
     """One t-SNE iteration with the FMM repulsion: d = 4/Z rep + attraction, y' = y + lr d
     (iteration 0, no momentum), against the oracle's FMM and attraction."""
