@@ -37,9 +37,11 @@
 template <typename T>
 class NBodySolverHipBH : public NBodySolver<T> {
 public:
+    // solver BH_SOLVER_FMM_SYM stands in for NBodySolverFMM_SYM_MORTON (BH_KERNEL_TSNE with
+    // BH_TREE_MORTON_FIXED); the default is the Barnes-Hut walk (NBodySolverBH)
     NBodySolverHipBH(bh_kernel kernel, int initMaxChildren, double initTheta, uint32_t n_max,
                      float G = 1.0f, float softening = 0.0f,
-                     bh_tree_mode mode = BH_TREE_QUADTREE) {
+                     bh_tree_mode mode = BH_TREE_QUADTREE, int solver = BH_SOLVER_BH) {
         this->maxChildren = initMaxChildren;
         this->theta = initTheta;
         bh_config cfg;
@@ -54,6 +56,7 @@ public:
         code_bits_ = mode == BH_TREE_QUADTREE ? cfg.code_bits : 16;
         mode_ = mode;
         check(bh_create(&ctx_, &cfg), "bh_create");
+        if (solver != BH_SOLVER_BH) check(bh_set_solver(ctx_, solver), "bh_set_solver");
     }
 
     NBodySolverHipBH(const NBodySolverHipBH&) = delete;

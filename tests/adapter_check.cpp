@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -59,9 +60,48 @@ double quantile(std::vector<double> v, double q) {
 
 }  // namespace
 
+// `adapter_check N THETA fmm OUT`: the adapter with BH_SOLVER_FMM_SYM on a fixed-depth Morton
+// tree, in the place tsne.h:121-122 registers NBodySolverFMM_SYM_MORTON (that solver needs
+// boost's sort, absent here, so the comparison is made by the caller against the oracle).
+// Writes n (int32), the FP32 positions x then y, the FP64 derivatives x then y and Z to OUT.
+int fmm_dump(int n, double theta, const char* out) {
+    NBodySolver<TsnePoint2D>* solver = new NBodySolverHipBH<TsnePoint2D>(
+        BH_KERNEL_TSNE, 16, theta, n, 1.0f, 0.0f, BH_TREE_MORTON_FIXED, BH_SOLVER_FMM_SYM);
+    std::vector<TsnePoint2D> pts = make_points(n);
+    glm::dvec2 lo(1e300), hi(-1e300);
+    for (auto& p : pts) {
+        lo = glm::min(lo, p.position);
+        hi = glm::max(hi, p.position);
+        p.derivative = glm::dvec2(0.0);
+    }
+    solver->updateTree(pts, lo, hi);
+    double total = 0.0;
+    solver->solveNbody(total, pts);
+    delete solver;
+    std::FILE* f = std::fopen(out, "wb");
+    if (!f) return 2;
+    std::vector<float> pos(2 * static_cast<size_t>(n));
+    std::vector<double> der(2 * static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        pos[i] = static_cast<float>(pts[i].position.x);
+        pos[n + i] = static_cast<float>(pts[i].position.y);
+        der[i] = pts[i].derivative.x;
+        der[n + i] = pts[i].derivative.y;
+    }
+    const int32_t n32 = n;
+    const bool ok = std::fwrite(&n32, sizeof n32, 1, f) == 1 &&
+                    std::fwrite(pos.data(), sizeof(float), pos.size(), f) == pos.size() &&
+                    std::fwrite(der.data(), sizeof(double), der.size(), f) == der.size() &&
+                    std::fwrite(&total, sizeof total, 1, f) == 1;
+    std::fclose(f);
+    std::printf("{\"n\": %d, \"theta\": %g, \"z\": %.17g}\n", n, theta, total);
+    return ok ? 0 : 2;
+}
+
 int main(int argc, char** argv) {
     const int n = argc > 1 ? std::atoi(argv[1]) : 20000;
     const double theta = argc > 2 ? std::atof(argv[2]) : 0.5;
+    if (argc > 4 && std::string(argv[3]) == "fmm") return fmm_dump(n, theta, argv[4]);
     std::map<std::string, NBodySolver<TsnePoint2D>*> solvers;
     solvers["BH"] = new NBodySolverBH<TsnePoint2D>(&TSNEBHPNKernel, &TSNEBHPPKernel, 16, theta);
     solvers["HIP_BH"] = new NBodySolverHipBH<TsnePoint2D>(BH_KERNEL_TSNE, 16, theta, n);
