@@ -371,11 +371,24 @@ __global__ __launch_bounds__(THREADS) void k_radix_counts(const K* __restrict__ 
         const uint32_t i = base + r * THREADS + threadIdx.x;
         k[r] = i < n ? keys[i] : (K)0;
     }
+    // A wave holds 64 consecutive keys; each run of equal digits among neighbouring lanes is
+    // counted by its first lane with one add of the run length. Keys that arrive nearly sorted
+    // (particles kept in the previous step's order) would otherwise send most of a wave's adds
+    // to one LDS word, which serialises them.
+    const int lane = threadIdx.x & 63;
+    const uint64_t le = lane_mask_lt() | (1ull << lane);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        if (base + r * THREADS + threadIdx.x >= n) continue;
-        for (int p = 0; p < passes; ++p)
-            atomicAdd(&h[p][(uint32_t)(k[r] >> (8 * p)) & 0xFFu], 1u);
+        const bool valid = base + r * THREADS + threadIdx.x < n;  // invalid lanes are a suffix
+        const K prev = (K)__shfl_up((unsigned long long)k[r], 1);
+        const uint64_t stops = __ballot(!valid);
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t d = (uint32_t)(k[r] >> (8 * p)) & 0xFFu;
+            const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> (8 * p)) & 0xFFu));
+            const uint64_t after = (__ballot(head) | stops) & ~le;
+            const uint32_t end = after ? (uint32_t)__builtin_ctzll(after) : 64u;
+            if (head) atomicAdd(&h[p][d], end - (uint32_t)lane);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < passes * 256; i += THREADS)
