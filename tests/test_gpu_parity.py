@@ -71,6 +71,43 @@ def test_sort_many_tiles_with_duplicates(gpu, mode, n):
     ctx.close()
 
 
+@pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
+@pytest.mark.parametrize("n", [70_001, 1_000_003])
+@pytest.mark.parametrize("order", ["sorted", "swapped", "reversed"])
+def test_sort_nearly_sorted_input(gpu, mode, n, order):
+    """Points handed over already in code order (as a leapfrog step keeps them), with 1% of
+    them swapped, or in reverse: a wave then holds long runs of equal digits, which the counting
+    kernel adds once per run. Codes and order bit-exact against the oracle's stable sort."""
+    rng = np.random.default_rng(n + len(order))
+    x = rng.normal(size=n).astype(np.float32)
+    y = rng.normal(size=n).astype(np.float32)
+    x[: n // 10] = x[0]  # one crowded spot: equal codes across many tiles
+    y[: n // 10] = y[0]
+    xd, yd = x.astype(np.float64), y.astype(np.float64)
+    minx, miny, maxx, maxy = xd.min(), yd.min(), xd.max(), yd.max()
+    if mode == "quadtree":
+        ref = O.morton64(xd, yd, minx, miny, O.quadtree_side(minx, miny, maxx, maxy), 32)
+    else:
+        ref = O.morton32(xd, yd, minx, miny, max(maxx + (-minx), maxy + (-miny))).astype(np.uint64)
+    p = O.stable_sort(ref)
+    if order == "swapped":
+        i = rng.integers(0, n, n // 100)
+        j = rng.integers(0, n, n // 100)
+        for a, b in zip(i, j):
+            p[a], p[b] = p[b], p[a]
+    elif order == "reversed":
+        p = p[::-1].copy()
+    x, y, ref = x[p], y[p], ref[p]
+    ctx = _ctx(n, kernel="gravity", tree_mode=mode, softening=EPS)
+    ctx.set_particles(x, y)
+    ctx.build()
+    codes, perm = ctx.morton()
+    rperm = O.stable_sort(ref)
+    assert np.array_equal(perm, rperm)
+    assert np.array_equal(codes, ref[rperm])
+    ctx.close()
+
+
 def test_morton32_caller_bounds(gpu):
     """updateTree honours the caller's minPos/maxPos in the Morton solvers."""
     p = uniform_disk(4096, seed=3)
