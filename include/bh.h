@@ -250,7 +250,12 @@ bh_status bh_counters(bh_ctx* ctx, uint64_t* pn, uint64_t* pp);
 /* ---- timing (HIP events on the context's stream) ---- */
 /* phases: 0 bounds+morton, 1 sort, 2 gather, 3 tree, 4 upward, 5 traversal, 6 integrate */
 #define BH_NUM_PHASES 7
+/* enable != 0 times every phase, 0 none. Each timed phase records two events per launch on the
+ * stream, which delays the next kernel's start: timing every phase lengthens a 1M-particle
+ * leapfrog step by about 5%. */
 bh_status bh_profile_enable(bh_ctx* ctx, int enable);
+/* time only the phases whose bit (1 << phase) is set in mask; BH_ERR_ARG for bits >= BH_NUM_PHASES */
+bh_status bh_profile_phases(bh_ctx* ctx, uint32_t mask);
 /* accumulated milliseconds and launch counts per phase since the last reset */
 bh_status bh_profile_read(bh_ctx* ctx, double* ms, uint64_t* launches, int reset);
 bh_status bh_synchronize(bh_ctx* ctx);

@@ -98,6 +98,7 @@ SIGNATURES = {
     "bh_get_tree_slots": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "bh_counters": (_i32, [_vp, _vp, _vp]),
     "bh_profile_enable": (_i32, [_vp, C.c_int]),
+    "bh_profile_phases": (_i32, [_vp, C.c_uint32]),
     "bh_profile_read": (_i32, [_vp, _vp, _vp, C.c_int]),
     "bh_synchronize": (_i32, [_vp]),
     "bh_traverse_variant_count": (_i32, []),

@@ -138,7 +138,7 @@ struct bh_ctx {
     float* part = nullptr;      // split traversal outputs: x, y, total parts, F n_max each
     uint32_t part_split = 0;    // F the part buffer was sized for
     uint32_t trav_blocks = 0;  // blocks of the last traversal launch
-    bool profile = false;
+    uint32_t profile = 0;  // bit p: phase p is timed with HIP events
     PhaseTimer phases[BH_NUM_PHASES];
 
     std::vector<void*> allocs;
@@ -174,7 +174,7 @@ bh_status dalloc(bh_ctx* c, T** p, size_t count) {
 }
 
 void phase_begin(bh_ctx* c, int ph) {
-    if (!c->profile) return;
+    if (!((c->profile >> ph) & 1u)) return;
     PhaseTimer& t = c->phases[ph];
     std::pair<hipEvent_t, hipEvent_t> ev;
     if (!t.spare.empty()) {
@@ -189,7 +189,7 @@ void phase_begin(bh_ctx* c, int ph) {
 }
 
 void phase_end(bh_ctx* c, int ph) {
-    if (!c->profile) return;
+    if (!((c->profile >> ph) & 1u)) return;
     PhaseTimer& t = c->phases[ph];
     (void)hipEventRecord(t.pending.back().second, c->stream);
     t.launches++;
@@ -834,8 +834,8 @@ bh_status bh_leapfrog_graph(bh_ctx* c, int steps, float dt, double theta) {
         graph_reset(c);
         // capture one step from each ping-pong parity; the host-side state the build updates
         // (current buffer, sort parity) is the same on replay, so it is restored afterwards
-        const bool prof = c->profile;
-        c->profile = false;
+        const uint32_t prof = c->profile;
+        c->profile = 0u;
         const int cur0 = c->cur;
         st = hip_check(c, hipStreamSynchronize(c->stream), "graph sync");
         for (int k = 0; k < 2 && st == BH_OK; ++k) {
@@ -1295,7 +1295,14 @@ bh_status bh_counters(bh_ctx* c, uint64_t* pn, uint64_t* pp) {
 
 bh_status bh_profile_enable(bh_ctx* c, int enable) {
     if (!c) return BH_ERR_ARG;
-    c->profile = enable != 0;
+    c->profile = enable ? (1u << BH_NUM_PHASES) - 1u : 0u;
+    return BH_OK;
+}
+
+bh_status bh_profile_phases(bh_ctx* c, uint32_t mask) {
+    if (!c) return BH_ERR_ARG;
+    if (mask >> BH_NUM_PHASES) return fail(c, BH_ERR_ARG, "phase mask 0x%x has bits beyond phase %d", mask, BH_NUM_PHASES - 1);
+    c->profile = mask;
     return BH_OK;
 }
 

@@ -210,6 +210,13 @@ class BHContext:
     def profile_enable(self, on=True):
         self._c(_lib.load().bh_profile_enable(self._h, 1 if on else 0))
 
+    def profile_phases(self, names):
+        """Time only the named phases (names from PHASE_NAMES); the others record no events."""
+        mask = 0
+        for name in names:
+            mask |= 1 << _lib.PHASE_NAMES.index(name)
+        self._c(_lib.load().bh_profile_phases(self._h, mask))
+
     def profile_read(self, reset=True):
         ms = np.zeros(_lib.NUM_PHASES, np.float64)
         la = np.zeros(_lib.NUM_PHASES, np.uint64)

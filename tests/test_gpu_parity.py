@@ -8,7 +8,8 @@ particles whose FP32 acceptance test lands on the other side of theta).
 import numpy as np
 import pytest
 
-from masterthesislucas_amd import BHContext
+from masterthesislucas_amd import BHContext, _lib
+from masterthesislucas_amd._lib import BHError
 from masterthesislucas_amd.workloads import (plummer, sample_indices, two_plummer_disks,
                                              uniform_disk)
 from oracle import oracle as O
@@ -397,3 +398,27 @@ def test_every_traversal_variant(gpu, variant, kernel):
     assert abs(tot - bt.sum()) <= 1e-5 * abs(bt.sum())
     pn, pp = ctx.counters()
     assert abs(int(pn) - int(cnt[0])) <= 1e-3 * cnt[0]
+
+
+def test_phase_timers_by_mask(gpu):
+    """bh_profile_phases: only the selected phases record events and launches; a mask with bits
+    beyond the last phase is refused; bh_profile_enable(1) times all of them again."""
+    n = 20000
+    p = uniform_disk(n, seed=23)
+    ctx = _ctx(n, kernel="gravity", tree_mode="quadtree", softening=EPS)
+    ctx.set_particles(p.x, p.y, p.m, p.vx, p.vy)
+    ctx.leapfrog(1, 1e-4, 0.5)
+    ctx.profile_phases(["traversal"])
+    ctx.profile_read(reset=True)
+    ctx.leapfrog(3, 1e-4, 0.5)
+    ph = ctx.profile_read(reset=True)
+    assert ph["traversal"][1] == 3 and ph["traversal"][0] > 0
+    assert all(ph[k] == (0.0, 0) for k in ph if k != "traversal")
+    with pytest.raises(BHError):
+        ctx._c(_lib.load().bh_profile_phases(ctx.handle, 1 << 7))
+    ctx.profile_enable(True)
+    ctx.leapfrog(2, 1e-4, 0.5)
+    ph = ctx.profile_read(reset=True)
+    assert all(ph[k][1] >= 2 for k in ph), ph
+    ctx.profile_enable(False)
+    ctx.close()
