@@ -12,6 +12,7 @@
 // additions used here are zero or normal numbers.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
@@ -159,16 +160,29 @@ BH_SFP_HD bh_sfp_quant bh_sfp_quant_prepare(uint64_t s_bits) {
     q.M = (s_bits & BH_SFP_FRAC) | BH_SFP_HIDDEN;
     q.es = e - 1075;
     q.pad = 0;
-    // restoring long division of the 116-bit all-ones numerator by M
-    uint64_t rem = 0, quo = 0;
-    for (int bit = 115; bit >= 0; --bit) {
-        rem = (rem << 1) | 1u;
-        if (rem >= q.M) {
-            rem -= q.M;
-            if (bit < 64) quo |= 1ull << bit;
-        }
+    if (q.M == BH_SFP_HIDDEN) {  // (2^116 - 1) / 2^52 = 2^64 - 2^-52
+        q.R = ~0ull;
+        return q;
     }
-    q.R = quo;
+    // An FP64 quotient is within 2^12 of the true one (relative error 2^-53, quotient below
+    // 2^64 since M > 2^52); the exact remainder N - R M (|.| < 2^66, two's complement in 128
+    // bits) then gives a second FP64 correction, and at most a step or two finishes it.
+    const __int128 M = (__int128)q.M;
+    const __int128 N = ((__int128)1 << 116) - 1;
+    uint64_t r = (uint64_t)(0x1p116 / (double)q.M);
+    __int128 rem = N - (__int128)r * M;
+    const int64_t adj = (int64_t)floor((double)(int64_t)(rem >> 11) * 2048.0 / (double)q.M);
+    r += (uint64_t)adj;
+    rem -= (__int128)adj * M;
+    while (rem < 0) {
+        --r;
+        rem += M;
+    }
+    while (rem >= M) {
+        ++r;
+        rem -= M;
+    }
+    q.R = r;
     return q;
 }
 

@@ -54,6 +54,26 @@ static double ref_quantize(double d, double S, int B) {
 int main(int argc, char** argv) {
     long n = argc > 1 ? atol(argv[1]) : 2000000;
     long cases = 0;
+    // 0. the quantiser's reciprocal R = floor((2^116 - 1) / M) against integer division, for
+    //    the significand range's ends and random significands
+    {
+        const unsigned __int128 N = ((unsigned __int128)1 << 116) - 1;
+        const uint64_t lo = 1ull << 52, hi = (1ull << 53) - 1;
+        for (long i = 0; i < n; ++i) {
+            uint64_t M;
+            if (i < 64) M = (i & 1) ? hi - (uint64_t)(i >> 1) : lo + (uint64_t)(i >> 1);
+            else M = lo | (next_u64() & (lo - 1));
+            const uint64_t s = (M & ~lo) | (1023ull << 52);  // 1.f with significand M
+            const uint64_t got = bh_sfp_quant_prepare(s).R;
+            const uint64_t want = (uint64_t)(N / M);
+            if (got != want) {
+                printf("FAIL reciprocal M=%llu got %llu want %llu\n", (unsigned long long)M,
+                       (unsigned long long)got, (unsigned long long)want);
+                return 1;
+            }
+            ++cases;
+        }
+    }
     // 1. float -> double conversion
     for (long i = 0; i < n; ++i) {
         uint32_t b;
