@@ -225,6 +225,16 @@ bh_status bh_learned_collect(bh_ctx* ctx, bh_mlp* mlp, double theta, double band
 bh_status bh_forces_learned(bh_ctx* ctx, const bh_mlp* mlp, double theta, double band, float* ax,
                             float* ay, double* total);
 
+/* ---- failure reporting (no reference counterpart: the reference checks no errors,
+ * cudaHelper.cuh:22-31) ----
+ * Every call that builds a tree (bh_build, bh_leapfrog, bh_leapfrog_graph, bh_tsne_step,
+ * bh_tsne_run, bh_tsne_kl) returns BH_ERR_ARG when a particle position is NaN or infinite (the
+ * bounds pass flags it) and BH_ERR_HIP when the radix sort's look-back gave up waiting (the
+ * order would not be valid); the tree is then marked unbuilt. Set the look-back bound (polls of
+ * a tile waiting on an earlier tile; default 2^22, a wait that always ends on a working device)
+ * to a small value only to exercise that error path. */
+bh_status bh_set_sort_spin_limit(bh_ctx* ctx, uint32_t polls);
+
 /* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
 /* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */
 bh_status bh_get_bounds(bh_ctx* ctx, double* bounds, double* root_side);

@@ -11,6 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmtl_bh.so")
 
 BH_OK = 0
+BH_ERR_ARG = 1
+BH_ERR_HIP = 2
+BH_ERR_OOM = 3
+BH_ERR_CAPACITY = 4
+BH_ERR_STATE = 5
 STATUS_NAMES = {0: "BH_OK", 1: "BH_ERR_ARG", 2: "BH_ERR_HIP", 3: "BH_ERR_OOM",
                 4: "BH_ERR_CAPACITY", 5: "BH_ERR_STATE"}
 
@@ -105,6 +110,7 @@ SIGNATURES = {
     "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
     "bh_set_traverse_split": (_i32, [_vp, C.c_int]),
     "bh_set_solver": (_i32, [_vp, C.c_int]),
+    "bh_set_sort_spin_limit": (_i32, [_vp, _u32]),
 }
 
 _lib = None

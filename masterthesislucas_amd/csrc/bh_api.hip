@@ -83,6 +83,9 @@ struct bh_ctx {
     double* dpart = nullptr;             // direct sum: per-split partial sums + output
     uint64_t dsink_cap = 0, dpart_cap = 0;
 
+    uint32_t* dstatus = nullptr;   // build status words (kStatusWords), see status_check
+    uint32_t* h_status = nullptr;  // pinned copy
+    uint32_t sort_spin = kSortSpinLimit;
     BuildParams* h_prm = nullptr;  // pinned
     double* h_dsum = nullptr;
     unsigned long long* h_counters = nullptr;  // 2 per traversal block
@@ -213,6 +216,45 @@ bh_status launch_status(bh_ctx* c, const char* what) {
     return hip_check(c, hipGetLastError(), what);
 }
 
+// Build status: the words are cleared before a call's first build and copied to the host
+// after its last launch; status_check reads them after the call's stream synchronisation.
+bh_status status_begin(bh_ctx* c) {
+    return hip_check(c, hipMemsetAsync(c->dstatus, 0, kStatusWords * sizeof(uint32_t), c->stream),
+                     "status clear");
+}
+
+bh_status status_end(bh_ctx* c) {
+    return hip_check(c, hipMemcpyAsync(c->h_status, c->dstatus, kStatusWords * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, c->stream),
+                     "status copy");
+}
+
+// after the stream synchronisation that follows status_end
+bh_status status_check(bh_ctx* c) {
+    if (c->h_status[1]) {
+        c->built = false;
+        c->have_acc = false;
+        return fail(c, BH_ERR_ARG, "a particle position is not finite (NaN or infinity)");
+    }
+    if (c->h_status[0]) {
+        c->built = false;
+        c->have_acc = false;
+        return fail(c, BH_ERR_HIP,
+                    "radix sort: a look-back wait exceeded %u polls; the sort order is not valid",
+                    c->sort_spin);
+    }
+    return BH_OK;
+}
+
+// end of a call that built: status copy, synchronisation, status check
+bh_status finish_build_call(bh_ctx* c, const char* what) {
+    bh_status st = status_end(c);
+    if (st != BH_OK) return st;
+    st = hip_check(c, hipStreamSynchronize(c->stream), what);
+    if (st != BH_OK) return st;
+    return status_check(c);
+}
+
 // the reference GPU step's tree depth (NBodySolverGpuBH constructed with 6, tsneGpu.h:145)
 constexpr int kGpuRefDepth = 6;
 
@@ -242,18 +284,20 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 0);
     if (c->nsys) {
         launch_bounds_batched(s, c->x[cur], c->y[cur], c->bblk, c->nbblk, c->bblk_off, c->sys_off,
-                              c->nsys, c->bpart, c->sys_prm, c->prm);
+                              c->nsys, c->bpart, c->sys_prm, c->prm, c->dstatus);
         launch_morton_batched(s, c->x[cur], c->y[cur], n, c->sys_off, c->nsys, c->sys_prm,
                               c->qbits, (uint64_t*)c->keys_a, c->vals_a);
     } else {
         if (c->cfg.tree_mode == BH_TREE_GPU_REF) {
             // the reference GPU step takes the bounds of the points themselves
-            launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm, kTreeGpuRef, nullptr);
+            launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm, kTreeGpuRef, nullptr,
+                          c->dstatus);
             launch_morton_gpuref(s, c->x[cur], c->y[cur], n, c->prm, fixed, (uint32_t*)c->keys_a,
                                  c->vals_a);
         } else {
             launch_bounds(s, c->x[cur], c->y[cur], n, c->bounds_part, c->prm,
-                          quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds);
+                          quadtree ? kTreeQuadTree : kTreeMortonFixed, quadtree ? nullptr : bounds,
+                          c->dstatus);
             launch_morton(s, c->x[cur], c->y[cur], n, c->prm, c->B, c->keys_a, c->vals_a,
                           c->wide);
         }
@@ -263,17 +307,18 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 1);
     bool in_b = false;
     const int key_bits = c->nsys ? 2 * c->qbits + 2 * c->R : 2 * c->B;
-    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, key_bits, c->wide, c->sort_scratch,
-                      &in_b);
+    launch_radix_sort(s, c->keys_a, c->vals_a, c->keys_b, c->vals_b, n, key_bits, c->wide,
+                      c->sort_scratch, c->dstatus, c->sort_spin, &in_b);
     phase_end(c, 1);
     c->keys_in_b = in_b;
-    const void* codes = in_b ? c->keys_b : c->keys_a;
+    void* codes = in_b ? c->keys_b : c->keys_a;
     const uint32_t* perm = in_b ? c->vals_b : c->vals_a;
 
     phase_begin(c, 2);
     const int nxt = 1 - cur;
     launch_gather(s, perm, n, c->x[cur], c->y[cur], c->m[cur], c->vx[cur], c->vy[cur], c->id[cur],
-                  c->x[nxt], c->y[nxt], c->m[nxt], c->vx[nxt], c->vy[nxt], c->id[nxt], c->pos4);
+                  c->x[nxt], c->y[nxt], c->m[nxt], c->vx[nxt], c->vy[nxt], c->id[nxt], c->pos4,
+                  c->dstatus, codes, c->wide);
     phase_end(c, 2);
     c->cur = nxt;
 
@@ -281,8 +326,8 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     phase_begin(c, 3);
     launch_tree(s, codes, c->wide, n, c->B, c->cfg.max_children, fixed, quadtree ? 1 : 0,
                 c->leaf_lvl, c->lcp1, c->cnt, c->noff, c->scan_part, c->prm, c->nodes, c->parent,
-                c->child4, c->cap_nodes, c->R, c->nsys ? c->sys_prm : nullptr);
-    if (c->nsys) launch_batch_roots(s, c->sys_off, c->nsys, c->noff, c->roots);
+                c->child4, c->cap_nodes, c->R, c->nsys ? c->sys_prm : nullptr, c->dstatus);
+    if (c->nsys) launch_batch_roots(s, c->sys_off, c->nsys, c->noff, c->prm, c->roots);
     phase_end(c, 3);
 
     // the FMM solver computes its own moments: the node records' moments are made when a
@@ -414,6 +459,7 @@ void bh_destroy(bh_ctx* c) {
     if (c->dpart) (void)hipFree(c->dpart);
     if (c->part) (void)hipFree(c->part);
     if (c->fmm_buf) (void)hipFree(c->fmm_buf);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -560,6 +606,11 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
         TRY(dalloc(c, &c->slot_b, c->slot_cap));
         TRY(dalloc(c, &c->slot_c, c->slot_cap));
     }
+    TRY(dalloc(c, &c->dstatus, kStatusWords));
+    TRY(hip_check(c, hipHostMalloc((void**)&c->h_status, kStatusWords * sizeof(uint32_t), 0),
+                  "hipHostMalloc"));
+    memset(c->h_status, 0, kStatusWords * sizeof(uint32_t));
+    launch_fill_u32(c->stream, c->dstatus, kStatusWords, 0u);
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_prm, sizeof(BuildParams), 0), "hipHostMalloc"));
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_dsum, sizeof(double), 0), "hipHostMalloc"));
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_stats, kStatsBlocks * sizeof(uint2), 0),
@@ -724,9 +775,10 @@ bh_status bh_get_particles(bh_ctx* c, float* x, float* y, float* vx, float* vy) 
 
 bh_status bh_build(bh_ctx* c, const double* bounds) {
     if (!c) return BH_ERR_ARG;
-    bh_status st = do_build(c, bounds);
+    bh_status st = status_begin(c);
+    if (st == BH_OK) st = do_build(c, bounds);
     if (st != BH_OK) return st;
-    return hip_check(c, hipStreamSynchronize(c->stream), "build");
+    return finish_build_call(c, "build");
 }
 
 static bh_status forces_out(bh_ctx* c, double theta, float* ax, float* ay, double* total,
@@ -771,7 +823,8 @@ bh_status bh_forces_device(bh_ctx* c, double theta, float* d_ax, float* d_ay, do
 bh_status bh_leapfrog(bh_ctx* c, int steps, float dt, double theta) {
     if (!c || steps < 0) return BH_ERR_ARG;
     if (c->n == 0) return fail(c, BH_ERR_STATE, "no particles");
-    bh_status st;
+    bh_status st = status_begin(c);
+    if (st != BH_OK) return st;
     if (!c->have_acc) {
         st = do_build(c, nullptr);
         if (st == BH_OK) st = do_forces(c, theta, false);
@@ -794,7 +847,7 @@ bh_status bh_leapfrog(bh_ctx* c, int steps, float dt, double theta) {
     }
     st = launch_status(c, "leapfrog launch");
     if (st != BH_OK) return st;
-    return hip_check(c, hipStreamSynchronize(c->stream), "leapfrog");
+    return finish_build_call(c, "leapfrog");
 }
 
 // one kick-drift-kick step on the context's stream (no host synchronisation, no profiling)
@@ -824,7 +877,8 @@ static void graph_reset(bh_ctx* c) {
 bh_status bh_leapfrog_graph(bh_ctx* c, int steps, float dt, double theta) {
     if (!c || steps < 0) return BH_ERR_ARG;
     if (c->n == 0) return fail(c, BH_ERR_STATE, "no particles");
-    bh_status st;
+    bh_status st = status_begin(c);
+    if (st != BH_OK) return st;
     if (!c->have_acc) {
         st = do_build(c, nullptr);
         if (st == BH_OK) st = do_forces(c, theta, false);
@@ -873,7 +927,7 @@ bh_status bh_leapfrog_graph(bh_ctx* c, int steps, float dt, double theta) {
     }
     c->built = true;
     c->have_acc = true;
-    return hip_check(c, hipStreamSynchronize(c->stream), "graph steps");
+    return finish_build_call(c, "graph steps");
 }
 
 bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* ax, double* ay,
@@ -1080,7 +1134,8 @@ bh_status bh_tsne_step(bh_ctx* c, const bh_tsne_params* p, int32_t iteration, fl
                        double* z) {
     if (!c || !p || iteration < 0) return BH_ERR_ARG;
     if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
-    bh_status st = tsne_iteration(c, p, iteration, dx || dy);
+    bh_status st = status_begin(c);
+    if (st == BH_OK) st = tsne_iteration(c, p, iteration, dx || dy);
     if (st != BH_OK) return st;
     hipStream_t s = c->stream;
     const size_t b = (size_t)c->p_n * 4;
@@ -1088,7 +1143,7 @@ bh_status bh_tsne_step(bh_ctx* c, const bh_tsne_params* p, int32_t iteration, fl
     if (st == BH_OK && dy) st = hip_check(c, hipMemcpyAsync(dy, c->tder_y, b, hipMemcpyDeviceToHost, s), "dy");
     if (st == BH_OK && z) st = hip_check(c, hipMemcpyAsync(c->h_dsum, c->dsum, 8, hipMemcpyDeviceToHost, s), "z");
     if (st != BH_OK) return st;
-    st = hip_check(c, hipStreamSynchronize(s), "t-SNE step");
+    st = finish_build_call(c, "t-SNE step");
     if (st == BH_OK && z) *z = *c->h_dsum;
     return st;
 }
@@ -1096,17 +1151,20 @@ bh_status bh_tsne_step(bh_ctx* c, const bh_tsne_params* p, int32_t iteration, fl
 bh_status bh_tsne_run(bh_ctx* c, const bh_tsne_params* p, int32_t first, int32_t count) {
     if (!c || !p || first < 0 || count < 0) return BH_ERR_ARG;
     if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
+    bh_status st = status_begin(c);
+    if (st != BH_OK) return st;
     for (int32_t t = first; t < first + count; ++t) {
-        bh_status st = tsne_iteration(c, p, t, false);
+        st = tsne_iteration(c, p, t, false);
         if (st != BH_OK) return st;
     }
-    return hip_check(c, hipStreamSynchronize(c->stream), "t-SNE run");
+    return finish_build_call(c, "t-SNE run");
 }
 
 bh_status bh_tsne_kl(bh_ctx* c, double* kl, double* z_exact) {
     if (!c) return BH_ERR_ARG;
     if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");
-    bh_status st = tsne_stage_build(c);
+    bh_status st = status_begin(c);
+    if (st == BH_OK) st = tsne_stage_build(c);
     if (st != BH_OK) return st;
     const uint32_t n = c->p_n;
     // exact Z: FP64 direct sum over all sinks (their totals), then the KL sum over P
@@ -1142,10 +1200,16 @@ bh_status bh_tsne_kl(bh_ctx* c, double* kl, double* z_exact) {
     double h[2] = {0.0, 0.0};
     if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&h[0], c->kl_out, 8, hipMemcpyDeviceToHost, s), "kl");
     if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&h[1], c->tz, 8, hipMemcpyDeviceToHost, s), "z");
-    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "KL");
+    if (st == BH_OK) st = finish_build_call(c, "KL");
     if (st != BH_OK) return st;
     if (kl) *kl = h[0];
     if (z_exact) *z_exact = h[1];
+    return BH_OK;
+}
+
+bh_status bh_set_sort_spin_limit(bh_ctx* c, uint32_t polls) {
+    if (!c) return BH_ERR_ARG;
+    c->sort_spin = polls;
     return BH_OK;
 }
 

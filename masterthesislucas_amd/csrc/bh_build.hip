@@ -39,15 +39,25 @@ __device__ __forceinline__ float4 block_minmax4(float4 r) {
     return r;
 }
 
+// One position as a bounds item. fminf/fmaxf drop NaN, so a NaN coordinate enters as -inf:
+// the bounds are then finite exactly when every position is (an infinity stays infinite).
+__device__ __forceinline__ float4 bounds_item(float x, float y) {
+    const float xl = x == x ? x : -INFINITY, yl = y == y ? y : -INFINITY;
+    return make_float4(xl, yl, x, y);
+}
+
+// status word 1: the build saw a non-finite position (read by the host after the build)
+__device__ __forceinline__ void flag_nonfinite(float4 r, uint32_t* status) {
+    if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z) && isfinite(r.w))) status[1] = 1u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* __restrict__ xs,
                                                            const float* __restrict__ ys,
                                                            uint32_t n,
                                                            float4* __restrict__ partials) {
     float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float x = xs[i], y = ys[i];
-        r = minmax4(r, make_float4(x, y, x, y));
-    }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        r = minmax4(r, bounds_item(xs[i], ys[i]));
     r = block_minmax4(r);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
@@ -60,11 +70,13 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const float4* __restric
                                                          BuildParams* __restrict__ prm,
                                                          uint32_t tree_mode, int use_caller,
                                                          double cminx, double cminy,
-                                                         double cmaxx, double cmaxy) {
+                                                         double cmaxx, double cmaxy,
+                                                         uint32_t* __restrict__ status) {
     float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
     for (uint32_t b = threadIdx.x; b < nb; b += kBlock) r = minmax4(r, partials[b]);
     r = block_minmax4(r);
     if (threadIdx.x != 0) return;
+    flag_nonfinite(r, status);
     uint64_t mnx64, mny64, mxx64, mxy64;
     if (use_caller) {
         mnx64 = bh_sfp_f64_bits(cminx);
@@ -87,10 +99,8 @@ __global__ __launch_bounds__(kBlock) void k_bounds_seg_partial(const float* __re
                                                                float4* __restrict__ partials) {
     const uint2 bk = blk[blockIdx.x];  // (first particle, count)
     float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
-    for (uint32_t i = threadIdx.x; i < bk.y; i += kBlock) {
-        const float x = xs[bk.x + i], y = ys[bk.x + i];
-        r = minmax4(r, make_float4(x, y, x, y));
-    }
+    for (uint32_t i = threadIdx.x; i < bk.y; i += kBlock)
+        r = minmax4(r, bounds_item(xs[bk.x + i], ys[bk.x + i]));
     r = block_minmax4(r);
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
@@ -100,13 +110,15 @@ __global__ __launch_bounds__(kBlock) void k_bounds_seg_final(const float4* __res
                                                              const uint32_t* __restrict__ blk_off,
                                                              const uint32_t* __restrict__ sys_off,
                                                              BuildParams* __restrict__ sys_prm,
-                                                             BuildParams* __restrict__ prm) {
+                                                             BuildParams* __restrict__ prm,
+                                                             uint32_t* __restrict__ status) {
     const uint32_t s = blockIdx.x;
     float4 r = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
     for (uint32_t b = blk_off[s] + threadIdx.x; b < blk_off[s + 1]; b += kBlock)
         r = minmax4(r, partials[b]);
     r = block_minmax4(r);
     if (threadIdx.x != 0) return;
+    flag_nonfinite(r, status);
     fill_params(sys_prm + s, r, bh_sfp_from_f32(r.x), bh_sfp_from_f32(r.y), bh_sfp_from_f32(r.z),
                 bh_sfp_from_f32(r.w), kTreeQuadTree, sys_off[s + 1] - sys_off[s]);
     if (s == 0) {
@@ -350,7 +362,6 @@ constexpr uint32_t kSortSmallMax = 1u << 18;  // sorts up to this size use the s
 __host__ __device__ constexpr int sort_tile(int items) { return kBlock * items; }
 constexpr int kSortMaxPasses = 8;
 constexpr uint32_t kLbCount = 1u << 30, kLbPrefix = 2u << 30, kLbMask = kLbCount - 1;
-constexpr uint32_t kLbSpinLimit = 1u << 22;
 constexpr int kLbWindow = 8;  // look-back loads in flight per digit
 static_assert(kBlock == 256, "one thread per digit");
 
@@ -419,8 +430,13 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(const K* __restrict__ ke
                                                        const uint32_t* __restrict__ totals,
                                                        uint32_t* __restrict__ status,
                                                        uint32_t* __restrict__ ticket,
-                                                       uint32_t* __restrict__ err) {
+                                                       uint32_t* __restrict__ err,
+                                                       uint32_t spin_limit) {
     constexpr int kTile = THREADS * ITEMS, kWaveKeys = kWave * ITEMS, kW = THREADS / kWave;
+    // After a give-up the keys of the next pass no longer match the digit totals, and placing
+    // them could write past the end: every later block leaves at once (the build reports the
+    // error and its gather falls back to the identity order).
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
     static_assert(THREADS >= 256, "one thread per digit");
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
@@ -508,7 +524,10 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(const K* __restrict__ ke
                 }
                 j -= used;
                 if (!done && used < kLbWindow) {
-                    if (++spins > kLbSpinLimit) {  // never expected: record it and stop waiting
+                    // never expected: record it and stop waiting. The tile's prefix is then
+                    // short of the true one, so this and every later tile still place their
+                    // keys inside [0, n)
+                    if (++spins > spin_limit) {
                         __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
@@ -556,10 +575,21 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ 
                                                    float* __restrict__ mo, float* __restrict__ vxo,
                                                    float* __restrict__ vyo,
                                                    uint32_t* __restrict__ ido,
-                                                   float4* __restrict__ pos4) {
+                                                   float4* __restrict__ pos4,
+                                                   const uint32_t* __restrict__ sort_err,
+                                                   void* __restrict__ codes, int wide) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t j = perm[i];
+    // a sort that gave up leaves no valid permutation: keep the input order and make every code
+    // 0 (sorted), so the tree build sees a well-formed input; the build reports the error
+    const bool failed = sort_err[0] != 0u;
+    const uint32_t j = failed ? i : min(perm[i], n - 1);
+    if (failed) {
+        if (wide)
+            static_cast<uint64_t*>(codes)[i] = 0ull;
+        else
+            static_cast<uint32_t*>(codes)[i] = 0u;
+    }
     const float px = x[j], py = y[j], pm = m[j];
     xo[i] = px;
     yo[i] = py;
@@ -580,7 +610,8 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
                                                         int B, int R, int cap, int fixed_depth,
                                                         uint8_t* __restrict__ leaf_lvl,
                                                         uint8_t* __restrict__ lcp1,
-                                                        uint32_t* __restrict__ cnt) {
+                                                        uint32_t* __restrict__ cnt,
+                                                        const uint32_t* __restrict__ status) {
     __shared__ K s_code[kBlock + 2 * kMaxLeafCap];
     __shared__ int8_t s_w[kBlock + kMaxLeafCap];
     const int64_t b0 = (int64_t)blockIdx.x * kBlock;
@@ -616,7 +647,10 @@ __global__ __launch_bounds__(kBlock) void k_tree_levels(const K* __restrict__ co
     const int lp = max((i == 0) ? -1 : lcp_level(codes[i - 1], codes[i], B), R - 1);
     leaf_lvl[i] = (uint8_t)leaf;
     lcp1[i] = (uint8_t)(lp + 1);
-    cnt[i] = (leaf > lp) ? (uint32_t)(leaf - lp) : 0u;
+    // a build whose sort gave up or whose input is not finite gets an empty tree: every walk
+    // over it ends at once (the call reports the error)
+    const bool failed = (status[0] | status[1]) != 0u;
+    cnt[i] = (!failed && leaf > lp) ? (uint32_t)(leaf - lp) : 0u;
 }
 
 // first j >= lo with codes[j] > hi_key (codes ascending), galloping then binary search
@@ -1139,9 +1173,12 @@ __global__ __launch_bounds__(kUpThreads) void k_up_top(Node* __restrict__ nodes,
 __global__ __launch_bounds__(kBlock) void k_batch_roots(const uint32_t* __restrict__ sys_off,
                                                         uint32_t nsys,
                                                         const uint32_t* __restrict__ noff,
+                                                        const BuildParams* __restrict__ prm,
                                                         uint32_t* __restrict__ roots) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < nsys) roots[s] = sys_off[s] < sys_off[s + 1] ? noff[sys_off[s]] : kNone;
+    if (s >= nsys) return;
+    const uint32_t r = sys_off[s] < sys_off[s + 1] ? noff[sys_off[s]] : kNone;
+    roots[s] = r < prm->n_nodes ? r : kNone;  // an empty tree (failed build) has no roots
 }
 
 // tree statistics on request: deepest leaf level and leaves at the depth limit holding more
@@ -1235,12 +1272,13 @@ uint32_t bounds_grid(uint32_t n) {
 }
 
 void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, float4* partials,
-                   BuildParams* prm, uint32_t tree_mode, const double* caller) {
+                   BuildParams* prm, uint32_t tree_mode, const double* caller, uint32_t* status) {
     const uint32_t g = bounds_grid(n);
     k_bounds_partial<<<g, kBlock, 0, s>>>(xs, ys, n, partials);
     k_bounds_final<<<1, kBlock, 0, s>>>(partials, g, n, prm, tree_mode, caller ? 1 : 0,
                                         caller ? caller[0] : 0.0, caller ? caller[1] : 0.0,
-                                        caller ? caller[2] : 0.0, caller ? caller[3] : 0.0);
+                                        caller ? caller[2] : 0.0, caller ? caller[3] : 0.0,
+                                        status);
 }
 
 void launch_morton_gpuref(hipStream_t s, const float* xs, const float* ys, uint32_t n,
@@ -1280,14 +1318,13 @@ uint64_t sort_scratch_words(uint32_t n) {
 
 void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys_b,
                        uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* scratch,
-                       bool* result_in_b) {
+                       uint32_t* err, uint32_t spin_limit, bool* result_in_b) {
     const uint32_t nt = sort_blocks(n);
     const int passes = (key_bits + 7) / 8;
     uint32_t* counts = scratch;
     uint32_t* status = counts + (size_t)kSortMaxPasses * 256 * nt;
     uint32_t* totals = status + (size_t)kSortMaxPasses * 256 * nt;
     uint32_t* tickets = totals + kSortMaxPasses * 256;
-    uint32_t* err = tickets + kSortMaxPasses;
     const bool small = sort_items(n) == kSortItemsSmall;
 #define BH_COUNTS(K, T, I)                                                                        \
     k_radix_counts<K, T, I><<<nt, T, 0, s>>>((const K*)keys_a, n, passes, nt, counts, status)
@@ -1308,7 +1345,7 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
         const uint32_t* tp = totals + p * 256;
 #define BH_PASS(K, T, I)                                                                          \
     k_radix_pass<K, T, I><<<nt, T, 0, s>>>((const K*)kin, vin, (K*)kout, vout, n, 8 * p, tp, st,  \
-                                           tickets + p, err)
+                                           tickets + p, err, spin_limit)
         static_assert(kSortItemsLarge * kBlock == 1024 * 4 && kSortItemsSmall * kBlock == 512 * 4,
                       "pass tiles match the counting tiles");
         if (wide) {
@@ -1325,23 +1362,24 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
 
 void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float* x, const float* y,
                    const float* m, const float* vx, const float* vy, const uint32_t* id, float* xo,
-                   float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4) {
+                   float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4,
+                   const uint32_t* sort_err, void* codes, bool wide) {
     k_gather<<<div_up(n, kBlock), kBlock, 0, s>>>(perm, n, x, y, m, vx, vy, id, xo, yo, mo, vxo,
-                                                  vyo, ido, pos4);
+                                                  vyo, ido, pos4, sort_err, codes, wide ? 1 : 0);
 }
 
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
                  Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes, int R,
-                 const BuildParams* sys_prm) {
+                 const BuildParams* sys_prm, const uint32_t* status) {
     const uint32_t g = div_up(n, kBlock);
     if (wide)
         k_tree_levels<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)codes, n, B, R, cap,
-                                                     fixed_depth, leaf_lvl, lcp1, cnt);
+                                                     fixed_depth, leaf_lvl, lcp1, cnt, status);
     else
         k_tree_levels<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)codes, n, B, R, cap,
-                                                     fixed_depth, leaf_lvl, lcp1, cnt);
+                                                     fixed_depth, leaf_lvl, lcp1, cnt, status);
     launch_exclusive_scan(s, cnt, noff, n, scan_partials, &prm->n_nodes);
     k_tree_emit_heads<<<g, kBlock, 0, s>>>(n, lcp1, cnt, noff, nodes, (uint4*)child4, cap_nodes);
     const uint32_t gn = div_up(cap_nodes, kBlock);
@@ -1358,9 +1396,9 @@ void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B,
 void launch_bounds_batched(hipStream_t s, const float* xs, const float* ys, const uint2* blk,
                            uint32_t nblk, const uint32_t* blk_off, const uint32_t* sys_off,
                            uint32_t nsys, float4* partials, BuildParams* sys_prm,
-                           BuildParams* prm) {
+                           BuildParams* prm, uint32_t* status) {
     k_bounds_seg_partial<<<nblk, kBlock, 0, s>>>(xs, ys, blk, partials);
-    k_bounds_seg_final<<<nsys, kBlock, 0, s>>>(partials, blk_off, sys_off, sys_prm, prm);
+    k_bounds_seg_final<<<nsys, kBlock, 0, s>>>(partials, blk_off, sys_off, sys_prm, prm, status);
 }
 
 void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint32_t n,
@@ -1371,8 +1409,8 @@ void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint
 }
 
 void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
-                        const uint32_t* noff, uint32_t* roots) {
-    k_batch_roots<<<div_up(nsys, kBlock), kBlock, 0, s>>>(sys_off, nsys, noff, roots);
+                        const uint32_t* noff, const BuildParams* prm, uint32_t* roots) {
+    k_batch_roots<<<div_up(nsys, kBlock), kBlock, 0, s>>>(sys_off, nsys, noff, prm, roots);
 }
 
 uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }

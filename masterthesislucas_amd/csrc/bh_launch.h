@@ -16,8 +16,12 @@ constexpr int kKernelGravity = 0;
 constexpr int kKernelTsne = 1;
 
 uint32_t bounds_grid(uint32_t n);
+// Build status words (device, cleared by the host before a build sequence, read after it):
+// [0] the radix sort's look-back gave up waiting, [1] a position was not finite.
+constexpr int kStatusWords = 4;
+constexpr uint32_t kSortSpinLimit = 1u << 22;  // look-back polls before giving up
 void launch_bounds(hipStream_t s, const float* xs, const float* ys, uint32_t n, float4* partials,
-                   BuildParams* prm, uint32_t tree_mode, const double* caller);
+                   BuildParams* prm, uint32_t tree_mode, const double* caller, uint32_t* status);
 // keys of the reference GPU step's tree (32-bit, path in the top 2 depth bits)
 void launch_morton_gpuref(hipStream_t s, const float* xs, const float* ys, uint32_t n,
                           const BuildParams* prm, int depth, uint32_t* keys, uint32_t* vals);
@@ -28,27 +32,31 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
                            uint32_t* partials, uint32_t* total);
 uint32_t sort_blocks(uint32_t n);
 uint64_t sort_scratch_words(uint32_t n);  // uint32 words of scratch launch_radix_sort needs
+// err: status word set when a look-back wait exceeds spin_limit polls (the order is then not
+// valid; later passes do nothing and launch_gather falls back to the input order)
 void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys_b,
                        uint32_t* vals_b, uint32_t n, int key_bits, bool wide, uint32_t* scratch,
-                       bool* result_in_b);
+                       uint32_t* err, uint32_t spin_limit, bool* result_in_b);
+// codes: the sorted codes, zeroed (with the identity order) when *sort_err is set
 void launch_gather(hipStream_t s, const uint32_t* perm, uint32_t n, const float* x, const float* y,
                    const float* m, const float* vx, const float* vy, const uint32_t* id, float* xo,
-                   float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4);
+                   float* yo, float* mo, float* vxo, float* vyo, uint32_t* ido, float4* pos4,
+                   const uint32_t* sort_err, void* codes, bool wide);
 // R: level of the tree roots (0; batched systems: the system id occupies the levels above R)
 void launch_tree(hipStream_t s, const void* codes, bool wide, uint32_t n, int B, int cap,
                  int fixed_depth, int single_child_pp, uint8_t* leaf_lvl, uint8_t* lcp1,
                  uint32_t* cnt, uint32_t* noff, uint32_t* scan_partials, BuildParams* prm,
                  Node* nodes, uint32_t* parent, uint32_t* child4, uint32_t cap_nodes, int R,
-                 const BuildParams* sys_prm);
+                 const BuildParams* sys_prm, const uint32_t* status);
 void launch_bounds_batched(hipStream_t s, const float* xs, const float* ys, const uint2* blk,
                            uint32_t nblk, const uint32_t* blk_off, const uint32_t* sys_off,
                            uint32_t nsys, float4* partials, BuildParams* sys_prm,
-                           BuildParams* prm);
+                           BuildParams* prm, uint32_t* status);
 void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint32_t n,
                            const uint32_t* sys_off, uint32_t nsys, const BuildParams* sys_prm,
                            int B, uint64_t* keys, uint32_t* vals);
 void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
-                        const uint32_t* noff, uint32_t* roots);
+                        const uint32_t* noff, const BuildParams* prm, uint32_t* roots);
 uint32_t up_chunks(uint32_t cap_nodes);
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
                    float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes,

@@ -232,6 +232,11 @@ class BHContext:
         """Traversal variant in use (the library starts with 4: two sinks per lane)."""
         return getattr(self, "_variant", 4)
 
+    def set_sort_spin_limit(self, polls):
+        """Look-back polls before the radix sort gives up (default 2^22); a small value only
+        exercises the error path (bh_set_sort_spin_limit)."""
+        self._c(_lib.load().bh_set_sort_spin_limit(self._h, int(polls)))
+
     SOLVERS = {"bh": 0, "fmm_sym": 1}
 
     def set_solver(self, solver):

@@ -544,6 +544,148 @@ void orc_bh_traverse(const double* xs, const double* ys, const double* ms, int64
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Barnes-Hut walk with single-precision acceptance decisions on given node records            */
+/* ------------------------------------------------------------------------------------------ */
+/* The walk of orc_bh_traverse (nBodySolverBH.h:56-89 control flow: accept -> monopole and skip
+ * the subtree, else a near-field cell -> particle-particle over its range, else descend), with
+ * the acceptance taken as the single-precision test d2 > l2 * (1/theta^2):
+ *   l = root_side * 2^-level (exact), l2 = RN32(l * l), lim = RN32(l2 * inv_theta2),
+ *   dx = RN32(cx - px), dy = RN32(cy - py), d2 = fma32(dx, dx, RN32(dy * dy)).
+ * Every interaction itself is evaluated in FP64 from the same single-precision inputs, so the
+ * result is the exact sum of the interactions the single-precision decisions select; abs_x/abs_y
+ * receive the per-sink sums of |contribution| that bound the rounding of any evaluation order.
+ * With FP64 node moments given (cx64 != NULL), each decision on the walk is also taken by the
+ * reference's FP64 rule of `semantics` (QUADTREE: l/|d| < theta, nBodySolverBH.h:58-62; MORTON:
+ * d^2 > (l/theta)^2, nBodySolverFMM_SYM_MORTON.h:276-280) and disagreements are counted. */
+typedef struct {
+    const float *xs, *ys, *ms;
+    const int32_t* level;
+    const uint32_t *first, *count, *skip;
+    const uint8_t* pp;
+    const float *cx, *cy, *mass;
+    int64_t nnodes;
+    float root_side, inv_theta2;
+    const double *cx64, *cy64;
+    double side64, theta;
+    int semantics, kernel;
+    double G, eps2;
+    const int64_t* sinks;
+    double *ax, *ay, *tot, *abs_x, *abs_y;
+    int64_t lo, hi;
+    uint64_t pn, ppc, mism;
+} trav32_job;
+
+static void* trav32_worker(void* arg) {
+    trav32_job* j = (trav32_job*)arg;
+    uint64_t pn = 0, ppc = 0, mism = 0;
+    for (int64_t s = j->lo; s < j->hi; ++s) {
+        const int64_t i = j->sinks ? j->sinks[s] : s;
+        const float px = j->xs[i], py = j->ys[i];
+        double ax = 0, ay = 0, tot = 0, bx = 0, by = 0;
+        int64_t k = 0;
+        while (k < j->nnodes) {
+            const float sl = ldexpf(j->root_side, -j->level[k]);
+            const float l2 = sl * sl;
+            const float lim = l2 * j->inv_theta2;
+            const float dx = j->cx[k] - px, dy = j->cy[k] - py;
+            const float d2 = fmaf(dx, dx, dy * dy);
+            const int accept = d2 > lim;
+            if (j->cx64) {
+                const double l = ldexp(j->side64, -j->level[k]);
+                const double ex = (double)px - j->cx64[k], ey = (double)py - j->cy64[k];
+                const double e2 = ex * ex + ey * ey;
+                int acc64;
+                if (j->semantics == ORC_SEM_QUADTREE) {
+                    acc64 = (l / sqrt(e2)) < j->theta;
+                } else {
+                    const double thr = l / j->theta;
+                    acc64 = e2 > thr * thr;
+                }
+                mism += (uint64_t)(acc64 != accept);
+            }
+            if (accept) {
+                double cxv = 0, cyv = 0, ctv = 0;
+                interact(j->kernel, j->G, j->eps2, px, py, j->cx[k], j->cy[k], j->mass[k], &cxv,
+                         &cyv, &ctv);
+                ax += cxv; ay += cyv; tot += ctv;
+                bx += fabs(cxv); by += fabs(cyv);
+                ++pn;
+                k = j->skip[k];
+            } else if (j->pp[k]) {
+                const int64_t f = j->first[k], e = f + j->count[k];
+                for (int64_t q = f; q < e; ++q) {
+                    if (q == i) continue;
+                    double cxv = 0, cyv = 0, ctv = 0;
+                    interact(j->kernel, j->G, j->eps2, px, py, j->xs[q], j->ys[q],
+                             j->ms ? j->ms[q] : 1.0, &cxv, &cyv, &ctv);
+                    ax += cxv; ay += cyv; tot += ctv;
+                    bx += fabs(cxv); by += fabs(cyv);
+                    ++ppc;
+                }
+                k = j->skip[k];
+            } else {
+                ++k;
+            }
+        }
+        j->ax[s] = ax;
+        j->ay[s] = ay;
+        if (j->tot) j->tot[s] = tot;
+        if (j->abs_x) j->abs_x[s] = bx;
+        if (j->abs_y) j->abs_y[s] = by;
+    }
+    j->pn = pn;
+    j->ppc = ppc;
+    j->mism = mism;
+    return NULL;
+}
+
+void orc_bh_traverse_f32(const float* xs, const float* ys, const float* ms, int64_t n,
+                         const int32_t* level, const uint32_t* first, const uint32_t* count,
+                         const uint32_t* skip, const uint8_t* pp, const float* cx, const float* cy,
+                         const float* mass, int64_t nnodes, float root_side, float inv_theta2,
+                         int kernel, double G, double eps2, const double* cx64, const double* cy64,
+                         double side64, double theta, int semantics, const int64_t* sinks,
+                         int64_t nsinks, double* out_ax, double* out_ay, double* out_tot,
+                         double* abs_x, double* abs_y, uint64_t* counters, int nthreads) {
+    if (!sinks) nsinks = n;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    trav32_job jobs[256];
+    pthread_t th[256];
+    const int64_t chunk = (nsinks + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        trav32_job* j = &jobs[t];
+        memset(j, 0, sizeof(*j));
+        j->xs = xs; j->ys = ys; j->ms = ms;
+        j->level = level; j->first = first; j->count = count; j->skip = skip; j->pp = pp;
+        j->cx = cx; j->cy = cy; j->mass = mass; j->nnodes = nnodes;
+        j->root_side = root_side; j->inv_theta2 = inv_theta2;
+        j->cx64 = cx64; j->cy64 = cy64; j->side64 = side64; j->theta = theta;
+        j->semantics = semantics; j->kernel = kernel; j->G = G; j->eps2 = eps2;
+        j->sinks = sinks; j->ax = out_ax; j->ay = out_ay; j->tot = out_tot;
+        j->abs_x = abs_x; j->abs_y = abs_y;
+        j->lo = t * chunk;
+        j->hi = (t + 1) * chunk < nsinks ? (t + 1) * chunk : nsinks;
+        if (j->lo > j->hi) j->lo = j->hi;
+    }
+    (void)n;
+    if (nthreads == 1) {
+        trav32_worker(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, trav32_worker, &jobs[t]);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    }
+    if (counters) {
+        counters[0] = counters[1] = counters[2] = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            counters[0] += jobs[t].pn;
+            counters[1] += jobs[t].ppc;
+            counters[2] += jobs[t].mism;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* FP64 compensated direct sum                                                                  */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct {

@@ -95,6 +95,23 @@ void orc_bh_traverse(const double* xs, const double* ys, const double* ms, int64
                      int64_t nsinks, double* out_ax, double* out_ay, double* out_tot,
                      uint64_t* counters /* [pn, pp] */, int nthreads);
 
+/* ---- the same walk with single-precision acceptance decisions on given node records ----
+ * Positions, masses and node records (cx, cy, mass) in single precision as the GPU holds them;
+ * node side l = root_side * 2^-level, decision d2 > l2 * inv_theta2 with
+ * l2 = RN32(l*l), d2 = fma32(dx, dx, RN32(dy*dy)), dx = RN32(cx - px): the expression of the
+ * GPU traversal. Interactions are summed in FP64; abs_x/abs_y (may be NULL) receive per-sink
+ * sums of |contribution|. counters = [pn, pp, mismatches]: when cx64/cy64 are given, the
+ * decisions on the walk that the reference's FP64 rule of `semantics` (with side64, theta)
+ * would take the other way. */
+void orc_bh_traverse_f32(const float* xs, const float* ys, const float* ms, int64_t n,
+                         const int32_t* level, const uint32_t* first, const uint32_t* count,
+                         const uint32_t* skip, const uint8_t* pp, const float* cx, const float* cy,
+                         const float* mass, int64_t nnodes, float root_side, float inv_theta2,
+                         int kernel, double G, double eps2, const double* cx64, const double* cy64,
+                         double side64, double theta, int semantics, const int64_t* sinks,
+                         int64_t nsinks, double* out_ax, double* out_ay, double* out_tot,
+                         double* abs_x, double* abs_y, uint64_t* counters, int nthreads);
+
 /* ---- FP64 compensated direct sum (nBodySolverNaive.h:53-140), self excluded by index ---- */
 void orc_direct(const double* x, const double* y, const double* m, int64_t n, int kernel, double G,
                 double eps2, const int64_t* sinks, int64_t nsinks, double* out_ax, double* out_ay,

@@ -57,6 +57,11 @@ def lib():
             "orc_bh_traverse": (None, [_vp, _vp, _vp, I64] + [_vp] * 8 + [I64, D, I32, I32, D, D, D,
                                                                           _vp, I64, _vp, _vp, _vp,
                                                                           _vp, I32]),
+            "orc_bh_traverse_f32": (None, [_vp, _vp, _vp, I64] + [_vp] * 8 + [I64, C.c_float,
+                                                                          C.c_float, I32, D, D,
+                                                                          _vp, _vp, D, D, I32,
+                                                                          _vp, I64] + [_vp] * 6
+                                    + [I32]),
             "orc_direct": (None, [_vp, _vp, _vp, I64, I32, D, D, _vp, I64, _vp, _vp, _vp, I32]),
             "orc_quadtree_build": (_vp, [_vp, _vp, I64, I32]),
             "orc_quadtree_free": (None, [_vp]),
@@ -246,6 +251,44 @@ def bh_traverse(xs, ys, ms, tree, mom, root_side, semantics, kernel, G=1.0, eps2
                           int(semantics), int(kernel), float(G), float(eps2), float(theta), _p(sk),
                           ns, _p(ax), _p(ay), _p(tot), _p(cnt), int(nthreads))
     return ax, ay, tot, cnt
+
+
+def bh_traverse_f32(xs, ys, ms, tree, root_side, inv_theta2, kernel, G=1.0, eps2=0.0, sinks=None,
+                    mom64=None, side64=0.0, theta=0.5, semantics=SEM_QUADTREE, nthreads=8):
+    """Walk over given single-precision node records (tree: level, first, count, skip, pp, cx,
+    cy, mass) with the single-precision acceptance expression d2 > l2 * inv_theta2; FP64
+    interaction sums. Returns ax, ay, tot, abs_x, abs_y (indexed like sinks) and the counters
+    [pn, pp, mismatches against the FP64 rule evaluated with mom64 (cx, cy) and side64]."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    ms = None if ms is None else np.ascontiguousarray(ms, np.float32)
+    lev = np.ascontiguousarray(tree["level"], np.int32)
+    fi = np.ascontiguousarray(tree["first"], np.uint32)
+    co = np.ascontiguousarray(tree["count"], np.uint32)
+    sk_ = np.ascontiguousarray(tree["skip"], np.uint32)
+    pp = np.ascontiguousarray(tree["pp"], np.uint8)
+    cx = np.ascontiguousarray(tree["cx"], np.float32)
+    cy = np.ascontiguousarray(tree["cy"], np.float32)
+    mass = np.ascontiguousarray(tree["mass"], np.float32)
+    ns = xs.size if sinks is None else len(sinks)
+    sk = None if sinks is None else np.ascontiguousarray(sinks, dtype=np.int64)
+    ax, ay, tot, bx, by = (np.empty(ns) for _ in range(5))
+    cnt = np.zeros(3, np.uint64)
+    c64x = None if mom64 is None else _f64(mom64["cx"])
+    c64y = None if mom64 is None else _f64(mom64["cy"])
+    lib().orc_bh_traverse_f32(_p(xs), _p(ys), _p(ms), xs.size, _p(lev), _p(fi), _p(co), _p(sk_),
+                              _p(pp), _p(cx), _p(cy), _p(mass), lev.size, float(np.float32(root_side)),
+                              float(np.float32(inv_theta2)), int(kernel), float(G), float(eps2),
+                              _p(c64x), _p(c64y), float(side64), float(theta), int(semantics),
+                              _p(sk), ns, _p(ax), _p(ay), _p(tot), _p(bx), _p(by), _p(cnt),
+                              int(nthreads))
+    return ax, ay, tot, bx, by, cnt
+
+
+def inv_theta2_f32(theta):
+    """1/theta^2 as the traversal receives it: computed in FP64, rounded to single precision."""
+    with np.errstate(divide="ignore"):
+        return np.float32(1.0 / (float(theta) * float(theta))) if theta > 0 else np.float32(np.inf)
 
 
 def direct(x, y, m, kernel, G=1.0, eps2=0.0, sinks=None, nthreads=8):

@@ -60,3 +60,76 @@ def rel_err(ax, ay, bx, by):
     num = np.hypot(np.asarray(ax, np.float64) - bx, np.asarray(ay, np.float64) - by)
     den = np.hypot(bx, by)
     return num / np.where(den > 0, den, 1.0)
+
+
+def exact_walk(ctx, x32, y32, m32, kernel, theta, eps=0.0, G=1.0, sinks=None, t64=None,
+               nthreads=8):
+    """The oracle's walk over the GPU's own exported node records with the traversal's
+    single-precision acceptance expression (orc_bh_traverse_f32), for comparison with the
+    forces ctx computed at `theta` on its last build.
+
+    Returns a dict in the caller's particle order (or per caller-order sink): ax, ay, tot (FP64
+    sums of the selected interactions), abs_x, abs_y (sums of |contribution|), and pn, pp (the
+    interaction counts of those sinks) and, when t64 (an oracle_tree of the same input) is
+    given, `mismatch`: the decisions on the walk that the reference's FP64 rule would take the
+    other way."""
+    _, perm = ctx.morton()
+    g = ctx.tree()
+    _, side = ctx.bounds()
+    n = perm.size
+    x32 = np.asarray(x32, np.float32)
+    y32 = np.asarray(y32, np.float32)
+    m = np.ones(n, np.float32) if m32 is None else np.asarray(m32, np.float32)
+    tree = dict(level=g["level"], first=g["first"], count=g["count"], skip=g["skip"],
+                pp=((g["flags"] >> 1) & 1).astype(np.uint8), cx=g["cx"], cy=g["cy"],
+                mass=g["mass"])
+    where = np.empty(n, np.int64)
+    where[perm] = np.arange(n)
+    ss = None if sinks is None else where[np.asarray(sinks, np.int64)]
+    e32 = np.float32(eps)
+    eps2 = float(e32 * e32)
+    sem = O.SEM_QUADTREE if ctx.tree_mode == "quadtree" else O.SEM_MORTON
+    mom64 = None if t64 is None else t64["mom"]
+    ax, ay, tot, bx, by, cnt = O.bh_traverse_f32(
+        x32[perm], y32[perm], m[perm], tree, np.float32(side), O.inv_theta2_f32(theta),
+        O.KERNEL_GRAVITY if kernel == "gravity" else O.KERNEL_TSNE, G=G, eps2=eps2, sinks=ss,
+        mom64=mom64, side64=side if t64 is None else t64["side"], theta=theta, semantics=sem,
+        nthreads=nthreads)
+    out = dict(pn=int(cnt[0]), pp=int(cnt[1]), mismatch=int(cnt[2]) if t64 is not None else None)
+    if sinks is None:
+        for k, v in (("ax", ax), ("ay", ay), ("tot", tot), ("abs_x", bx), ("abs_y", by)):
+            a = np.empty(n)
+            a[perm] = v
+            out[k] = a
+    else:
+        out.update(ax=ax, ay=ay, tot=tot, abs_x=bx, abs_y=by)
+    return out
+
+
+def walk_error(ax, ay, w):
+    """Per-particle error of GPU forces against exact_walk's sums, each component normalised by
+    the sum of the absolute values of its contributions (a bound that does not grow where the
+    contributions cancel)."""
+    ex = np.abs(np.asarray(ax, np.float64) - w["ax"]) / np.where(w["abs_x"] > 0, w["abs_x"], 1.0)
+    ey = np.abs(np.asarray(ay, np.float64) - w["ay"]) / np.where(w["abs_y"] > 0, w["abs_y"], 1.0)
+    return np.maximum(ex, ey)
+
+
+# the bar of the exact-walk comparison: FP32 interactions with compensated accumulation against
+# FP64 sums of the same interactions, per component relative to the sum of |contributions|
+WALK_TOL = 1e-5
+
+
+def check_exact_walk(ctx, x, y, m, kernel, theta, eps, ax, ay, tot, t64=None):
+    """GPU forces and counters against the oracle's walk over the GPU's own node records with
+    the traversal's single-precision acceptance: equal particle-node and particle-particle
+    counts, every particle's error within WALK_TOL of its |contribution| sums, the total within
+    the same bar. Returns the oracle walk (with the FP64-rule disagreement count when t64 is
+    given)."""
+    w = exact_walk(ctx, x, y, m, kernel, theta, eps=eps, t64=t64)
+    pn, pp = ctx.counters()
+    assert (int(pn), int(pp)) == (w["pn"], w["pp"]), ((pn, pp), (w["pn"], w["pp"]))
+    e = walk_error(ax, ay, w)
+    assert e.max() <= WALK_TOL, (float(e.max()), int(np.argmax(e)))
+    assert abs(tot - w["tot"].sum()) <= WALK_TOL * np.abs(w["tot"]).sum()
+    return w

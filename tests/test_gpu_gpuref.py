@@ -1,8 +1,9 @@
 """The reference GPU step's tree and iteration (NBodySolverGpuBH + TsneGpu,
 nBodySolverGpuBH.cu:78-460, tsneGpu.h:145) as BH_TREE_GPU_REF and TsneGpu, against the oracle:
 partition paths (orc_gpuref_codes32, pinned in test_oracle.py by a per-point descent), sort
-order and topology bit-exact; forces on the same tree within the FP32 bar of the other modes
-(median 1e-5, p99 1e-4 relative); one TsneGpu iteration against the oracle's repulsion on that
+order and topology bit-exact; forces exactly against the oracle's single-precision-decision
+walk over the GPU's node records (equal counts, WALK_TOL per particle) and within median 1e-5,
+p99 1e-4 of the FP64 oracle on its own tree; one TsneGpu iteration against the oracle's repulsion on that
 tree scaled by 1/Z, the attraction without exaggeration and the update with learning rate 1000
 and momentum 0.2."""
 import numpy as np
@@ -11,7 +12,7 @@ import pytest
 from masterthesislucas_amd import BHContext, tsne
 from masterthesislucas_amd.workloads import plummer, uniform_disk
 from oracle import oracle as O
-from tests.helpers import oracle_forces, oracle_tree, rel_err
+from tests.helpers import check_exact_walk, oracle_forces, oracle_tree, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -56,8 +57,9 @@ def test_gpuref_forces_match_oracle(gpu, kernel, theta):
     ctx.set_particles(x, y, m)
     ctx.build()
     ax, ay, tot = ctx.forces(theta)
-    ctx.close()
     t = oracle_tree(x, y, m, mode="gpu_ref")
+    check_exact_walk(ctx, x, y, m, kernel, theta, eps, ax, ay, tot)
+    ctx.close()
     bx, by, bt, _ = oracle_forces(t, O.KERNEL_GRAVITY if grav else O.KERNEL_TSNE, theta,
                                   mode="gpu_ref", eps=eps)
     r = rel_err(ax, ay, bx, by)

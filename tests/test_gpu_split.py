@@ -13,14 +13,14 @@ from masterthesislucas_amd import BHContext, tsne
 from masterthesislucas_amd._lib import BHError
 from masterthesislucas_amd.workloads import plummer, two_plummer_disks, uniform_disk
 from oracle import oracle as O
-from tests.helpers import oracle_forces, oracle_tree, rel_err
+from tests.helpers import check_exact_walk, oracle_forces, oracle_tree, rel_err
 
 pytestmark = pytest.mark.gpu
 
 EPS = float(np.float32(1e-3))
 
 
-def _forces(p, kernel, variant, split, theta, m=True):
+def _forces(p, kernel, variant, split, theta, m=True, exact=False):
     ctx = BHContext(p.n, kernel=kernel, tree_mode="quadtree", softening=EPS)
     ctx.set_particles(p.x, p.y, p.m if m else None)
     ctx.set_variant(variant)
@@ -28,6 +28,9 @@ def _forces(p, kernel, variant, split, theta, m=True):
     ctx.build()
     ax, ay, tot = ctx.forces(theta)
     cnt = ctx.counters()
+    if exact:
+        check_exact_walk(ctx, p.x, p.y, p.m if m else None, kernel, theta,
+                         EPS if kernel == "gravity" else 0.0, ax, ay, tot)
     ctx.close()
     return ax, ay, tot, cnt
 
@@ -48,7 +51,7 @@ def test_split_equals_one_wave_walk(gpu, variant, split):
 def test_split_matches_oracle(gpu, kernel):
     p = uniform_disk(20000, seed=62)
     grav = kernel == "gravity"
-    ax, ay, tot, (pn, pp) = _forces(p, kernel, 4, 4, 0.5, m=grav)
+    ax, ay, tot, (pn, pp) = _forces(p, kernel, 4, 4, 0.5, m=grav, exact=True)
     t = oracle_tree(p.x, p.y, p.m if grav else None, mode="quadtree")
     okern = O.KERNEL_GRAVITY if grav else O.KERNEL_TSNE
     bx, by, bt, cnt = oracle_forces(t, okern, 0.5, eps=EPS if grav else 0.0)

@@ -567,3 +567,120 @@ def test_gpuref_partition_equals_per_point_descent(depth):
     x[200:220] = x[0]
     y[200:220] = y[0]
     assert np.array_equal(O.gpuref_codes32(x, y, depth), _gpuref_descent(x, y, depth))
+
+
+# ---------------------------------------------------------------------------------------------
+# the single-precision-decision walk (orc_bh_traverse_f32) and the acceptance thresholds
+# ---------------------------------------------------------------------------------------------
+def _hand_tree(com_x, com_x64=None):
+    """Pre-order tree by hand: root (level 0, side 4) over particles 0..2; node 1 = particle 0
+    (the sink at the origin) alone; node 2 = particles 1, 2 (mass 1 each) with its centre of
+    mass at (com_x, 0). The level-1 side is 2, so at theta = 0.5 node 2 sits on the acceptance
+    threshold l/theta = 4 when com_x = 4."""
+    xs = np.array([0.0, com_x - 0.5, com_x + 0.5], np.float32)
+    ys = np.zeros(3, np.float32)
+    tree = dict(level=np.array([0, 1, 1], np.int32), first=np.array([0, 0, 1], np.uint32),
+                count=np.array([3, 1, 2], np.uint32), skip=np.array([3, 2, 3], np.uint32),
+                pp=np.array([0, 1, 1], np.uint8),
+                cx=np.array([np.float32(com_x) * np.float32(2 / 3), 0.0, com_x], np.float32),
+                cy=np.zeros(3, np.float32), mass=np.array([3, 1, 2], np.float32))
+    mom64 = dict(cx=np.array([2 * (com_x64 or com_x) / 3, 0.0, com_x64 or com_x]),
+                 cy=np.zeros(3), mass=np.array([3.0, 1.0, 2.0]))
+    return xs, ys, tree, mom64
+
+
+@pytest.mark.parametrize("semantics", [O.SEM_QUADTREE, O.SEM_MORTON])
+def test_acceptance_threshold_is_strict(semantics):
+    """A cell exactly at l/theta is opened and one ulp beyond it is accepted, by the FP64 rules
+    (QuadTree l/|d| < theta, Morton d^2 > (l/theta)^2) and by the single-precision
+    d2 > l2/theta^2 expression; the level's side (not the parent's) is l. A '>=' or '<=' slip,
+    or the parent's side, fails one of the cases."""
+    for com, want_pn in ((4.0, 0), (float(np.nextafter(np.float32(4.0), np.float32(5))), 1)):
+        xs, ys, tree, mom = _hand_tree(com)
+        ax, ay, _, cnt = O.bh_traverse(xs, ys, None, tree, mom, 4.0, semantics, O.KERNEL_GRAVITY,
+                                       theta=0.5, sinks=[0], pp=tree["pp"], nthreads=1)
+        assert (int(cnt[0]), int(cnt[1])) == (want_pn, 2 * (1 - want_pn)), (com, cnt)
+        fx, _, _, _, _, c3 = O.bh_traverse_f32(xs, ys, None, tree, 4.0, O.inv_theta2_f32(0.5),
+                                               O.KERNEL_GRAVITY, sinks=[0], mom64=mom, side64=4.0,
+                                               theta=0.5, semantics=semantics, nthreads=1)
+        assert (int(c3[0]), int(c3[1]), int(c3[2])) == (want_pn, 2 * (1 - want_pn), 0)
+        assert fx[0] == pytest.approx(ax[0], rel=1e-6)
+        if want_pn:  # one monopole of mass 2 at distance com
+            assert fx[0] == pytest.approx(2.0 / com ** 2, rel=1e-14)
+        else:  # two particles at com -+ 0.5
+            assert fx[0] == pytest.approx(1 / (com - 0.5) ** 2 + 1 / (com + 0.5) ** 2, rel=1e-6)
+
+
+def test_single_precision_walk_counts_fp64_disagreements():
+    """FP64 centre of mass just beyond the threshold, its single-precision value on it: the FP64
+    rule accepts, the single-precision expression does not, and the walk counts one
+    disagreement."""
+    xs, ys, tree, mom = _hand_tree(4.0, com_x64=4.0 + 1e-12)
+    _, _, _, _, _, c3 = O.bh_traverse_f32(xs, ys, None, tree, 4.0, O.inv_theta2_f32(0.5),
+                                          O.KERNEL_GRAVITY, sinks=[0], mom64=mom, side64=4.0,
+                                          theta=0.5, semantics=O.SEM_MORTON, nthreads=1)
+    assert list(map(int, c3)) == [0, 2, 1]
+
+
+def _f32_records(t):
+    tr = dict(t["tree"])
+    tr["pp"] = t["pp"]
+    for k in ("cx", "cy", "mass"):
+        tr[k] = t["mom"][k].astype(np.float32)
+    return tr
+
+
+@pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
+def test_single_precision_walk_theta_zero_and_fp64_walk(mode):
+    """theta = 0 (1/theta^2 = inf) accepts nothing: the direct sum. At theta > 0 with no
+    disagreement on the walk it selects exactly the FP64 walk's interactions, so the sums agree
+    to FP64 rounding; |contribution| sums bound |sum|."""
+    rng = np.random.default_rng(14)
+    x = rng.normal(0, 1, 4000).astype(np.float32)
+    y = rng.normal(0, 0.7, 4000).astype(np.float32)
+    t = oracle_tree(x, y, mode=mode)
+    tr = _f32_records(t)
+    xs, ys = t["xs"].astype(np.float32), t["ys"].astype(np.float32)
+    sem = O.SEM_QUADTREE if mode == "quadtree" else O.SEM_MORTON
+    eps2 = float(np.float32(1e-3) * np.float32(1e-3))
+    ax, ay, tot, bx, by, c3 = O.bh_traverse_f32(xs, ys, None, tr, np.float32(t["side"]),
+                                                O.inv_theta2_f32(0.0), O.KERNEL_GRAVITY,
+                                                eps2=eps2)
+    assert c3[0] == 0 and c3[1] == 4000 * 3999
+    dx, dy, _ = O.direct(xs, ys, None, O.KERNEL_GRAVITY, 1.0, eps2)
+    assert rel_err(ax, ay, dx, dy).max() < 1e-12
+    compared = 0
+    for kernel in (O.KERNEL_GRAVITY, O.KERNEL_TSNE):
+        for theta in (0.4, 0.8):
+            mom32 = dict(cx=tr["cx"].astype(np.float64), cy=tr["cy"].astype(np.float64),
+                         mass=tr["mass"].astype(np.float64))
+            ax, ay, tot, bx, by, c3 = O.bh_traverse_f32(
+                xs, ys, None, tr, np.float32(t["side"]), O.inv_theta2_f32(theta), kernel,
+                eps2=eps2, mom64=mom32, side64=float(np.float32(t["side"])), theta=theta,
+                semantics=sem)
+            if c3[2]:
+                continue  # a decision on the threshold: the walks differ there by design
+            fx, fy, ft, cnt = O.bh_traverse(xs, ys, None, t["tree"], mom32,
+                                            float(np.float32(t["side"])), sem, kernel,
+                                            eps2=eps2, theta=theta, pp=t["pp"])
+            assert np.array_equal(cnt, c3[:2])
+            assert np.max(np.abs(ax - fx) / bx) < 1e-13 and np.max(np.abs(ay - fy) / by) < 1e-13
+            assert np.all(bx >= np.abs(ax)) and np.all(by >= np.abs(ay))
+            compared += 1
+    assert compared >= 2
+
+
+def test_internal_second_moment_unweighted_closed_form():
+    """bottomUpNodeConstruction adds d d^T of each child's offset without its mass
+    (nBodySolverFMM_SYM_MORTON.h:451-466). Depth 1, leaf A = (0.1, 0.1), (0.3, 0.1), leaf
+    B = (0.9, 0.9): root COM (13/30, 11/30), offsets d_A = (-7, -8)/30, d_B = (14, 16)/30,
+    M2_A = (0.02, 0, 0), so root M2 = (245/900 + 0.02, 280/900, 320/900); the mass-weighted
+    rule would give (294/900 + 0.02, 336/900, 384/900)."""
+    xs = np.array([0.1, 0.3, 0.9])
+    ys = np.array([0.1, 0.1, 0.9])
+    t = O.fixed_tree(xs, ys, 1, 0.1, 0.1, 0.9, 0.9)
+    assert t["count"][1] == 2 and t["count"][4] == 1  # leaves (0, 0) and (1, 1)
+    assert t["m2"][1] == pytest.approx([0.02, 0.0, 0.0], abs=1e-15)
+    assert t["com_x"][0] == pytest.approx(13 / 30, rel=1e-15)
+    assert t["com_y"][0] == pytest.approx(11 / 30, rel=1e-15)
+    assert t["m2"][0] == pytest.approx([245 / 900 + 0.02, 280 / 900, 320 / 900], rel=1e-13)
