@@ -141,6 +141,18 @@ bh_status bh_leapfrog_graph(bh_ctx* ctx, int steps, float dt, double theta);
 bh_status bh_direct_fp64(bh_ctx* ctx, const uint32_t* sinks, uint32_t nsinks, double* ax,
                          double* ay, double* total);
 
+/* bh_direct_fp64 without any device allocation (no counterpart in the reference; for callers
+ * that cannot allocate: code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). bh_direct_fp64 grows its sink and partial-sum buffers with hipMalloc on demand;
+ * this variant keeps them in a static device array of the library (7.75 MiB per GPU, held until
+ * the process ends) and so takes at most 65,536 sinks (BH_ERR_CAPACITY beyond). d_sinks
+ * (caller-order indices) and the outputs (FP64, nsinks each; any output may be NULL) are device
+ * pointers; an index >= the particle count gives NaN for that sink. Asynchronous: the work is
+ * queued on the context's stream (bh_synchronize waits for it); the results equal
+ * bh_direct_fp64's bit for bit. Calls that overlap on one GPU share the array and race. */
+bh_status bh_direct_fp64_noalloc_async(bh_ctx* ctx, const uint32_t* d_sinks, uint32_t nsinks,
+                                       double* d_ax, double* d_ay, double* d_total);
+
 /* ---- t-SNE (the reference TSNE driver, tsne.h:42-563) ---- */
 typedef struct bh_tsne_params {
     double learning_rate;    /* tsne.h:90: N / 15 */

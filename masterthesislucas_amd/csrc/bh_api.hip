@@ -82,6 +82,8 @@ struct bh_ctx {
     uint32_t* dsinks = nullptr;          // direct sum: sink indices (caller, sorted)
     double* dpart = nullptr;             // direct sum: per-split partial sums + output
     uint64_t dsink_cap = 0, dpart_cap = 0;
+    void* static_scratch = nullptr;      // this device's copy of the allocation-free direct
+                                         // sum's static array (direct_static_scratch)
 
     uint32_t* dstatus = nullptr;   // build status words (kStatusWords), see status_check
     uint32_t* h_status = nullptr;  // pinned copy
@@ -509,6 +511,8 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
         }                         \
     } while (0)
     TRY(hip_check(c, hipGetDevice(&c->device), "hipGetDevice"));
+    TRY(hip_check(c, direct_static_scratch(c->device, &c->static_scratch),
+                  "hipGetSymbolAddress (static direct-sum scratch)"));
     TRY(hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"));
     const uint32_t N = cfg->n_max;
     if (c->wide) {
@@ -964,7 +968,8 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
     const double eps = c->cfg.softening;
     launch_direct_f64(s, grav ? kKernelGravity : kKernelTsne, c->pos4, c->id[c->cur], c->n,
                       c->dsinks, ns, grav ? (double)c->cfg.G : 1.0, eps * eps, c->where,
-                      c->dsinks + ns, c->dpart, S, out, c->nsys ? c->sys_off : nullptr, c->nsys);
+                      c->dsinks + ns, c->dpart, S, out, out + ns, out + 2ull * ns,
+                      c->nsys ? c->sys_off : nullptr, c->nsys);
     st = launch_status(c, "direct launch");
     if (st != BH_OK) return st;
     const size_t b = (size_t)ns * 8;
@@ -974,6 +979,24 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
         st = hip_check(c, hipMemcpyAsync(total, out + 2 * (uint64_t)ns, b, hipMemcpyDeviceToHost, s), "tot");
     if (st != BH_OK) return st;
     return hip_check(c, hipStreamSynchronize(s), "direct");
+}
+
+bh_status bh_direct_fp64_noalloc_async(bh_ctx* c, const uint32_t* d_sinks, uint32_t ns,
+                                       double* d_ax, double* d_ay, double* d_total) {
+    if (!c || (!d_sinks && ns)) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no particles in sorted state (build first)");
+    if (ns == 0) return BH_OK;
+    if (!direct_static_fits(c->n, ns))
+        return fail(c, BH_ERR_CAPACITY,
+                    "%u sinks exceed the static scratch of the allocation-free direct sum (at "
+                    "most %u sinks)", ns, kDirectStaticSinks);
+    const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
+    const double eps = c->cfg.softening;
+    launch_direct_f64_static(c->stream, grav ? kKernelGravity : kKernelTsne, c->pos4,
+                             c->id[c->cur], c->n, d_sinks, ns, grav ? (double)c->cfg.G : 1.0,
+                             eps * eps, c->where, c->static_scratch, d_ax, d_ay, d_total,
+                             c->nsys ? c->sys_off : nullptr, c->nsys);
+    return launch_status(c, "direct launch");
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1192,7 +1215,7 @@ bh_status bh_tsne_kl(bh_ctx* c, double* kl, double* z_exact) {
     if (st != BH_OK) return st;
     double* out = c->dpart + (uint64_t)3 * S * n;
     launch_direct_f64(s, kKernelTsne, c->pos4, c->id[c->cur], n, c->dsinks, n, 1.0, 0.0, c->where,
-                      c->dsinks + n, c->dpart, S, out);
+                      c->dsinks + n, c->dpart, S, out, out + n, out + 2ull * n);
     launch_sum_f64(s, out + 2ull * n, n, c->tz);
     launch_tsne_kl(s, c->p_row, c->p_col, c->p_val, n, c->ty_x[c->ty_cur], c->ty_y[c->ty_cur],
                    c->tz, c->kl_part, c->kl_out);

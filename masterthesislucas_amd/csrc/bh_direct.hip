@@ -3,6 +3,8 @@
 // result and to label training pairs. A block takes 64 sinks (one per lane); its four waves
 // split a chunk of sources, and the source chunks of a sink are summed by a second kernel in
 // a fixed order, so the result does not depend on scheduling.
+#include <mutex>
+
 #include "bh_common.h"
 #include "bh_launch.h"
 
@@ -18,8 +20,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict_
     __shared__ double red[3][kWavesPerBlock][kWave];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t s = blockIdx.x * kWave + lane;
-    const bool valid = s < ns;
-    const uint32_t si = valid ? sinks[s] : 0u;
+    const uint32_t sraw = s < ns ? sinks[s] : 0u;
+    const bool valid = s < ns && sraw < n;
+    const uint32_t si = valid ? sraw : 0u;
     const float4 me = pos4[si];
     const double px = me.x, py = me.y;
     const uint32_t split = blockIdx.y;
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict_
     red[1][w][lane] = sy;
     red[2][w][lane] = st;
     __syncthreads();
-    if (w == 0 && valid) {
+    if (w == 0 && s < ns) {
         double a = 0.0, b = 0.0, c = 0.0;
         for (int k = 0; k < kWavesPerBlock; ++k) {
             a += red[0][k][lane];
@@ -73,8 +76,14 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const float4* __restrict_
     }
 }
 
+// sums the S partials of each sink in split order; a sink index past the particles (only the
+// device-pointer entry point can pass one) gives NaN
 __global__ __launch_bounds__(kBlock) void k_direct_sum(const double* __restrict__ part, uint32_t S,
-                                                       uint32_t ns, double* __restrict__ out) {
+                                                       uint32_t ns,
+                                                       const uint32_t* __restrict__ sinks_sorted,
+                                                       uint32_t n, double* __restrict__ ox,
+                                                       double* __restrict__ oy,
+                                                       double* __restrict__ ot) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= ns) return;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -83,9 +92,10 @@ __global__ __launch_bounds__(kBlock) void k_direct_sum(const double* __restrict_
         b += part[((uint64_t)k * 3 + 1) * ns + s];
         c += part[((uint64_t)k * 3 + 2) * ns + s];
     }
-    out[s] = a;
-    out[ns + s] = b;
-    out[2 * (uint64_t)ns + s] = c;
+    if (sinks_sorted[s] >= n) a = b = c = __builtin_nan("");
+    if (ox) ox[s] = a;
+    if (oy) oy[s] = b;
+    if (ot) ot[s] = c;
 }
 
 // inverse of the sorted-order id array: where[id[i]] = i
@@ -95,11 +105,14 @@ __global__ __launch_bounds__(kBlock) void k_invert(const uint32_t* __restrict__ 
     if (i < n) where[id[i]] = i;
 }
 
-__global__ __launch_bounds__(kBlock) void k_take(const uint32_t* __restrict__ where,
+// sorted position of each selected caller index; an index >= n maps to kNone
+__global__ __launch_bounds__(kBlock) void k_take(const uint32_t* __restrict__ where, uint32_t n,
                                                  const uint32_t* __restrict__ sel, uint32_t ns,
                                                  uint32_t* __restrict__ out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < ns) out[i] = where[sel[i]];
+    if (i >= ns) return;
+    const uint32_t q = sel[i];
+    out[i] = q < n ? where[q] : kNone;
 }
 
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -115,9 +128,10 @@ uint32_t direct_splits(uint32_t n, uint32_t ns) {
 void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
                        uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
-                       double* out, const uint32_t* sys_off, uint32_t nsys) {
+                       double* out_x, double* out_y, double* out_t, const uint32_t* sys_off,
+                       uint32_t nsys) {
     k_invert<<<div_up(n, kBlock), kBlock, 0, s>>>(id, n, where);
-    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, sinks_caller, ns, sinks_sorted);
+    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, n, sinks_caller, ns, sinks_sorted);
     dim3 g(div_up(ns, kWave), S);
     if (kernel == kKernelGravity)
         k_direct_f64<kKernelGravity><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part,
@@ -125,13 +139,65 @@ void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint
     else
         k_direct_f64<kKernelTsne><<<g, kBlock, 0, s>>>(pos4, n, sinks_sorted, ns, G, eps2, part, S,
                                                        sys_off, nsys);
-    k_direct_sum<<<div_up(ns, kBlock), kBlock, 0, s>>>(part, S, ns, out);
+    k_direct_sum<<<div_up(ns, kBlock), kBlock, 0, s>>>(part, S, ns, sinks_sorted, n, out_x, out_y,
+                                                       out_t);
+}
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free direct sum (bh_direct_fp64_noalloc_async): the sorted sink indices and the
+// per-split partial sums live in this file-scope array, one copy per GPU, held until the
+// process ends. With S = direct_splits(n, ns) <= ceil(4096 / ceil(ns / 64)), S ns is at most
+// 64 * 4096 + ns, so kDirectStaticSinks sinks need 3 (262,144 + kDirectStaticSinks) doubles of
+// partials plus kDirectStaticSinks indices: 7.75 MiB. Every kernel writes each scratch entry
+// it later reads within the same call, so nothing needs clearing; calls that overlap on one GPU
+// share the array and race.
+// ------------------------------------------------------------------------------------------
+constexpr uint64_t kStaticPartDoubles = 3ull * (64ull * 4096ull + kDirectStaticSinks);
+constexpr uint64_t kStaticWords = kStaticPartDoubles + kDirectStaticSinks / 2;  // 8-byte words
+static __device__ __align__(256) uint64_t scratch[kStaticWords];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t direct_static_scratch(int device, void** ptr) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+bool direct_static_fits(uint32_t n, uint32_t ns) {
+    if (ns > kDirectStaticSinks) return false;
+    return 3ull * direct_splits(n, ns) * ns <= kStaticPartDoubles;
+}
+
+void launch_direct_f64_static(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
+                              uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G,
+                              double eps2, uint32_t* where, void* scratch_ptr, double* out_x,
+                              double* out_y, double* out_t, const uint32_t* sys_off,
+                              uint32_t nsys) {
+    double* part = static_cast<double*>(scratch_ptr);
+    uint32_t* sinks_sorted = reinterpret_cast<uint32_t*>(part + kStaticPartDoubles);
+    launch_direct_f64(s, kernel, pos4, id, n, sinks_caller, ns, G, eps2, where, sinks_sorted, part,
+                      direct_splits(n, ns), out_x, out_y, out_t, sys_off, nsys);
 }
 
 void launch_sinks_sorted(hipStream_t s, const uint32_t* id, uint32_t n, const uint32_t* sinks_caller,
                          uint32_t ns, uint32_t* where, uint32_t* sinks_sorted) {
     k_invert<<<div_up(n, kBlock), kBlock, 0, s>>>(id, n, where);
-    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, sinks_caller, ns, sinks_sorted);
+    k_take<<<div_up(ns, kBlock), kBlock, 0, s>>>(where, n, sinks_caller, ns, sinks_sorted);
 }
 
 }  // namespace bh

@@ -135,10 +135,25 @@ void launch_kick(hipStream_t s, float* vx, float* vy, const float* ax, const flo
 
 uint32_t direct_splits(uint32_t n, uint32_t ns);
 // sys_off (nsys + 1 offsets, may be null): sources restricted to the sink's own system
+// outputs (any may be null) indexed like sinks_caller; an index >= n gives NaN
 void launch_direct_f64(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
                        uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G, double eps2,
                        uint32_t* where, uint32_t* sinks_sorted, double* part, uint32_t S,
-                       double* out, const uint32_t* sys_off = nullptr, uint32_t nsys = 0);
+                       double* out_x, double* out_y, double* out_t,
+                       const uint32_t* sys_off = nullptr, uint32_t nsys = 0);
+// the same with the sorted sinks and partials in the library's static device array (no
+// allocation): at most kDirectStaticSinks sinks, and direct_static_fits(n, ns) must hold
+constexpr uint32_t kDirectStaticSinks = 65536;
+constexpr int kMaxDevices = 64;
+// device address of this GPU's copy of the static array (hipGetSymbolAddress once per device,
+// with `device` current)
+hipError_t direct_static_scratch(int device, void** ptr);
+bool direct_static_fits(uint32_t n, uint32_t ns);
+void launch_direct_f64_static(hipStream_t s, int kernel, const float4* pos4, const uint32_t* id,
+                              uint32_t n, const uint32_t* sinks_caller, uint32_t ns, double G,
+                              double eps2, uint32_t* where, void* scratch_ptr, double* out_x,
+                              double* out_y, double* out_t, const uint32_t* sys_off,
+                              uint32_t nsys);
 
 // stage also writes the interleaved copy xy that the update's neighbour gathers read
 void launch_tsne_stage(hipStream_t s, const float* yx, const float* yy, uint32_t n, float* x,
