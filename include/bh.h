@@ -134,6 +134,15 @@ bh_status bh_leapfrog(bh_ctx* ctx, int steps, float dt, double theta);
  * re-captured when dt, theta or the traversal variant change. */
 bh_status bh_leapfrog_graph(bh_ctx* ctx, int steps, float dt, double theta);
 
+/* Conserved quantities of a gravity context in FP64 (no counterpart in the reference, whose
+ * gravity integrator is commented out, gravitysim.h:99-121): kinetic energy sum m |v|^2 / 2,
+ * potential energy sum m phi / 2 with phi the per-particle potential of the Barnes-Hut walk at
+ * theta on the current tree (G included), and momentum sum m v. Runs one walk with totals on the
+ * last built tree (bh_leapfrog leaves one built at the current positions); the next bh_leapfrog
+ * then evaluates its first forces anew. Any output may be NULL. */
+bh_status bh_energy(bh_ctx* ctx, double theta, double* kinetic, double* potential,
+                    double* momentum_x, double* momentum_y);
+
 /* FP64 direct sum for sampled sinks (NBodySolverNaive::solveNbody with the same kernel,
  * nBodySolverNaive.h:53-140), over all current particles with the sink itself excluded by
  * index. sinks: caller-order indices (host). Outputs (host, FP64, nsinks each): the exact

@@ -112,6 +112,7 @@ SIGNATURES = {
     "bh_set_solver": (_i32, [_vp, C.c_int]),
     "bh_set_sort_spin_limit": (_i32, [_vp, _u32]),
     "bh_direct_fp64_noalloc_async": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),
+    "bh_energy": (_i32, [_vp, _d, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -577,7 +577,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     TRY(dalloc(c, &c->prm, 1));
     TRY(dalloc(c, &c->bounds_part, kMaxBoundsBlocks));
     TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
-    TRY(dalloc(c, &c->dsum, 1));
+    TRY(dalloc(c, &c->dsum, 5));  // [0] the walk's FP64 total, [1..4] state sums
     c->max_blocks = traverse_blocks(N) * kMaxSplit + extra_blocks;
     TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
@@ -979,6 +979,34 @@ bh_status bh_direct_fp64(bh_ctx* c, const uint32_t* sinks, uint32_t ns, double* 
         st = hip_check(c, hipMemcpyAsync(total, out + 2 * (uint64_t)ns, b, hipMemcpyDeviceToHost, s), "tot");
     if (st != BH_OK) return st;
     return hip_check(c, hipStreamSynchronize(s), "direct");
+}
+
+bh_status bh_energy(bh_ctx* c, double theta, double* kinetic, double* potential,
+                    double* momentum_x, double* momentum_y) {
+    if (!c) return BH_ERR_ARG;
+    if (c->cfg.kernel != BH_KERNEL_GRAVITY)
+        return fail(c, BH_ERR_ARG, "energy and momentum need the gravity kernel");
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built for the current positions");
+    if (c->solver != BH_SOLVER_BH) return fail(c, BH_ERR_STATE, "energy needs the Barnes-Hut walk");
+    bh_status st = do_forces(c, theta, true);  // per-particle potential in tot
+    if (st != BH_OK) return st;
+    // the accelerations now belong to this theta: the next leapfrog call evaluates its own
+    c->have_acc = false;
+    const int cur = c->cur;
+    launch_state_sums(c->stream, c->m[cur], c->vx[cur], c->vy[cur], c->tot, c->n, c->red_part,
+                      c->dsum + 1);
+    st = launch_status(c, "state sums");
+    if (st != BH_OK) return st;
+    double h[4];
+    st = hip_check(c, hipMemcpyAsync(h, c->dsum + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream),
+                   "state copy");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "energy");
+    if (st != BH_OK) return st;
+    if (kinetic) *kinetic = h[0];
+    if (potential) *potential = h[1];
+    if (momentum_x) *momentum_x = h[2];
+    if (momentum_y) *momentum_y = h[3];
+    return BH_OK;
 }
 
 bh_status bh_direct_fp64_noalloc_async(bh_ctx* c, const uint32_t* d_sinks, uint32_t ns,

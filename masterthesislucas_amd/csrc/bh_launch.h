@@ -110,6 +110,10 @@ uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* p
 constexpr uint32_t kMaxSplit = 8;
 void launch_sum_parts(hipStream_t s, const float* part, uint32_t F, uint32_t n, float* out);
 uint32_t reduce_grid(uint32_t n);
+// out[0..3] = FP64 kinetic energy, potential energy (sum m phi / 2), momentum x, y; partials
+// holds 4 doubles per block (at most kMaxReduceBlocks / 4 blocks)
+void launch_state_sums(hipStream_t s, const float* m, const float* vx, const float* vy,
+                       const float* phi, uint32_t n, double* partials, double* out);
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,
                        double* out);
 uint32_t traverse_blocks(uint32_t n);

@@ -697,6 +697,45 @@ __global__ __launch_bounds__(kBlock) void k_reduce_f64_final(const double* __res
     if (threadIdx.x == 0) *out = s;
 }
 
+// Conserved quantities of the gravity state in FP64: per block the partial sums of kinetic
+// energy sum m (vx^2 + vy^2) / 2, potential energy sum m phi / 2 (phi = the walk's per-particle
+// potential, G included) and momentum sum m v; then one block adds the partials in block order.
+__global__ __launch_bounds__(kBlock) void k_state_partial(const float* __restrict__ m,
+                                                          const float* __restrict__ vx,
+                                                          const float* __restrict__ vy,
+                                                          const float* __restrict__ phi, uint32_t n,
+                                                          double* __restrict__ partials) {
+    double ke = 0.0, pe = 0.0, px = 0.0, py = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double mi = m[i], u = vx[i], v = vy[i];
+        ke += 0.5 * mi * (u * u + v * v);
+        pe += 0.5 * mi * (double)phi[i];
+        px += mi * u;
+        py += mi * v;
+    }
+    ke = block_sum_d(ke);
+    pe = block_sum_d(pe);
+    px = block_sum_d(px);
+    py = block_sum_d(py);
+    if (threadIdx.x == 0) {
+        partials[4 * (size_t)blockIdx.x + 0] = ke;
+        partials[4 * (size_t)blockIdx.x + 1] = pe;
+        partials[4 * (size_t)blockIdx.x + 2] = px;
+        partials[4 * (size_t)blockIdx.x + 3] = py;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_state_final(const double* __restrict__ partials,
+                                                        uint32_t nb, double* __restrict__ out) {
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
+        for (int q = 0; q < 4; ++q) r[q] += partials[4 * (size_t)b + q];
+    for (int q = 0; q < 4; ++q) {
+        const double t = block_sum_d(r[q]);
+        if (threadIdx.x == 0) out[q] = t;
+    }
+}
+
 // out[id[i]] = a[i] (results back to the caller's particle order)
 __global__ __launch_bounds__(kBlock) void k_unsort2(const uint32_t* __restrict__ id, uint32_t n,
                                                     const float* __restrict__ a,
@@ -896,6 +935,13 @@ uint32_t reduce_grid(uint32_t n) {
     if (g < 1) g = 1;
     if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
     return g;
+}
+
+void launch_state_sums(hipStream_t s, const float* m, const float* vx, const float* vy,
+                       const float* phi, uint32_t n, double* partials, double* out) {
+    const uint32_t g = reduce_grid(n) < kMaxReduceBlocks / 4 ? reduce_grid(n) : kMaxReduceBlocks / 4;
+    k_state_partial<<<g, kBlock, 0, s>>>(m, vx, vy, phi, n, partials);
+    k_state_final<<<1, kBlock, 0, s>>>(partials, g, out);
 }
 
 void launch_reduce_f64(hipStream_t s, const float* v, const float* w, uint32_t n, double* partials,

@@ -159,6 +159,15 @@ class BHContext:
     def leapfrog(self, steps, dt, theta):
         self._c(_lib.load().bh_leapfrog(self._h, int(steps), float(dt), float(theta)))
 
+    def energy(self, theta):
+        """FP64 kinetic and potential energy (the walk's potential at theta) and momentum of the
+        current state (bh_energy): dict(kinetic, potential, total, px, py)."""
+        k, w, px, py = (C.c_double(0.0) for _ in range(4))
+        self._c(_lib.load().bh_energy(self._h, float(theta), C.byref(k), C.byref(w), C.byref(px),
+                                      C.byref(py)))
+        return dict(kinetic=k.value, potential=w.value, total=k.value + w.value, px=px.value,
+                    py=py.value)
+
     def direct_fp64(self, sinks):
         """FP64 direct sum on the GPU for caller-order sink indices -> (ax, ay, total)."""
         sk = np.ascontiguousarray(sinks, dtype=np.uint32)

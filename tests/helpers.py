@@ -133,3 +133,42 @@ def check_exact_walk(ctx, x, y, m, kernel, theta, eps, ax, ay, tot, t64=None):
     assert e.max() <= WALK_TOL, (float(e.max()), int(np.argmax(e)))
     assert abs(tot - w["tot"].sum()) <= WALK_TOL * np.abs(w["tot"]).sum()
     return w
+
+
+def batched_export(ctx):
+    """(offsets, sort permutation, exported tree) of a batched context, for exact_walk_system."""
+    _, off = ctx.batch_info()
+    _, perm = ctx.morton()
+    return off, perm, ctx.tree()
+
+
+def exact_walk_system(ctx, s, x32, y32, m32, kernel, theta, sinks, eps=0.0, G=1.0, R=4,
+                      nthreads=8, export=None):
+    """exact_walk for system s of a batched context: the oracle's single-precision-decision walk
+    over that system's subtree of the GPU's node records (levels counted from the system root at
+    level R, side = the system's own QuadTree root square), for caller-order sink indices local
+    to the system. x32, y32, m32: the system's own particles; export: batched_export(ctx), taken
+    once for many systems. Returns (ax, ay, abs_x, abs_y) per sink."""
+    off, perm, g = batched_export(ctx) if export is None else export
+    lo, hi = int(off[s]), int(off[s + 1])
+    root = int(np.flatnonzero((g["first"] == lo) & (g["level"] == R))[0])
+    end = int(g["skip"][root])
+    sl = slice(root, end)
+    tree = dict(level=g["level"][sl] - R, first=g["first"][sl] - lo, count=g["count"][sl],
+                skip=np.minimum(g["skip"][sl], end) - root,
+                pp=((g["flags"][sl] >> 1) & 1).astype(np.uint8), cx=g["cx"][sl], cy=g["cy"][sl],
+                mass=g["mass"][sl])
+    local = perm[lo:hi].astype(np.int64) - lo  # system-local caller index at each sorted slot
+    x32 = np.asarray(x32, np.float32)
+    y32 = np.asarray(y32, np.float32)
+    m = np.ones(hi - lo, np.float32) if m32 is None else np.asarray(m32, np.float32)
+    xd, yd = x32.astype(np.float64), y32.astype(np.float64)
+    side = O.quadtree_side(xd.min(), yd.min(), xd.max(), yd.max())
+    where = np.empty(hi - lo, np.int64)
+    where[local] = np.arange(hi - lo)
+    e32 = np.float32(eps)
+    ax, ay, _, bx, by, _ = O.bh_traverse_f32(
+        x32[local], y32[local], m[local], tree, np.float32(side), O.inv_theta2_f32(theta),
+        O.KERNEL_GRAVITY if kernel == "gravity" else O.KERNEL_TSNE, G=G, eps2=float(e32 * e32),
+        sinks=where[np.asarray(sinks, np.int64)], nthreads=nthreads)
+    return ax, ay, bx, by

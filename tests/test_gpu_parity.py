@@ -300,9 +300,10 @@ def test_direct_fp64_matches_cpu_oracle(gpu):
 
 
 def test_leapfrog_matches_oracle(gpu):
-    """Ten KDK steps of the uniform-disk case: positions and velocities vs the FP64 oracle."""
-    n, steps, dt, theta = 16384, 10, 1e-4, 0.5
-    p = uniform_disk(n, seed=21)
+    """Config 1: ten KDK steps of 65,536 points uniform in the unit disk: positions and
+    velocities vs the FP64 oracle."""
+    n, steps, dt, theta = N1, 10, 1e-4, 0.5
+    p = uniform_disk(n)
     ctx = _ctx(n, kernel="gravity", tree_mode="quadtree", softening=EPS)
     ctx.set_particles(p.x, p.y, p.m, p.vx, p.vy)
     ctx.leapfrog(steps, dt, theta)
