@@ -132,12 +132,15 @@ def learned_comparison(args, eps):
     ctx.forces_learned(net, th_l, band=args.learned_band)  # warm
     ctx.profile_enable(True)
     ctx.profile_read(reset=True)
-    for _ in range(3):
+    for _ in range(3):  # a learned step: build, cell features, per-cell inference, walk
+        ctx.build()
         ctx.forces_learned(net, th_l, band=args.learned_band)
-    lms = ctx.profile_read(reset=True)["traversal"][0] / 3
+    ph = ctx.profile_read(reset=True)
     ctx.profile_enable(False)
+    lms = ph["traversal"][0] / 3  # per-cell inference + corrected walk
+    l_step_ms = sum(v[0] for v in ph.values()) / 3
     le = med(*ctx.forces_learned(net, th_l, band=args.learned_band)[:2])
-    learned_rate = n / ((lms + build_ms) / 1e3)
+    learned_rate = n / (l_step_ms / 1e3)
     p05 = next(r for r in plain if r["theta"] == 0.5)
     # plain theta with the same median error (linear interpolation in the sweep)
     eq = None
@@ -151,7 +154,8 @@ def learned_comparison(args, eps):
            "theta_learned": th_l, "band": args.learned_band, "training_pairs": added,
            "training_epochs": args.learned_epochs, "train_loss": loss,
            "collect_s": t_collect, "train_s": t_train, "build_ms": build_ms,
-           "learned_traversal_ms": lms, "learned_median_rel": le,
+           "learned_traversal_ms": lms, "learned_step_ms": l_step_ms,
+           "learned_median_rel": le,
            "learned_force_evals_per_s": learned_rate, "plain": plain,
            "plain_at_equal_error": eq,
            "speedup_vs_plain_theta_0.5": learned_rate / p05["force_evals_per_s"],

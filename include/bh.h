@@ -207,44 +207,58 @@ bh_status bh_tsne_run(bh_ctx* ctx, const bh_tsne_params* p, int32_t first, int32
 bh_status bh_tsne_kl(bh_ctx* ctx, double* kl, double* z_exact);
 
 /* ---- learned far-field correction ----------------------------------------------------------
- * No counterpart in the reference. A 10 -> 64 -> 64 -> 2 SiLU MLP predicts, for a cell the
- * opening-angle test accepts, the residual between the exact sum over the cell's particles and
- * its monopole (inputs and units in masterthesislucas_amd/csrc/bh_mlp.h). Training data come
- * from the tree itself, labelled by FP64 direct sums over cell contents; training runs on the
- * GPU (FP32 forward/backward kernels, Adam); inference inside the traversal uses BF16 MFMA with
- * FP32 accumulation. Gravity kernel only. */
+ * No counterpart in the reference. Per cell and build, a 10 -> 64 -> 64 -> 14 SiLU MLP maps the
+ * cell's features (normalised second and third moments, centre-of-mass offset in its box, size)
+ * to 7 complex coefficients c_j; an accepted (sink, cell) pair at d = c - p, r = |d|, side l then
+ * gets (M l^2 / r^4) sum_j c_j b_j(d / r, l / r) added to its monopole, b the quadrupole and
+ * octupole angular forms (exact definitions in masterthesislucas_amd/csrc/bh_mlp.h). Training
+ * data come from the tree itself, labelled by FP64 direct sums over cell contents; training
+ * runs on the GPU (FP32 forward/backward kernels, Adam); the per-cell inference uses BF16 MFMA
+ * with FP32 accumulation. Gravity kernel, one-system BH_TREE_QUADTREE contexts only. */
 typedef struct bh_mlp bh_mlp;
-uint32_t bh_mlp_param_count(void); /* 4994: W1[10x64] b1 W2[64x64] b2 W3[64x2] b3, W row-major */
+/* 5774: W1[10x64] b1 W2[64x64] b2 W3[64x14] b3, W row-major [in x out] */
+uint32_t bh_mlp_param_count(void);
 /* seeded uniform(+-1/sqrt(fan_in)) weights, zero biases; room for max_samples training pairs */
 bh_status bh_mlp_create(bh_mlp** out, uint32_t max_samples, uint64_t seed);
 void bh_mlp_destroy(bh_mlp* mlp);
 const char* bh_mlp_last_error(const bh_mlp* mlp);
 bh_status bh_mlp_set_params(bh_mlp* mlp, const float* params, uint32_t count); /* resets Adam */
 bh_status bh_mlp_get_params(bh_mlp* mlp, float* params, uint32_t count);
-/* Y[n x 2] for host X[n x 10]: FP32 kernels (bf16_mfma = 0) or the BF16 MFMA inference path */
+/* coefficients Y[n x 14] of host cell features X[n x 10]: FP32 kernels (bf16_mfma = 0) or the
+ * BF16 MFMA inference path the walk uses */
 bh_status bh_mlp_forward(bh_mlp* mlp, const float* X, uint32_t n, float* Y, int bf16_mfma);
-/* mean squared error over the n x 2 outputs and its gradient (FP32 kernels) */
+/* Samples X[n x 13] (cell features, w_x, w_y, q = l / r) with labels T[n x 2] (the residual
+ * of the exact cell sum over the monopole, in units of M l^2 / r^4): loss = sum over samples of
+ * |sum_j c_j b_j - T|^2 / (2 n) and its gradient (FP32 kernels) */
 bh_status bh_mlp_loss_grad(bh_mlp* mlp, const float* X, const float* T, uint32_t n, float* grads,
                            double* loss);
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with a given gradient */
 bh_status bh_mlp_adam_step(bh_mlp* mlp, const float* grads, float lr);
 bh_status bh_mlp_dataset_clear(bh_mlp* mlp);
 bh_status bh_mlp_dataset_size(bh_mlp* mlp, uint32_t* n);
+/* the data set: X[n x 13], T[n x 2] as in bh_mlp_loss_grad */
 bh_status bh_mlp_dataset_get(bh_mlp* mlp, float* X, float* T, uint32_t n);
 bh_status bh_mlp_dataset_set(bh_mlp* mlp, const float* X, const float* T, uint32_t n);
-/* epochs over the data set in shuffled full batches; final_loss = MSE over the data set */
+/* epochs over the data set in shuffled full batches; final_loss = the loss over the data set */
 bh_status bh_mlp_train(bh_mlp* mlp, int epochs, uint32_t batch, float lr, uint64_t seed,
                        double* final_loss);
-/* keep cell second moments in the builds of ctx (needed by the two calls below) */
+/* keep cell second moments and features in the builds of ctx (needed by the calls below) */
 bh_status bh_learned_enable(bh_ctx* ctx, int enable);
 /* append the (sink, cell) pairs the walk at theta accepts with l/r >= band (band 0: all) for
  * the given caller-order sinks, labelled by FP64 sums over the cells, to mlp's data set */
 bh_status bh_learned_collect(bh_ctx* ctx, bh_mlp* mlp, double theta, double band,
                              const uint32_t* sinks, uint32_t nsinks, uint32_t max_pairs_per_sink,
                              uint32_t* added);
-/* bh_forces with the learned correction on accepted cells with l/r >= band */
+/* bh_forces with the learned correction on accepted cells with l/r >= band; the coefficient
+ * records are computed from mlp for the current tree first (part of the call) */
 bh_status bh_forces_learned(bh_ctx* ctx, const bh_mlp* mlp, double theta, double band, float* ax,
                             float* ay, double* total);
+/* the current tree's cell features (features[n_nodes x 10]) and, with mlp, the walk's
+ * coefficient records (coef[n_nodes x 16]: for c_j = a_j + i b_j, a1+a2, b2-b1, a1-a2, b1+b2,
+ * a3, b3, a4, b4, a5+a6, b6-b5, a5-a6, b5+b6, a7, b7, 0, 0); n_nodes must equal the node count
+ * of the tree (bh_tree_info). Either output may be NULL. */
+bh_status bh_learned_export(bh_ctx* ctx, const bh_mlp* mlp, float* features, float* coef,
+                            uint32_t n_nodes);
 
 /* ---- failure reporting (no reference counterpart: the reference checks no errors,
  * cudaHelper.cuh:22-31) ----

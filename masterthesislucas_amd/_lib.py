@@ -93,6 +93,7 @@ SIGNATURES = {
     "bh_learned_enable": (_i32, [_vp, _i32]),
     "bh_learned_collect": (_i32, [_vp, _vp, _d, _d, _vp, _u32, _u32, _vp]),
     "bh_forces_learned": (_i32, [_vp, _vp, _d, _d, _vp, _vp, _vp]),
+    "bh_learned_export": (_i32, [_vp, _vp, _vp, _vp, _u32]),
     "bh_create_batched": (_i32, [_vp, _vp, _vp, _u32]),
     "bh_batch_info": (_i32, [_vp, _vp, _vp]),
     "bh_leapfrog_graph": (_i32, [_vp, _i32, _f, _d]),

@@ -111,6 +111,9 @@ struct bh_ctx {
     uint32_t* iota_dev = nullptr;
 
     bool learned = false;  // second moments in builds, learned correction available
+    float* lfeat = nullptr;   // cell features of the last build (kMlpIn per node)
+    float4* lcoef = nullptr;  // coefficient records (kCoefStride floats per node)
+    bool lfeat_ok = false;    // lfeat holds the features of the current tree
 
     // batched systems (bh_create_batched): particles grouped by system, one tree per system
     uint32_t nsys = 0;             // 0: one system
@@ -335,6 +338,7 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
     // the FMM solver computes its own moments: the node records' moments are made when a
     // consumer of them asks (ensure_moments)
     c->moments = false;
+    c->lfeat_ok = false;
     if (c->solver != BH_SOLVER_FMM_SYM) launch_node_moments(c);
     bh_status st = launch_status(c, "build launch");
     if (st != BH_OK) return st;
@@ -931,6 +935,7 @@ bh_status bh_leapfrog_graph(bh_ctx* c, int steps, float dt, double theta) {
     }
     c->built = true;
     c->have_acc = true;
+    c->lfeat_ok = false;
     return finish_build_call(c, "graph steps");
 }
 
@@ -1624,8 +1629,8 @@ bh_status bh_mlp_create(bh_mlp** out, uint32_t max_samples, uint64_t seed) {
     if (st == BH_OK) st = malloc_m(m, &m->m1, kMlpParams);
     if (st == BH_OK) st = malloc_m(m, &m->m2, kMlpParams);
     if (st == BH_OK) st = malloc_m(m, &m->wbf, 1);
-    if (st == BH_OK) st = malloc_m(m, &m->X, (size_t)max_samples * kMlpIn);
-    if (st == BH_OK) st = malloc_m(m, &m->T, (size_t)max_samples * kMlpOut);
+    if (st == BH_OK) st = malloc_m(m, &m->X, (size_t)max_samples * kLearnSample);
+    if (st == BH_OK) st = malloc_m(m, &m->T, (size_t)max_samples * 2);
     if (st == BH_OK) st = malloc_m(m, &m->order, max_samples);
     if (st != BH_OK) {
         *out = m;  // error string readable; caller destroys
@@ -1699,8 +1704,8 @@ bh_status bh_mlp_loss_grad(bh_mlp* m, const float* X, const float* T, uint32_t n
                            double* loss) {
     if (!m || !X || !T || !n) return BH_ERR_ARG;
     float *dX = nullptr, *dT = nullptr, *dG = nullptr;
-    bh_status st = to_device(m, X, (size_t)n * kMlpIn, &dX);
-    if (st == BH_OK) st = to_device(m, T, (size_t)n * kMlpOut, &dT);
+    bh_status st = to_device(m, X, (size_t)n * kLearnSample, &dX);
+    if (st == BH_OK) st = to_device(m, T, (size_t)n * 2, &dT);
     if (st == BH_OK) st = mcheck(m, hipMalloc((void**)&dG, kMlpParams * 4), "hipMalloc");
     if (st == BH_OK) st = mlp_grad_device(m, dX, dT, nullptr, n);
     if (st == BH_OK) {
@@ -1758,9 +1763,9 @@ bh_status bh_mlp_dataset_get(bh_mlp* m, float* X, float* T, uint32_t n) {
     if (!m) return BH_ERR_ARG;
     if (n > m->n) return mfail(m, BH_ERR_ARG, "data set holds %u samples", m->n);
     bh_status st = BH_OK;
-    if (X) st = mcheck(m, hipMemcpyAsync(X, m->X, (size_t)n * kMlpIn * 4, hipMemcpyDeviceToHost, m->stream), "X");
+    if (X) st = mcheck(m, hipMemcpyAsync(X, m->X, (size_t)n * kLearnSample * 4, hipMemcpyDeviceToHost, m->stream), "X");
     if (st == BH_OK && T)
-        st = mcheck(m, hipMemcpyAsync(T, m->T, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, m->stream), "T");
+        st = mcheck(m, hipMemcpyAsync(T, m->T, (size_t)n * 2 * 4, hipMemcpyDeviceToHost, m->stream), "T");
     if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "data set");
     return st;
 }
@@ -1769,9 +1774,9 @@ bh_status bh_mlp_dataset_set(bh_mlp* m, const float* X, const float* T, uint32_t
     if (!m || (n && (!X || !T))) return BH_ERR_ARG;
     if (n > m->cap) return mfail(m, BH_ERR_CAPACITY, "%u samples exceed the capacity %u", n, m->cap);
     bh_status st = BH_OK;
-    if (n) st = mcheck(m, hipMemcpyAsync(m->X, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, m->stream), "X");
+    if (n) st = mcheck(m, hipMemcpyAsync(m->X, X, (size_t)n * kLearnSample * 4, hipMemcpyHostToDevice, m->stream), "X");
     if (st == BH_OK && n)
-        st = mcheck(m, hipMemcpyAsync(m->T, T, (size_t)n * kMlpOut * 4, hipMemcpyHostToDevice, m->stream), "T");
+        st = mcheck(m, hipMemcpyAsync(m->T, T, (size_t)n * 2 * 4, hipMemcpyHostToDevice, m->stream), "T");
     if (st == BH_OK) st = mcheck(m, hipStreamSynchronize(m->stream), "data set");
     if (st == BH_OK) m->n = n;
     return st;
@@ -1814,7 +1819,7 @@ bh_status bh_mlp_train(bh_mlp* m, int epochs, uint32_t batch, float lr, uint64_t
         const uint32_t C = 65536;
         for (uint32_t o = 0; o < n && st == BH_OK; o += C) {
             const uint32_t b = std::min(C, n - o);
-            st = mlp_grad_device(m, m->X + (size_t)o * kMlpIn, m->T + (size_t)o * kMlpOut, nullptr, b);
+            st = mlp_grad_device(m, m->X + (size_t)o * kLearnSample, m->T + (size_t)o * 2, nullptr, b);
             std::vector<double> lp(mlp_loss_blocks(b));
             if (st == BH_OK)
                 st = mcheck(m, hipMemcpyAsync(lp.data(), m->loss_part, lp.size() * 8,
@@ -1828,15 +1833,50 @@ bh_status bh_mlp_train(bh_mlp* m, int epochs, uint32_t batch, float lr, uint64_t
     return st;
 }
 
+}  // extern "C"
+
+namespace {
+
+// cell features of the current tree (after the upward pass: they use the second moments)
+bh_status ensure_features(bh_ctx* c) {
+    if (ensure_moments(c) != BH_OK) return BH_ERR_HIP;
+    if (c->lfeat_ok) return BH_OK;
+    const void* codes = c->keys_in_b ? c->keys_b : c->keys_a;
+    phase_begin(c, 4);
+    launch_cell_features(c->stream, c->pos4, c->nodes, c->m2, codes, c->wide, c->B, c->prm,
+                         c->cap_nodes, c->lfeat);
+    phase_end(c, 4);
+    bh_status st = launch_status(c, "cell features");
+    if (st == BH_OK) c->lfeat_ok = true;
+    return st;
+}
+
+// the network's coefficient record of every cell, for the walk
+bh_status ensure_coefs(bh_ctx* c, const bh_mlp* m) {
+    bh_status st = ensure_features(c);
+    if (st != BH_OK) return st;
+    launch_cell_coefs(c->stream, c->lfeat, c->prm, c->cap_nodes, m->wbf, c->lcoef);
+    return launch_status(c, "cell coefficients");
+}
+
+}  // namespace
+
+extern "C" {
+
 bh_status bh_learned_enable(bh_ctx* c, int enable) {
     if (!c) return BH_ERR_ARG;
     if (c->cfg.kernel != BH_KERNEL_GRAVITY)
         return fail(c, BH_ERR_STATE, "the learned correction is trained for the gravity kernel");
+    if (c->cfg.tree_mode != BH_TREE_QUADTREE || c->nsys)
+        return fail(c, BH_ERR_STATE, "the learned correction needs a one-system QUADTREE context");
     if (enable && !c->m2) {
         bh_status st = dalloc(c, &c->m2, c->cap_nodes);
+        if (st == BH_OK) st = dalloc(c, &c->lfeat, (size_t)c->cap_nodes * kMlpIn);
+        if (st == BH_OK) st = dalloc(c, &c->lcoef, (size_t)c->cap_nodes * (kCoefStride / 4));
         if (st != BH_OK) return st;
     }
     c->learned = enable != 0;
+    c->lfeat_ok = false;
     c->built = false;  // second moments appear with the next build
     return BH_OK;
 }
@@ -1847,7 +1887,7 @@ bh_status bh_learned_collect(bh_ctx* c, bh_mlp* m, double theta, double band,
     if (!c || !m || (!sinks && ns) || !(theta > 0.0) || band < 0.0 || max_pairs_per_sink == 0)
         return BH_ERR_ARG;
     if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
-    if (ensure_moments(c) != BH_OK) return BH_ERR_HIP;
+    if (ensure_features(c) != BH_OK) return BH_ERR_HIP;
     if (ns == 0) return BH_OK;
     for (uint32_t k = 0; k < ns; ++k)
         if (sinks[k] >= c->n) return fail(c, BH_ERR_ARG, "sink index %u out of range", sinks[k]);
@@ -1881,9 +1921,9 @@ bh_status bh_learned_collect(bh_ctx* c, bh_mlp* m, double theta, double band,
                   (unsigned long long)total);
     if (st == BH_OK) {
         const double eps = c->cfg.softening;
-        launch_label_pairs(s, c->pos4, c->nodes, c->m2, d_ss, pairs, counts, offs, ns,
-                           max_pairs_per_sink, eps * eps, m->X + (size_t)m->n * kMlpIn,
-                           m->T + (size_t)m->n * kMlpOut, (uint32_t)total);
+        launch_label_pairs(s, c->pos4, c->nodes, c->lfeat, c->prm, d_ss, pairs, counts, offs, ns,
+                           max_pairs_per_sink, eps * eps, m->X + (size_t)m->n * kLearnSample,
+                           m->T + (size_t)m->n * 2, (uint32_t)total);
         st = launch_status(c, "label");
     }
     if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "label");
@@ -1901,14 +1941,15 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
                             float* ay, double* total) {
     if (!c || !m || !(theta > 0.0) || band < 0.0) return BH_ERR_ARG;
     if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
-    if (ensure_moments(c) != BH_OK) return BH_ERR_HIP;
+    if (ensure_features(c) != BH_OK) return BH_ERR_HIP;
     hipStream_t s = c->stream;
     const float inv_t2 = (float)(1.0 / (theta * theta));
     const float inv_b2 = band > 0.0 ? (float)(1.0 / (band * band)) : 0.f;
     const float eps = c->cfg.softening;
-    phase_begin(c, 5);
-    c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->m2, c->prm, c->n,
-                                             inv_t2, inv_b2, eps * eps, c->cfg.G, m->wbf, c->ax,
+    phase_begin(c, 5);  // the per-cell inference is part of the learned force evaluation
+    launch_cell_coefs(s, c->lfeat, c->prm, c->cap_nodes, m->wbf, c->lcoef);
+    c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->lcoef, c->prm,
+                                             c->n, inv_t2, inv_b2, eps * eps, c->cfg.G, c->ax,
                                              c->ay, total ? c->tot : nullptr, c->counters,
                                              batch_of(c, variant_ns(c->variant) == 1 ? 0 : 1));
     phase_end(c, 5);
@@ -1925,6 +1966,26 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
     if (st == BH_OK && total) st = hip_check(c, hipMemcpyAsync(c->h_dsum, c->dsum, 8, hipMemcpyDeviceToHost, s), "total");
     if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "learned forces");
     if (st == BH_OK && total) *total = *c->h_dsum;
+    return st;
+}
+
+bh_status bh_learned_export(bh_ctx* c, const bh_mlp* m, float* features, float* coef,
+                            uint32_t n_nodes) {
+    if (!c || (!features && !coef) || (coef && !m)) return BH_ERR_ARG;
+    if (!c->learned || !c->built) return fail(c, BH_ERR_STATE, "enable the learned mode and build first");
+    hipStream_t s = c->stream;
+    bh_status st = coef ? ensure_coefs(c, m) : ensure_features(c);
+    uint32_t nn = 0;
+    if (st == BH_OK) st = hip_check(c, hipMemcpyAsync(&c->h_prm->n_nodes, &c->prm->n_nodes, 4, hipMemcpyDeviceToHost, s), "nodes");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "nodes");
+    if (st == BH_OK) nn = c->h_prm->n_nodes;
+    if (st == BH_OK && n_nodes != nn)
+        return fail(c, BH_ERR_ARG, "the tree has %u nodes, %u requested", nn, n_nodes);
+    if (st == BH_OK && features)
+        st = hip_check(c, hipMemcpyAsync(features, c->lfeat, (size_t)nn * kMlpIn * 4, hipMemcpyDeviceToHost, s), "features");
+    if (st == BH_OK && coef)
+        st = hip_check(c, hipMemcpyAsync(coef, c->lcoef, (size_t)nn * kCoefStride * 4, hipMemcpyDeviceToHost, s), "coefficients");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "export");
     return st;
 }
 

@@ -196,14 +196,22 @@ void launch_collect_pairs(hipStream_t s, const float4* pos4, const Node* nodes,
                           uint32_t* counts);
 void launch_sinks_sorted(hipStream_t s, const uint32_t* id, uint32_t n, const uint32_t* sinks_caller,
                          uint32_t ns, uint32_t* where, uint32_t* sinks_sorted);
+// the walk with the learned correction from the per-node coefficient records
 uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
-                                 const float4* m2, const BuildParams* prm, uint32_t n,
+                                 const float4* coef, const BuildParams* prm, uint32_t n,
                                  float inv_theta2, float inv_band2, float eps2, float scale,
-                                 const MlpBf16* w, float* ax, float* ay, float* tot,
+                                 float* ax, float* ay, float* tot,
                                  unsigned long long* counters, TraverseBatch batch);
-void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
-                        const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
+void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float* feat,
+                        const BuildParams* prm, const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
                         const uint32_t* offsets, uint32_t ns, uint32_t cap, double eps2, float* X,
                         float* T, uint32_t total);
+// cell features (kMlpIn per node, bh_mlp.h) of the last build and the network's coefficient
+// records (kCoefStride floats per node) from them
+void launch_cell_features(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
+                          const void* codes, bool wide, int B, const BuildParams* prm,
+                          uint32_t cap_nodes, float* feat);
+void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
+                       uint32_t cap_nodes, const MlpBf16* w, float4* coef);
 
 }  // namespace bh

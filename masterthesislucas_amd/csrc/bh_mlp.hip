@@ -38,19 +38,55 @@ __global__ __launch_bounds__(64) void k_mlp_prep_bf16(const float* __restrict__ 
         W->w1[u][lane] = v;
     }
     for (int u = 0; u < 4; ++u)
-        for (int k = 0; k < 4; ++k) {
-            bf16x4 v;
-            for (int i = 0; i < 4; ++i) v[i] = f32_to_bf16(W2[(16 * k + 4 * q4 + i) * kMlpH + 16 * u + r16]);
-            W->w2[u][k][lane] = v;
+        for (int c = 0; c < 2; ++c) {
+            bf16x8 v;
+            for (int j = 0; j < 8; ++j)
+                v[j] = (__bf16)W2[mlp_unit_of(c, 8 * q4 + j) * kMlpH + 16 * u + r16];
+            W->w2[u][c][lane] = v;
         }
-    for (int k = 0; k < 4; ++k) {
-        bf16x4 v;
-        for (int i = 0; i < 4; ++i)
-            v[i] = r16 < kMlpOut ? f32_to_bf16(W3[(16 * k + 4 * q4 + i) * kMlpOut + r16]) : (short)0;
-        W->w3[k][lane] = v;
+    for (int c = 0; c < 2; ++c) {
+        bf16x8 v;
+        for (int j = 0; j < 8; ++j)
+            v[j] = (__bf16)(r16 < kMlpOut ? W3[mlp_unit_of(c, 8 * q4 + j) * kMlpOut + r16] : 0.f);
+        W->w3[c][lane] = v;
     }
     W->b2[lane] = p[kMlpOffB2 + lane];
-    if (lane < kMlpOut) W->b3[lane] = p[kMlpOffB3 + lane];
+    if (lane < 16) W->b3[lane] = lane < kMlpOut ? p[kMlpOffB3 + lane] : 0.f;
+}
+
+// stage the BF16 weights in LDS once per block
+__device__ __forceinline__ void stage_weights(const MlpBf16* __restrict__ W, MlpBf16* ws) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(W);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(ws);
+    for (uint32_t i = threadIdx.x; i < sizeof(MlpBf16) / 4; i += kBlock) dst[i] = src[i];
+}
+
+// one wave: features of 64 rows (stride `stride` floats, the first kMlpIn used) -> the 14
+// outputs of each row in LDS (ys[64][16])
+__device__ __forceinline__ void infer_rows(const MlpBf16* ws, const float* __restrict__ X,
+                                           uint32_t base, uint32_t n, uint32_t stride, short* xs,
+                                           float* ys) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t i = base + lane;
+    short* row = xs + lane * kMlpInPad;
+#pragma unroll
+    for (int c = 0; c < kMlpInPad; ++c) {
+        float v = 0.f;
+        if (c < kMlpIn) v = i < n ? X[(size_t)i * stride + c] : 0.f;
+        else if (c == kMlpIn) v = 1.f;
+        row[c] = f32_to_bf16(v);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    f32x4 out[4];
+    mlp_wave_infer(ws, xs, out);
+    const int r16 = lane & 15, q4 = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ys[(16 * s + r16) * 16 + 4 * q4 + e] = out[s][e];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -60,35 +96,43 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer_bf16(const float* __restri
                                                            const MlpBf16* __restrict__ W,
                                                            float* __restrict__ Y) {
     __shared__ short xs[kWavesPerBlock][64 * kMlpInPad];
+    __shared__ float ys[kWavesPerBlock][64 * 16];
     __shared__ MlpBf16 ws;
-    // stage the weights once per block
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(W);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&ws);
-        for (uint32_t i = threadIdx.x; i < sizeof(MlpBf16) / 4; i += kBlock) dst[i] = src[i];
-    }
+    stage_weights(W, &ws);
+    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t base = (blockIdx.x * kWavesPerBlock + w) * 64u;
+    infer_rows(&ws, X, base, n, kMlpIn, xs[w], ys[w]);
     const uint32_t i = base + lane;
-    short* row = &xs[w][lane * kMlpInPad];
-#pragma unroll
-    for (int c = 0; c < kMlpInPad; ++c) {
-        float v = 0.f;
-        if (c < kMlpIn) v = i < n ? X[(size_t)i * kMlpIn + c] : 0.f;
-        else if (c == kMlpIn) v = 1.f;
-        row[c] = f32_to_bf16(v);
-    }
+    if (i < n)
+        for (int o = 0; o < kMlpOut; ++o) Y[(size_t)i * kMlpOut + o] = ys[w][lane * 16 + o];
+}
+
+// per node: the network on its features -> the walk's coefficient record
+__global__ __launch_bounds__(kBlock) void k_cell_coefs(const float* __restrict__ feat,
+                                                       const BuildParams* __restrict__ prm,
+                                                       uint32_t cap_nodes,
+                                                       const MlpBf16* __restrict__ W,
+                                                       float4* __restrict__ coef) {
+    __shared__ short xs[kWavesPerBlock][64 * kMlpInPad];
+    __shared__ float ys[kWavesPerBlock][64 * 16];
+    __shared__ MlpBf16 ws;
+    const uint32_t nn = min(prm->n_nodes, cap_nodes);
+    if (blockIdx.x * (64u * kWavesPerBlock) >= nn) return;  // whole block: past the last node
+    stage_weights(W, &ws);
     __syncthreads();
-    float o0[4], o1[4];
-    mlp_wave_infer(&ws, xs[w], o0, o1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * kWavesPerBlock + w) * 64u;
+    if (base >= nn) return;  // wave-uniform
+    infer_rows(&ws, feat, base, nn, kMlpIn, xs[w], ys[w]);
+    const uint32_t k = base + lane;
+    if (k >= nn) return;
+    float r[kCoefStride];
+    learned_record(&ys[w][lane * 16], r);
+    float4* dst = coef + (size_t)k * (kCoefStride / 4);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const uint32_t j = base + 16 * s + lane;
-        if (lane < 16 && j < n) {
-            Y[(size_t)j * 2] = o0[s];
-            Y[(size_t)j * 2 + 1] = o1[s];
-        }
-    }
+    for (int q = 0; q < kCoefStride / 4; ++q)
+        dst[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
 }
 
 __device__ __forceinline__ float silu_f(float v) { return v / (1.0f + __expf(-v)); }
@@ -102,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward_f32(const float* __restr
                                                             float* __restrict__ Y) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    float x[kMlpIn], a1[kMlpH];
+    float x[kMlpIn], a1[kMlpH], y[kMlpOut];
 #pragma unroll
     for (int c = 0; c < kMlpIn; ++c) x[c] = X[(size_t)i * kMlpIn + c];
 #pragma unroll
@@ -112,25 +156,45 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward_f32(const float* __restr
         for (int c = 0; c < kMlpIn; ++c) z = fmaf(x[c], p[kMlpOffW1 + c * kMlpH + j], z);
         a1[j] = silu_f(z);
     }
-    float y0 = p[kMlpOffB3], y1 = p[kMlpOffB3 + 1];
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o) y[o] = p[kMlpOffB3 + o];
     for (int j = 0; j < kMlpH; ++j) {
         float z = p[kMlpOffB2 + j];
 #pragma unroll
         for (int k = 0; k < kMlpH; ++k) z = fmaf(a1[k], p[kMlpOffW2 + k * kMlpH + j], z);
         const float a2 = silu_f(z);
-        y0 = fmaf(a2, p[kMlpOffW3 + j * 2], y0);
-        y1 = fmaf(a2, p[kMlpOffW3 + j * 2 + 1], y1);
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[o] = fmaf(a2, p[kMlpOffW3 + j * kMlpOut + o], y[o]);
     }
-    Y[(size_t)i * 2] = y0;
-    Y[(size_t)i * 2 + 1] = y1;
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o) Y[(size_t)i * kMlpOut + o] = y[o];
+}
+
+// basis of the pair offset (complex, interleaved re/im): w, conj w, w^3, q, q w^2, q conj(w)^2,
+// q w^4
+__device__ __forceinline__ void learned_basis_f(float ux, float uy, float q, float* b) {
+    const float w2r = fmaf(ux, ux, -uy * uy), w2i = 2.0f * ux * uy;
+    const float w3r = fmaf(w2r, ux, -w2i * uy), w3i = fmaf(w2r, uy, w2i * ux);
+    const float w4r = fmaf(w2r, w2r, -w2i * w2i), w4i = 2.0f * w2r * w2i;
+    b[0] = ux; b[1] = uy;
+    b[2] = ux; b[3] = -uy;
+    b[4] = w3r; b[5] = w3i;
+    b[6] = q; b[7] = 0.f;
+    b[8] = q * w2r; b[9] = q * w2i;
+    b[10] = q * w2r; b[11] = -q * w2i;
+    b[12] = q * w4r; b[13] = q * w4i;
 }
 
 // ---------------------------------------------------------------------------------------------
 // training: per-sample forward + backward to the pre-activation gradients, written
-// feature-major to scratch S (rows of length B): A1[64] DZ1[64] A2[64] DZ2[64] DY[2] X[10]
+// feature-major to scratch S (rows of length B): A1[64] DZ1[64] A2[64] DZ2[64] DY[14] X[10].
+// A sample is a row of kLearnSample floats (10 cell features, w_x, w_y, q) and a label
+// T[2]; its prediction is p = sum_j c_j b_j (complex) with c = the network's 14 outputs and b
+// the basis of (w, q); loss = sum |p - T|^2 / (2 B).
 // ---------------------------------------------------------------------------------------------
-constexpr int kRowA1 = 0, kRowDZ1 = 64, kRowA2 = 128, kRowDZ2 = 192, kRowDY = 256, kRowX = 258;
-constexpr int kScratchRows = 268;
+constexpr int kRowA1 = 0, kRowDZ1 = 64, kRowA2 = 128, kRowDZ2 = 192, kRowDY = 256;
+constexpr int kRowX = kRowDY + kMlpOut;
+constexpr int kScratchRows = kRowX + kMlpIn;
 
 __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__ X,
                                                        const float* __restrict__ T,
@@ -143,10 +207,11 @@ __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__
     double sq = 0.0;
     if (i < B) {
         const uint32_t src = order ? order[i] : i;
+        const float* row = X + (size_t)src * kLearnSample;
         float x[kMlpIn];
 #pragma unroll
         for (int c = 0; c < kMlpIn; ++c) {
-            x[c] = X[(size_t)src * kMlpIn + c];
+            x[c] = row[c];
             S[(size_t)(kRowX + c) * B + i] = x[c];
         }
         float a1[kMlpH];
@@ -159,7 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__
             S[(size_t)(kRowA1 + j) * B + i] = a1[j];
             S[(size_t)(kRowDZ1 + j) * B + i] = dsilu_f(z);  // silu'(z1), scaled below
         }
-        float y0 = p[kMlpOffB3], y1 = p[kMlpOffB3 + 1];
+        float y[kMlpOut];
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[o] = p[kMlpOffB3 + o];
         for (int j = 0; j < kMlpH; ++j) {
             float z = p[kMlpOffB2 + j];
 #pragma unroll
@@ -167,19 +234,36 @@ __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__
             const float a2 = silu_f(z);
             S[(size_t)(kRowA2 + j) * B + i] = a2;
             S[(size_t)(kRowDZ2 + j) * B + i] = dsilu_f(z);  // silu'(z2), scaled below
-            y0 = fmaf(a2, p[kMlpOffW3 + j * 2], y0);
-            y1 = fmaf(a2, p[kMlpOffW3 + j * 2 + 1], y1);
+#pragma unroll
+            for (int o = 0; o < kMlpOut; ++o) y[o] = fmaf(a2, p[kMlpOffW3 + j * kMlpOut + o], y[o]);
         }
-        const float e0 = y0 - T[(size_t)src * 2], e1 = y1 - T[(size_t)src * 2 + 1];
-        sq = (double)e0 * e0 + (double)e1 * e1;
-        // loss = sum(e^2) / (2 B): dL/dy = e / B
-        const float dy0 = e0 / (float)B, dy1 = e1 / (float)B;
-        S[(size_t)kRowDY * B + i] = dy0;
-        S[(size_t)(kRowDY + 1) * B + i] = dy1;
+        float bas[kMlpOut];
+        learned_basis_f(row[kMlpIn], row[kMlpIn + 1], row[kMlpIn + 2], bas);
+        float pr = 0.f, pi = 0.f;
+#pragma unroll
+        for (int j = 0; j < kLearnTerms; ++j) {
+            pr = fmaf(y[2 * j], bas[2 * j], fmaf(-y[2 * j + 1], bas[2 * j + 1], pr));
+            pi = fmaf(y[2 * j], bas[2 * j + 1], fmaf(y[2 * j + 1], bas[2 * j], pi));
+        }
+        const float ex = pr - T[(size_t)src * 2], ey = pi - T[(size_t)src * 2 + 1];
+        sq = (double)ex * ex + (double)ey * ey;
+        // loss = sum |e|^2 / (2 B): dL/dRe c_j = Re(conj(e) b_j) / B, dL/dIm c_j = Im(conj(b_j) e)/B
+        const float inv_b = 1.0f / (float)B;
+        float dy[kMlpOut];
+#pragma unroll
+        for (int j = 0; j < kLearnTerms; ++j) {
+            const float br = bas[2 * j], bi = bas[2 * j + 1];
+            dy[2 * j] = fmaf(ex, br, ey * bi) * inv_b;
+            dy[2 * j + 1] = fmaf(ey, br, -ex * bi) * inv_b;
+            S[(size_t)(kRowDY + 2 * j) * B + i] = dy[2 * j];
+            S[(size_t)(kRowDY + 2 * j + 1) * B + i] = dy[2 * j + 1];
+        }
         float dz2[kMlpH];
 #pragma unroll
         for (int j = 0; j < kMlpH; ++j) {
-            const float da2 = fmaf(dy0, p[kMlpOffW3 + j * 2], dy1 * p[kMlpOffW3 + j * 2 + 1]);
+            float da2 = 0.f;
+#pragma unroll
+            for (int o = 0; o < kMlpOut; ++o) da2 = fmaf(dy[o], p[kMlpOffW3 + j * kMlpOut + o], da2);
             dz2[j] = da2 * S[(size_t)(kRowDZ2 + j) * B + i];
             S[(size_t)(kRowDZ2 + j) * B + i] = dz2[j];
         }
@@ -307,10 +391,12 @@ __global__ __launch_bounds__(kBlock) void k_collect_pairs(const float4* __restri
 }
 
 // One wave per pair (slot t, c-th cell): FP64 sum over the cell's particles (sink excluded by
-// index) minus the FP64 monopole, features in FP32.
+// index) minus the FP64 monopole, in units of M l^2 / r^4; the sample row holds the cell's
+// features and the pair's offset features w = d / r, q = l / r (d = c - p).
 __global__ __launch_bounds__(kBlock) void k_label_pairs(const float4* __restrict__ pos4,
                                                         const Node* __restrict__ nodes,
-                                                        const float4* __restrict__ m2,
+                                                        const float* __restrict__ feat,
+                                                        const BuildParams* __restrict__ prm,
                                                         const uint32_t* __restrict__ sinks,
                                                         const uint32_t* __restrict__ pairs,
                                                         const uint32_t* __restrict__ counts,
@@ -353,16 +439,86 @@ __global__ __launch_bounds__(kBlock) void k_label_pairs(const float4* __restrict
     const double r2m = dx * dx + dy * dy;
     const double invm = 1.0 / sqrt(r2m + eps2);
     const double sm = (double)nd.mass * invm * invm * invm;
-    const double rx = sx - sm * dx, ry = sy - sm * dy;
-    float f[kMlpIn];
-    const float4 mm = m2[k];
-    const float fdx = nd.cx - me.x, fdy = nd.cy - me.y;
-    const float scale = mlp_features(fdx, fdy, fmaf(fdx, fdx, fdy * fdy), nd.l2, nd.mass, nd.count,
-                                     mm.x, mm.y, mm.z, f);
+    // the cell side as the walk takes it: the FP32 root side scaled by 2^-level (exact)
+    const double l = ldexp((double)prm->side, -(int)(nd.meta & kMetaLevelMask));
+    const double rm = sqrt(r2m);
+    const double scale = (double)nd.mass * l * l / (r2m * r2m);
+    float* row = Xo + (size_t)gw * kLearnSample;
 #pragma unroll
-    for (int q = 0; q < kMlpIn; ++q) Xo[(size_t)gw * kMlpIn + q] = f[q];
-    To[(size_t)gw * 2] = (float)(rx / (double)scale);
-    To[(size_t)gw * 2 + 1] = (float)(ry / (double)scale);
+    for (int q = 0; q < kMlpIn; ++q) row[q] = feat[(size_t)k * kMlpIn + q];
+    row[kMlpIn] = (float)(dx / rm);
+    row[kMlpIn + 1] = (float)(dy / rm);
+    row[kMlpIn + 2] = (float)(l / rm);
+    To[(size_t)gw * 2] = (float)((sx - sm * dx) / scale);
+    To[(size_t)gw * 2 + 1] = (float)((sy - sm * dy) / scale);
+}
+
+// even (x) and odd (y) bits of a Morton prefix, compacted
+__device__ __forceinline__ uint64_t compact_even(uint64_t v) {
+    v &= 0x5555555555555555ull;
+    v = (v | (v >> 1)) & 0x3333333333333333ull;
+    v = (v | (v >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v >> 4)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v >> 8)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v >> 16)) & 0x00000000FFFFFFFFull;
+    return v;
+}
+
+__device__ __forceinline__ double f64_of_bits(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// One wave per node: the cell features (see bh_mlp.h). Second moments come from the upward pass
+// (m2); third moments are summed here about the node's centre of mass over its particles (cells
+// of at most kLearnT3Max particles); the box centre follows from the Morton prefix of the
+// node's first particle (B code bits per axis) and the root square, in FP64.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_cell_features(const float4* __restrict__ pos4,
+                                                          const Node* __restrict__ nodes,
+                                                          const float4* __restrict__ m2,
+                                                          const K* __restrict__ codes, int B,
+                                                          const BuildParams* __restrict__ prm,
+                                                          uint32_t cap_nodes,
+                                                          float* __restrict__ feat) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (k >= min(prm->n_nodes, cap_nodes)) return;
+    const Node nd = nodes[k];
+    const float M = nd.mass, cx = nd.cx, cy = nd.cy;
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+    if (nd.count <= kLearnT3Max) {
+        for (uint32_t j = nd.first + lane; j < nd.first + nd.count; j += kWave) {
+            const float4 q = pos4[j];
+            const float u = q.x - cx, v = q.y - cy;
+            const float mu = q.z * u, mv = q.z * v;
+            t0 = fmaf(mu * u, u, t0);
+            t1 = fmaf(mu * u, v, t1);
+            t2 = fmaf(mu * v, v, t2);
+            t3 = fmaf(mv * v, v, t3);
+        }
+        t0 = wave_sum_f(t0);
+        t1 = wave_sum_f(t1);
+        t2 = wave_sum_f(t2);
+        t3 = wave_sum_f(t3);
+    }
+    if (lane != 0) return;
+    const int L = (int)(nd.meta & kMetaLevelMask);
+    const uint64_t pre = L > 0 ? (uint64_t)codes[nd.first] >> (2 * (B - L)) : 0ull;
+    const double l64 = ldexp(f64_of_bits(prm->side64), -L);
+    const double bx = f64_of_bits(prm->minx64) + ((double)compact_even(pre) + 0.5) * l64;
+    const double by = f64_of_bits(prm->miny64) + ((double)compact_even(pre >> 1) + 0.5) * l64;
+    const float l = (float)l64;
+    const float4 s2 = m2[k];
+    const float a = 1.0f / (M * l * l), b = a / l;
+    float* o = feat + (size_t)k * kMlpIn;
+    o[0] = s2.x * a;
+    o[1] = s2.y * a;
+    o[2] = s2.z * a;
+    o[3] = t0 * b;
+    o[4] = t1 * b;
+    o[5] = t2 * b;
+    o[6] = t3 * b;
+    o[7] = (float)(((double)cx - bx) / l64);
+    o[8] = (float)(((double)cy - by) / l64);
+    o[9] = (float)(log2((double)nd.count) / 16.0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -403,13 +559,29 @@ void launch_collect_pairs(hipStream_t s, const float4* pos4, const Node* nodes,
                                                             inv_theta2, inv_band2, cap, pairs,
                                                             counts);
 }
-void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
-                        const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
+void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float* feat,
+                        const BuildParams* prm, const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
                         const uint32_t* offsets, uint32_t ns, uint32_t cap, double eps2, float* X,
                         float* T, uint32_t total) {
     if (!total) return;
     k_label_pairs<<<div_up_u(total, kWavesPerBlock), kBlock, 0, s>>>(
-        pos4, nodes, m2, sinks, pairs, counts, offsets, ns, cap, eps2, X, T, total);
+        pos4, nodes, feat, prm, sinks, pairs, counts, offsets, ns, cap, eps2, X, T, total);
+}
+void launch_cell_features(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
+                          const void* codes, bool wide, int B, const BuildParams* prm,
+                          uint32_t cap_nodes, float* feat) {
+    const uint32_t g = div_up_u(cap_nodes, kWavesPerBlock);
+    if (wide)
+        k_cell_features<uint64_t><<<g, kBlock, 0, s>>>(pos4, nodes, m2, (const uint64_t*)codes, B,
+                                                       prm, cap_nodes, feat);
+    else
+        k_cell_features<uint32_t><<<g, kBlock, 0, s>>>(pos4, nodes, m2, (const uint32_t*)codes, B,
+                                                       prm, cap_nodes, feat);
+}
+void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
+                       uint32_t cap_nodes, const MlpBf16* w, float4* coef) {
+    k_cell_coefs<<<div_up_u(cap_nodes, 64 * kWavesPerBlock), kBlock, 0, s>>>(feat, prm, cap_nodes,
+                                                                            w, coef);
 }
 
 }  // namespace bh

@@ -60,6 +60,9 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane(v);
 }
+__device__ __forceinline__ float uniform_f32(float v) {
+    return __uint_as_float(uniform_u32(__float_as_uint(v)));
+}
 
 // Node record as raw bits (cx, cy, mass, l2 | skip, first, count, meta), fetched with scalar
 // loads at a wave-uniform index.
@@ -421,33 +424,19 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
 // ------------------------------------------------------------------------------------------
 // Traversal with the learned far-field correction (gravity). The walk and every acceptance
 // decision are those of k_traverse; in addition, each accepted cell whose l / r is at least
-// `band` (all accepted cells when band = 0) gets its monopole corrected by the MLP. Lanes that
-// take such a cell write its features to a per-wave LDS ring of BF16 rows; whenever 64 rows
-// are pending, the wave evaluates the MLP on them with MFMA (mlp_wave_infer) and adds each
-// correction to its owner's monopole batch, walking the 64 results in ring order so the sum
-// order is fixed.
+// `band` (all accepted cells when band = 0) gets its monopole corrected by
+// (M l^2 / r^4) sum_j c_j b_j(d / r, l / r), with the cell's coefficient record c (computed once
+// per cell and build by the network, kCoefStride floats) read by the wave with scalar loads.
 // ------------------------------------------------------------------------------------------
 template <int NS, bool TOT>
 __global__ __launch_bounds__(kBlock) void k_traverse_learned(
     const float4* __restrict__ pos4, const Node* __restrict__ nodes,
-    const float4* __restrict__ m2, const BuildParams* __restrict__ prm, uint32_t n,
-    float inv_theta2, float inv_band2, float eps2, float scale, const MlpBf16* __restrict__ wg,
-    float* __restrict__ ax_out, float* __restrict__ ay_out, float* __restrict__ tot_out,
+    const float4* __restrict__ coef, const BuildParams* __restrict__ prm, uint32_t n,
+    float inv_theta2, float inv_band2, float eps2, float scale, float* __restrict__ ax_out,
+    float* __restrict__ ay_out, float* __restrict__ tot_out,
     unsigned long long* __restrict__ counters, TraverseBatch batch) {
-    constexpr int kRing = 64 * (NS + 1);  // pending < 64 before a visit adds at most 64 NS
-    __shared__ MlpBf16 ws;
-    __shared__ __attribute__((aligned(16))) short q_x[kWavesPerBlock][kRing * kMlpInPad];
-    __shared__ float q_scale[kWavesPerBlock][kRing];
-    __shared__ uint16_t q_owner[kWavesPerBlock][kRing];
-    __shared__ float2 q_out[kWavesPerBlock][64];
     __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
     __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&ws);
-        for (uint32_t i = threadIdx.x; i < sizeof(MlpBf16) / 4; i += kBlock) dst[i] = src[i];
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
@@ -470,47 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse_learned(
     for (int s = 0; s < NS; ++s) ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] =
         by[s] = bt[s] = 0.f;
     uint64_t cnt_pn = 0, cnt_pp = 0;
-    uint32_t q_head = 0, q_tail = 0;  // ring positions (uniform)
-    short* xw = q_x[w];
-
-    // evaluate the 64 rows at q_head (all pending rows when `count` < 64; the rest are padding)
-    auto flush = [&](uint32_t count) {
-        const uint32_t h = q_head % kRing;
-        if (count < 64) {  // zero the padding rows so the MFMA reads finite values
-            for (uint32_t e = count; e < 64; ++e) {
-                if (lane < kMlpInPad) xw[(h + e) * kMlpInPad + lane] = 0;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        float o0[4], o1[4];
-        mlp_wave_infer(&ws, xw + h * kMlpInPad, o0, o1);
-#pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
-            const uint32_t e = 16 * sb + lane;
-            if (lane < 16) {
-                const float sc = q_scale[w][h + e];
-                q_out[w][e] = make_float2(o0[sb] * sc, o1[sb] * sc);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t e = 0; e < count; ++e) {
-            const uint32_t o = uniform_u32(q_owner[w][h + e]);
-            const float2 v = q_out[w][e];
-            if ((int)(o & 63u) == lane) {
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    if ((int)(o >> 6) == s) {
-                        bx[s] += v.x;
-                        by[s] += v.y;
-                    }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        q_head += 64;
-    };
-
+    const float side = uniform_f32(prm->side);
     const uint32_t last = uniform_u32(prm->n_nodes) - 1;
     uint32_t k = k0, end_cur = end0;
     int sp = 0, since_fold = 0;
@@ -548,29 +497,24 @@ __global__ __launch_bounds__(kBlock) void k_traverse_learned(
                 const uint64_t bandm =
                     uniform_u64(inv_band2 > 0.f ? (accm & __ballot(!(d2 > lim_band))) : accm);
                 if (bandm) {
-                    const float4 mm = m2[k];
+                    const float4* cr = coef + (size_t)k * (kCoefStride / 4);
+                    const float4 c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
                     if (__builtin_amdgcn_inverse_ballot_w64(bandm)) {
-                        float f[kMlpIn];
-                        const float sc = mlp_features(dx, dy, d2, cur.l2(), cur.mass(),
-                                                      cur.count(), mm.x, mm.y, mm.z, f);
-                        const uint32_t pos =
-                            (q_tail + (uint32_t)__popcll(bandm & lane_mask_lt())) % kRing;
-                        bf16x4* row = reinterpret_cast<bf16x4*>(xw + pos * kMlpInPad);
-                        row[0] = pack_bf16x4(f[0], f[1], f[2], f[3]);
-                        row[1] = pack_bf16x4(f[4], f[5], f[6], f[7]);
-                        row[2] = pack_bf16x4(f[8], f[9], 1.0f, 0.f);
-                        row[3] = pack_bf16x4(0.f, 0.f, 0.f, 0.f);
-                        q_scale[w][pos] = sc * scale;
-                        q_owner[w][pos] = (uint16_t)(s * 64 + lane);
+                        const float rec[kCoefStride] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y,
+                                                        c1.z, c1.w, c2.x, c2.y, c2.z, c2.w,
+                                                        c3.x, c3.y, c3.z, c3.w};
+                        const float l = ldexpf(side, -(int)(meta & kMetaLevelMask));
+                        float gx, gy;
+                        learned_correction(dx, dy, d2, l, cur.mass() * l * l, rec, gx, gy);
+                        bx[s] += gx;
+                        by[s] += gy;
                     }
-                    q_tail += (uint32_t)__popcll(bandm);
                 }
             }
             openm[s] = onm[s] & ~accm;
             open_any |= openm[s];
             acc_any |= accm;
         }
-        while (q_tail - q_head >= 64) flush(64);
         const bool is_pp = (meta & kMetaPP) != 0;
         const bool down = open_any && !is_pp;
         const uint32_t knext = uniform_u32(down ? k + 1 : next);
@@ -631,7 +575,6 @@ __global__ __launch_bounds__(kBlock) void k_traverse_learned(
         asm volatile("; prefetched %0" ::"s"(touch));
         k = knext;
     }
-    if (q_tail != q_head) flush(q_tail - q_head);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         kahan_add(ax[s], cax[s], bx[s]);
@@ -896,16 +839,16 @@ uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* p
 }
 
 uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
-                                 const float4* m2, const BuildParams* prm, uint32_t n,
+                                 const float4* coef, const BuildParams* prm, uint32_t n,
                                  float inv_theta2, float inv_band2, float eps2, float scale,
-                                 const MlpBf16* w, float* ax, float* ay, float* tot,
+                                 float* ax, float* ay, float* tot,
                                  unsigned long long* counters, TraverseBatch batch) {
     const int ns = variant_ns(variant) == 1 ? 1 : 2;
     const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
                                   : div_up(n, (uint64_t)kBlock * ns);
 #define BH_TRAVL(NS, TOT)                                                                         \
-    k_traverse_learned<NS, TOT><<<g, kBlock, 0, s>>>(pos4, nodes, m2, prm, n, inv_theta2,        \
-                                                      inv_band2, eps2, scale, w, ax, ay, tot,     \
+    k_traverse_learned<NS, TOT><<<g, kBlock, 0, s>>>(pos4, nodes, coef, prm, n, inv_theta2,      \
+                                                      inv_band2, eps2, scale, ax, ay, tot,        \
                                                       counters, batch)
     if (ns == 1) {
         if (tot) BH_TRAVL(1, true); else BH_TRAVL(1, false);

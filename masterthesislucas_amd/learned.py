@@ -1,11 +1,14 @@
 """Learned far-field correction: the network object and the data-set / training / inference
 calls of the C ABI (include/bh.h, bh_mlp_* and bh_learned_*).
 
-No counterpart in the reference. The network is a 10 -> 64 -> 64 -> 2 SiLU MLP that predicts
-the residual between the exact sum over an accepted cell's particles and the cell's monopole
-(features and units: masterthesislucas_amd/csrc/bh_mlp.h). `FarFieldMLP.collect` labels pairs
-taken from a built gravity context by FP64 direct sums over the cells, `train` runs Adam on the
-GPU, and `BHContext.forces_learned` applies the network inside the traversal (BF16 MFMA).
+No counterpart in the reference. The network is a 10 -> 64 -> 64 -> 14 SiLU MLP evaluated once
+per cell and build: from the cell's features (normalised second and third moments, centre of
+mass in its box, size) it gives 7 complex coefficients c_j, and every accepted (sink, cell) pair
+gets (M l^2 / r^4) sum_j c_j b_j(d / r, l / r) added to its monopole, b the quadrupole and
+octupole angular forms (definitions in masterthesislucas_amd/csrc/bh_mlp.h).
+`FarFieldMLP.collect` labels pairs taken from a built gravity context by FP64 direct sums over
+the cells, `train` runs Adam on the GPU, and `BHContext.forces_learned` computes the per-cell
+coefficients (BF16 MFMA) and applies them inside the traversal.
 """
 from __future__ import annotations
 
@@ -16,7 +19,9 @@ import numpy as np
 from . import _lib
 from .solver import BHContext, _ptr
 
-N_IN, N_HIDDEN, N_OUT = 10, 64, 2
+N_IN, N_HIDDEN, N_OUT = 10, 64, 14
+N_SAMPLE = N_IN + 3  # data set row: cell features, w_x, w_y, q
+N_LABEL = 2
 
 
 def param_layout():
@@ -85,12 +90,14 @@ class FarFieldMLP:
 
     # compute ----------------------------------------------------------------------------
     def forward(self, X, mfma=False):
+        """Coefficients [n x 14] of cell features X [n x 10]."""
         X = np.ascontiguousarray(X, np.float32)
         Y = np.empty((X.shape[0], N_OUT), np.float32)
         self._c(_lib.load().bh_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y), int(mfma)))
         return Y
 
     def loss_grad(self, X, T):
+        """Loss and gradient over samples X [n x 13] with labels T [n x 2]."""
         X = np.ascontiguousarray(X, np.float32)
         T = np.ascontiguousarray(T, np.float32)
         g = np.empty(self.count, np.float32)
@@ -111,7 +118,7 @@ class FarFieldMLP:
 
     def dataset(self):
         n = self.dataset_size()
-        X, T = np.empty((n, N_IN), np.float32), np.empty((n, N_OUT), np.float32)
+        X, T = np.empty((n, N_SAMPLE), np.float32), np.empty((n, N_LABEL), np.float32)
         self._c(_lib.load().bh_mlp_dataset_get(self._h, _ptr(X), _ptr(T), n))
         return X, T
 

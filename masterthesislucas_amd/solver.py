@@ -156,6 +156,16 @@ class BHContext:
                                               C.byref(tot) if want_total else None))
         return ax, ay, tot.value
 
+    def learned_export(self, mlp=None):
+        """The current tree's cell features [n_nodes x 10] and, with a network, the walk's
+        coefficient records [n_nodes x 16] (bh_learned_export); (features, coef or None)."""
+        nn, _, _ = self.tree_info()
+        f = np.empty((nn, 10), np.float32)
+        c = np.empty((nn, 16), np.float32) if mlp is not None else None
+        self._c(_lib.load().bh_learned_export(self._h, mlp.handle if mlp is not None else None,
+                                              _ptr(f), _ptr(c) if c is not None else None, nn))
+        return f, c
+
     def leapfrog(self, steps, dt, theta):
         self._c(_lib.load().bh_leapfrog(self._h, int(steps), float(dt), float(theta)))
 

@@ -1317,6 +1317,34 @@ void orc_mlp_forward(const double* X, int64_t B, int din, int h, int dout, const
     free(buf);
 }
 
+/* backward of one sample with output gradient dy, accumulated into the parameter gradients */
+static void mlp_bwd_one(const double* xin, int din, int h, int dout, const double* W2,
+                        const double* W3, const double* z1, const double* a1, const double* z2,
+                        const double* a2, const double* dy, double* d1, double* d2, double* gW1,
+                        double* gb1, double* gW2, double* gb2, double* gW3, double* gb3) {
+    for (int o = 0; o < dout; ++o) gb3[o] += dy[o];
+    for (int i = 0; i < h; ++i) {
+        double s = 0.0;
+        for (int o = 0; o < dout; ++o) {
+            gW3[i * dout + o] += a2[i] * dy[o];
+            s += W3[i * dout + o] * dy[o];
+        }
+        d2[i] = s * dsilu(z2[i]);
+        gb2[i] += d2[i];
+    }
+    for (int i = 0; i < h; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < h; ++j) {
+            gW2[i * h + j] += a1[i] * d2[j];
+            s += W2[i * h + j] * d2[j];
+        }
+        d1[i] = s * dsilu(z1[i]);
+        gb1[i] += d1[i];
+    }
+    for (int i = 0; i < din; ++i)
+        for (int j = 0; j < h; ++j) gW1[i * h + j] += xin[i] * d1[j];
+}
+
 double orc_mlp_loss_grad(const double* X, const double* T, int64_t B, int din, int h, int dout,
                          const double* W1, const double* b1, const double* W2, const double* b2,
                          const double* W3, const double* b3, double* gW1, double* gb1,
@@ -1618,4 +1646,210 @@ double orc_fmm_sym(const double* xs, const double* ys, int64_t n, int depth,
     free(c.c2);
     free(c.c3);
     return c.total;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* learned far-field correction (no reference counterpart)                                      */
+/* ------------------------------------------------------------------------------------------ */
+/* Per cell: mass M, centre of mass c, side l, count n; s_j = x_j - c over the cell's particles.
+ * Features (ORC_CELL_FEATURES):
+ *   0..2  S/(M l^2):  sum m s_x s_x, sum m s_x s_y, sum m s_y s_y
+ *   3..6  T/(M l^3):  sum m s_x^3, sum m s_x^2 s_y, sum m s_x s_y^2, sum m s_y^3
+ *   7, 8  (c - box centre) / l, the box of the cell's level-L Morton prefix
+ *   9     log2(n) / 16
+ * Pair (sink p, accepted cell): d = c - p, r = |d|, w = d / r (complex), q = l / r,
+ * scale s = M l^2 / r^4. The correction to the monopole (before G) is s * sum_j c_j b_j with the
+ * network's 7 complex outputs c_j and the basis b = (w, conj w, w^3, q, q w^2, q conj(w)^2,
+ * q w^4): the angular forms of the quadrupole (q^0) and octupole (q^1) terms of the Taylor
+ * expansion of (d + s_j)/|d + s_j|^3 about s = 0. With c_1 = 0.75 tr(S), c_2 = 0.375 Q,
+ * c_3 = 1.875 conj(Q), Q = S_xx - S_yy + 2i S_xy (S normalised as in the features), the
+ * q^0 part is exactly the quadrupole term. */
+static void deinterleave_prefix(uint64_t code, int bits, int level, uint64_t* ix, uint64_t* iy) {
+    const uint64_t pre = level > 0 ? code >> (2 * (bits - level)) : 0;
+    uint64_t x = 0, y = 0;
+    for (int b = 0; b < level; ++b) {
+        x |= ((pre >> (2 * b)) & 1ull) << b;
+        y |= ((pre >> (2 * b + 1)) & 1ull) << b;
+    }
+    *ix = x;
+    *iy = y;
+}
+
+void orc_cell_features(const double* xs, const double* ys, const double* ms, int64_t nnodes,
+                       const int32_t* level, const uint32_t* first, const uint32_t* count,
+                       const uint64_t* codes_sorted, int bits, double minx, double miny,
+                       double root_side, const double* cx, const double* cy, const double* mass,
+                       double* f) {
+    for (int64_t k = 0; k < nnodes; ++k) {
+        const double M = mass[k], l = ldexp(root_side, -level[k]);
+        double sxx = 0, sxy = 0, syy = 0, t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        for (uint32_t j = first[k]; j < first[k] + count[k]; ++j) {
+            const double u = xs[j] - cx[k], v = ys[j] - cy[k], m = ms ? ms[j] : 1.0;
+            sxx += m * u * u;
+            sxy += m * u * v;
+            syy += m * v * v;
+            t0 += m * u * u * u;
+            t1 += m * u * u * v;
+            t2 += m * u * v * v;
+            t3 += m * v * v * v;
+        }
+        if (count[k] > ORC_LEARN_T3_MAX) t0 = t1 = t2 = t3 = 0.0;
+        const double a = 1.0 / (M * l * l), b = a / l;
+        uint64_t ix, iy;
+        deinterleave_prefix(codes_sorted[first[k]], bits, level[k], &ix, &iy);
+        const double bx = minx + ((double)ix + 0.5) * l, by = miny + ((double)iy + 0.5) * l;
+        double* o = f + k * ORC_CELL_FEATURES;
+        o[0] = sxx * a; o[1] = sxy * a; o[2] = syy * a;
+        o[3] = t0 * b; o[4] = t1 * b; o[5] = t2 * b; o[6] = t3 * b;
+        o[7] = (cx[k] - bx) / l;
+        o[8] = (cy[k] - by) / l;
+        o[9] = log2((double)count[k]) / 16.0;
+    }
+}
+
+void orc_learned_basis(double ux, double uy, double q, double* b) {
+    const double w2r = ux * ux - uy * uy, w2i = 2.0 * ux * uy;
+    const double w3r = w2r * ux - w2i * uy, w3i = w2r * uy + w2i * ux;
+    const double w4r = w2r * w2r - w2i * w2i, w4i = 2.0 * w2r * w2i;
+    b[0] = ux; b[1] = uy;            /* w */
+    b[2] = ux; b[3] = -uy;           /* conj w */
+    b[4] = w3r; b[5] = w3i;          /* w^3 */
+    b[6] = q; b[7] = 0.0;            /* q */
+    b[8] = q * w2r; b[9] = q * w2i;  /* q w^2 */
+    b[10] = q * w2r; b[11] = -q * w2i; /* q conj(w)^2 */
+    b[12] = q * w4r; b[13] = q * w4i;  /* q w^4 */
+}
+
+/* complex sum_j c_j b_j of 7 coefficient / basis pairs (re, im interleaved) */
+static void learned_apply(const double* c, const double* b, double* re, double* im) {
+    double x = 0, y = 0;
+    for (int j = 0; j < ORC_LEARNED_TERMS; ++j) {
+        x += c[2 * j] * b[2 * j] - c[2 * j + 1] * b[2 * j + 1];
+        y += c[2 * j] * b[2 * j + 1] + c[2 * j + 1] * b[2 * j];
+    }
+    *re = x;
+    *im = y;
+}
+
+void orc_pair_label(const double* xs, const double* ys, const double* ms, uint32_t first,
+                    uint32_t count, double cx, double cy, double mass, double l, double px,
+                    double py, int64_t self, double eps2, double* out) {
+    double sx = 0, sy = 0;
+    for (uint32_t j = first; j < first + count; ++j) {
+        if ((int64_t)j == self) continue;
+        const double dx = xs[j] - px, dy = ys[j] - py;
+        const double inv = 1.0 / sqrt(dx * dx + dy * dy + eps2);
+        const double w = (ms ? ms[j] : 1.0) * inv * inv * inv;
+        sx += w * dx;
+        sy += w * dy;
+    }
+    const double dx = cx - px, dy = cy - py, r2 = dx * dx + dy * dy;
+    const double inv = 1.0 / sqrt(r2 + eps2);
+    const double w = mass * inv * inv * inv;
+    const double r = sqrt(r2), scale = mass * l * l / (r2 * r2);
+    out[0] = (sx - w * dx) / scale;  /* label: residual in units of M l^2 / r^4 */
+    out[1] = (sy - w * dy) / scale;
+    out[2] = dx / r;                 /* offset features: direction and l / r */
+    out[3] = dy / r;
+    out[4] = l / r;
+    out[5] = scale;
+}
+
+double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h,
+                             const double* W1, const double* b1, const double* W2,
+                             const double* b2, const double* W3, const double* b3, double* gW1,
+                             double* gb1, double* gW2, double* gb2, double* gW3, double* gb3) {
+    const int din = ORC_CELL_FEATURES, dout = 2 * ORC_LEARNED_TERMS, dx_ = din + 3;
+    memset(gW1, 0, sizeof(double) * din * h);
+    memset(gb1, 0, sizeof(double) * h);
+    memset(gW2, 0, sizeof(double) * h * h);
+    memset(gb2, 0, sizeof(double) * h);
+    memset(gW3, 0, sizeof(double) * h * dout);
+    memset(gb3, 0, sizeof(double) * dout);
+    double* buf = (double*)malloc(sizeof(double) * (6 * h + 2 * dout));
+    double *z1 = buf, *a1 = buf + h, *z2 = buf + 2 * h, *a2 = buf + 3 * h, *d2 = buf + 4 * h,
+           *d1 = buf + 5 * h, *y = buf + 6 * h, *dy = buf + 6 * h + dout;
+    double loss = 0.0, bas[2 * ORC_LEARNED_TERMS];
+    for (int64_t s = 0; s < B; ++s) {
+        const double* xin = X + s * dx_;
+        mlp_fwd_one(xin, din, h, dout, W1, b1, W2, b2, W3, b3, z1, a1, z2, a2, y);
+        orc_learned_basis(xin[din], xin[din + 1], xin[din + 2], bas);
+        double pr, pi;
+        learned_apply(y, bas, &pr, &pi);
+        const double ex = pr - T[2 * s], ey = pi - T[2 * s + 1];
+        loss += (ex * ex + ey * ey) / (2.0 * (double)B);
+        for (int j = 0; j < ORC_LEARNED_TERMS; ++j) {
+            const double br = bas[2 * j], bi = bas[2 * j + 1];
+            dy[2 * j] = (ex * br + ey * bi) / (double)B;
+            dy[2 * j + 1] = (ey * br - ex * bi) / (double)B;
+        }
+        mlp_bwd_one(xin, din, h, dout, W2, W3, z1, a1, z2, a2, dy, d1, d2, gW1, gb1, gW2, gb2,
+                    gW3, gb3);
+    }
+    free(buf);
+    return loss;
+}
+
+/* the single-precision-decision walk of orc_bh_traverse_f32 (gravity) with the correction
+ * s * sum_j c_j b_j added to every accepted cell with d2 <= l2 * inv_band2 in single precision
+ * (l/r >= band; inv_band2 = 0: every accepted cell), c_j from coef[k * 14 ..] (FP64); the
+ * correction is evaluated in FP64 from the single-precision node records and positions. */
+void orc_bh_traverse_learned(const float* xs, const float* ys, const float* ms, int64_t n,
+                             const int32_t* level, const uint32_t* first, const uint32_t* count,
+                             const uint32_t* skip, const uint8_t* pp, const float* cx,
+                             const float* cy, const float* mass, int64_t nnodes, float root_side,
+                             float inv_theta2, double G, double eps2, const double* coef,
+                             float inv_band2, const int64_t* sinks, int64_t nsinks,
+                             double* out_ax, double* out_ay, uint64_t* counters) {
+    (void)n;
+    uint64_t pn = 0, ppc = 0, corrected = 0;
+    double bas[2 * ORC_LEARNED_TERMS];
+    for (int64_t s = 0; s < nsinks; ++s) {
+        const int64_t i = sinks ? sinks[s] : s;
+        const float px = xs[i], py = ys[i];
+        double ax = 0, ay = 0, tot = 0;
+        int64_t k = 0;
+        while (k < nnodes) {
+            const float sl = ldexpf(root_side, -level[k]);
+            const float l2 = sl * sl;
+            const float lim = l2 * inv_theta2;
+            const float dx = cx[k] - px, dy = cy[k] - py;
+            const float d2 = fmaf(dx, dx, dy * dy);
+            if (d2 > lim) {
+                interact(ORC_KERNEL_GRAVITY, 1.0, eps2, px, py, cx[k], cy[k], mass[k], &ax, &ay,
+                         &tot);
+                ++pn;
+                const double ddx = (double)cx[k] - px, ddy = (double)cy[k] - py;
+                const double r2 = ddx * ddx + ddy * ddy, r = sqrt(r2), l = (double)sl;
+                if (inv_band2 == 0.0f || !(d2 > l2 * inv_band2)) {
+                    orc_learned_basis(ddx / r, ddy / r, l / r, bas);
+                    double cr, ci;
+                    learned_apply(coef + k * 2 * ORC_LEARNED_TERMS, bas, &cr, &ci);
+                    const double sc = (double)mass[k] * l * l / (r2 * r2);
+                    ax += sc * cr;
+                    ay += sc * ci;
+                    ++corrected;
+                }
+                k = skip[k];
+            } else if (pp[k]) {
+                const int64_t f0 = first[k], e = f0 + count[k];
+                for (int64_t q = f0; q < e; ++q) {
+                    if (q == i) continue;
+                    interact(ORC_KERNEL_GRAVITY, 1.0, eps2, px, py, xs[q], ys[q],
+                             ms ? ms[q] : 1.0, &ax, &ay, &tot);
+                    ++ppc;
+                }
+                k = skip[k];
+            } else {
+                ++k;
+            }
+        }
+        out_ax[s] = G * ax;
+        out_ay[s] = G * ay;
+    }
+    if (counters) {
+        counters[0] = pn;
+        counters[1] = ppc;
+        counters[2] = corrected;
+    }
 }

@@ -24,6 +24,16 @@
  *                        with theta; two far clusters against the exact pair sum
  *   orc_gpuref_codes32   hand cases (centre and boundary points go low, the maximum goes high)
  *                        and a per-point descent by the same comparisons in tests/test_oracle.py
+ *   orc_bh_traverse_f32  strict-threshold hand cases (FP32 expression and both FP64 rules),
+ *                        theta = 0 equals the direct sum, equal to orc_bh_traverse when no
+ *                        decision differs, a hand case where the FP64 rule disagrees
+ *   orc_cell_features    one- and two-particle cells and a box offset in closed form
+ *   orc_learned_basis,   the exact quadrupole coefficients reproduce the exact cell residual
+ *   orc_pair_label       with error O(q^2) on a cluster with zero third moments and O(q)
+ *                        otherwise; a one-particle cell has residual 0
+ *   orc_learned_loss_grad finite differences
+ *   orc_bh_traverse_learned zero coefficients equal orc_bh_traverse_f32; the exact quadrupole
+ *                        coefficients cut the error against the direct sum
  */
 #ifndef BH_ORACLE_H
 #define BH_ORACLE_H
@@ -178,6 +188,46 @@ double orc_mlp_loss_grad(const double* X, const double* T, int64_t B, int din, i
                          double* gW2, double* gb2, double* gW3, double* gb3);
 void orc_adam(double* p, const double* g, double* mom1, double* mom2, int64_t n, double lr,
               double beta1, double beta2, double eps, int64_t t);
+
+/* ---- learned far-field correction (no reference counterpart) ----
+ * Cell features (ORC_CELL_FEATURES per node): mass-weighted second (3) and third (4) moments
+ * about the centre of mass normalised by M l^2 and M l^3, the centre of mass's offset from the
+ * cell's box centre / l (2), log2(count) / 16; the third moments are 0 for cells of more than
+ * ORC_LEARN_T3_MAX particles. The network maps them to ORC_LEARNED_TERMS
+ * complex coefficients c_j of the basis (w, conj w, w^3, q, q w^2, q conj(w)^2, q w^4) of the
+ * pair offset (w = d/|d|, d = com - sink, q = l/|d|); the correction to a monopole is
+ * (M l^2 / |d|^4) sum_j c_j b_j. */
+#define ORC_CELL_FEATURES 10
+#define ORC_LEARNED_TERMS 7
+#define ORC_LEARN_T3_MAX 65536u
+void orc_cell_features(const double* xs, const double* ys, const double* ms, int64_t nnodes,
+                       const int32_t* level, const uint32_t* first, const uint32_t* count,
+                       const uint64_t* codes_sorted, int bits, double minx, double miny,
+                       double root_side, const double* cx, const double* cy, const double* mass,
+                       double* f);
+/* b: 14 doubles (re, im of the 7 basis functions) */
+void orc_learned_basis(double ux, double uy, double q, double* b);
+/* out[0..5]: label (exact cell sum - monopole) / scale (x, y), offset features w_x, w_y, q, and
+ * scale = M l^2 / r^4; the sink `self` (sorted index, -1: none) is left out of the cell sum */
+void orc_pair_label(const double* xs, const double* ys, const double* ms, uint32_t first,
+                    uint32_t count, double cx, double cy, double mass, double l, double px,
+                    double py, int64_t self, double eps2, double* out);
+/* loss sum |sum_j c_j b_j - T|^2 / (2 B) over samples X[B x 13] (10 cell features, w_x, w_y, q)
+ * and labels T[B x 2], with c = MLP(10 -> h -> h -> 14) of the cell features; gradients */
+double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h,
+                             const double* W1, const double* b1, const double* W2,
+                             const double* b2, const double* W3, const double* b3, double* gW1,
+                             double* gb1, double* gW2, double* gb2, double* gW3, double* gb3);
+/* gravity walk with single-precision decisions on given node records plus the correction on
+ * accepted cells with d2 <= l2 * inv_band2 (single precision; inv_band2 = 0: all accepted
+ * cells); coef: 14 doubles per node; counters = [pn, pp, corrected] */
+void orc_bh_traverse_learned(const float* xs, const float* ys, const float* ms, int64_t n,
+                             const int32_t* level, const uint32_t* first, const uint32_t* count,
+                             const uint32_t* skip, const uint8_t* pp, const float* cx,
+                             const float* cy, const float* mass, int64_t nnodes, float root_side,
+                             float inv_theta2, double G, double eps2, const double* coef,
+                             float inv_band2, const int64_t* sinks, int64_t nsinks,
+                             double* out_ax, double* out_ay, uint64_t* counters);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,15 @@ def lib():
                                                                           _vp, I64] + [_vp] * 6
                                     + [I32]),
             "orc_direct": (None, [_vp, _vp, _vp, I64, I32, D, D, _vp, I64, _vp, _vp, _vp, I32]),
+            "orc_cell_features": (None, [_vp, _vp, _vp, I64] + [_vp] * 4 + [I32, D, D, D]
+                                  + [_vp] * 4),
+            "orc_learned_basis": (None, [D, D, D, _vp]),
+            "orc_pair_label": (None, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, D, D, D, D, D, D,
+                                      I64, D, _vp]),
+            "orc_learned_loss_grad": (D, [_vp, _vp, I64, I32] + [_vp] * 12),
+            "orc_bh_traverse_learned": (None, [_vp, _vp, _vp, I64] + [_vp] * 8
+                                        + [I64, C.c_float, C.c_float, D, D, _vp, C.c_float, _vp,
+                                           I64, _vp, _vp, _vp]),
             "orc_quadtree_build": (_vp, [_vp, _vp, I64, I32]),
             "orc_quadtree_free": (None, [_vp]),
             "orc_quadtree_node_count": (I64, [_vp]),
@@ -454,3 +463,96 @@ def ref_naive_tsne(x, y):
     ax, ay = np.empty(x.size), np.empty(x.size)
     tot = ref().ref_naive_tsne(_p(x), _p(y), x.size, _p(ax), _p(ay))
     return ax, ay, tot
+
+
+# ---------------------------------------------------------------------------------------------
+# learned far-field correction
+# ---------------------------------------------------------------------------------------------
+CELL_FEATURES = 10
+LEARNED_TERMS = 7
+
+
+def cell_features(xs, ys, ms, tree, codes_sorted, bits, minx, miny, root_side, mom):
+    """FP64 features of every node (see orc_cell_features): [nnodes, 10]."""
+    xs, ys = _f64(xs), _f64(ys)
+    ms = None if ms is None else _f64(ms)
+    nn = tree["first"].size
+    f = np.empty((nn, CELL_FEATURES))
+    lib().orc_cell_features(_p(xs), _p(ys), _p(ms), nn,
+                            _p(np.ascontiguousarray(tree["level"], np.int32)),
+                            _p(np.ascontiguousarray(tree["first"], np.uint32)),
+                            _p(np.ascontiguousarray(tree["count"], np.uint32)),
+                            _p(np.ascontiguousarray(codes_sorted, np.uint64)), int(bits),
+                            float(minx), float(miny), float(root_side), _p(_f64(mom["cx"])),
+                            _p(_f64(mom["cy"])), _p(_f64(mom["mass"])), _p(f))
+    return f
+
+
+def learned_basis(ux, uy, q):
+    b = np.empty(2 * LEARNED_TERMS)
+    lib().orc_learned_basis(float(ux), float(uy), float(q), _p(b))
+    return b
+
+
+def pair_label(xs, ys, ms, first, count, cx, cy, mass, l, px, py, self_index=-1, eps2=0.0):
+    """(label x, label y, w_x, w_y, q, scale) of one (sink, cell) pair."""
+    xs, ys = _f64(xs), _f64(ys)
+    ms = None if ms is None else _f64(ms)
+    out = np.empty(6)
+    lib().orc_pair_label(_p(xs), _p(ys), _p(ms), int(first), int(count), float(cx), float(cy),
+                         float(mass), float(l), float(px), float(py), int(self_index), float(eps2),
+                         _p(out))
+    return out
+
+
+def learned_loss_grad(X, T, params):
+    X, T = _f64(X), _f64(T)
+    B = X.shape[0]
+    W1, b1, W2, b2, W3, b3 = (_f64(params[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3"))
+    g = {k: np.empty_like(v) for k, v in (("W1", W1), ("b1", b1), ("W2", W2), ("b2", b2),
+                                          ("W3", W3), ("b3", b3))}
+    loss = lib().orc_learned_loss_grad(_p(X), _p(T), B, W1.shape[1], _p(W1), _p(b1), _p(W2),
+                                       _p(b2), _p(W3), _p(b3), _p(g["W1"]), _p(g["b1"]),
+                                       _p(g["W2"]), _p(g["b2"]), _p(g["W3"]), _p(g["b3"]))
+    return float(loss), g
+
+
+def bh_traverse_learned(xs, ys, ms, tree, root_side, inv_theta2, coef, inv_band2=0.0, G=1.0,
+                        eps2=0.0, sinks=None):
+    """Single-precision-decision gravity walk plus the learned correction with per-node
+    coefficients coef [nnodes, 14]; returns (ax, ay, counters [pn, pp, corrected])."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    ms = None if ms is None else np.ascontiguousarray(ms, np.float32)
+    ns = xs.size if sinks is None else len(sinks)
+    sk = None if sinks is None else np.ascontiguousarray(sinks, dtype=np.int64)
+    ax, ay = np.empty(ns), np.empty(ns)
+    cnt = np.zeros(3, np.uint64)
+    arr = [np.ascontiguousarray(tree["level"], np.int32),
+           np.ascontiguousarray(tree["first"], np.uint32),
+           np.ascontiguousarray(tree["count"], np.uint32),
+           np.ascontiguousarray(tree["skip"], np.uint32),
+           np.ascontiguousarray(tree["pp"], np.uint8),
+           np.ascontiguousarray(tree["cx"], np.float32),
+           np.ascontiguousarray(tree["cy"], np.float32),
+           np.ascontiguousarray(tree["mass"], np.float32)]
+    lib().orc_bh_traverse_learned(_p(xs), _p(ys), _p(ms), xs.size, *[_p(a) for a in arr],
+                                  arr[0].size, float(np.float32(root_side)),
+                                  float(np.float32(inv_theta2)), float(G), float(eps2),
+                                  _p(_f64(coef).reshape(-1)), float(np.float32(inv_band2)), _p(sk),
+                                  ns, _p(ax), _p(ay), _p(cnt))
+    return ax, ay, cnt
+
+
+def quadrupole_coefficients(features):
+    """The coefficients that make the learned correction the exact quadrupole term: c_1 =
+    0.75 tr S, c_2 = 0.375 Q, c_3 = 1.875 conj(Q), Q = S_xx - S_yy + 2i S_xy (S as normalised
+    in the features), the rest 0. [nnodes, 14]."""
+    f = np.atleast_2d(features)
+    c = np.zeros((f.shape[0], 2 * LEARNED_TERMS))
+    sxx, sxy, syy = f[:, 0], f[:, 1], f[:, 2]
+    qr, qi = sxx - syy, 2.0 * sxy
+    c[:, 0] = 0.75 * (sxx + syy)
+    c[:, 2], c[:, 3] = 0.375 * qr, 0.375 * qi
+    c[:, 4], c[:, 5] = 1.875 * qr, -1.875 * qi
+    return c

@@ -684,3 +684,146 @@ def test_internal_second_moment_unweighted_closed_form():
     assert t["com_x"][0] == pytest.approx(13 / 30, rel=1e-15)
     assert t["com_y"][0] == pytest.approx(11 / 30, rel=1e-15)
     assert t["m2"][0] == pytest.approx([245 / 900 + 0.02, 280 / 900, 320 / 900], rel=1e-13)
+
+
+# ---------------------------------------------------------------------------------------------
+# learned far-field correction: features, basis, labels, loss gradient, corrected walk
+# ---------------------------------------------------------------------------------------------
+def _one_cell_tree(n):
+    return dict(level=np.array([0], np.int32), first=np.array([0], np.uint32),
+                count=np.array([n], np.uint32), skip=np.array([1], np.uint32))
+
+
+def test_cell_features_closed_forms():
+    """Two unit masses at (0.5 +- 0.25, 0.5) in the root box [0, 2)^2 (side 2, centre (1, 1)):
+    S = diag(2 * 0.25^2, 0), T = 0, COM offset ((0.5 - 1)/2, (0.5 - 1)/2), log2(2)/16. One
+    particle: S = T = 0."""
+    xs, ys = np.array([0.25, 0.75]), np.array([0.5, 0.5])
+    mom = dict(cx=np.array([0.5]), cy=np.array([0.5]), mass=np.array([2.0]))
+    f = O.cell_features(xs, ys, None, _one_cell_tree(2), np.zeros(2, np.uint64), 32, 0.0, 0.0,
+                        2.0, mom)[0]
+    M, l = 2.0, 2.0
+    assert f[0] == pytest.approx(2 * 0.25 ** 2 / (M * l * l), rel=1e-15)
+    assert f[1] == 0.0 and f[2] == 0.0 and np.all(f[3:7] == 0.0)
+    assert f[7] == pytest.approx(-0.25) and f[8] == pytest.approx(-0.25)
+    assert f[9] == pytest.approx(1.0 / 16.0)
+    # a level-1 cell: the code prefix selects the box (x high, y low -> child 1: [1, 2) x [0, 1))
+    tree = dict(level=np.array([1], np.int32), first=np.array([0], np.uint32),
+                count=np.array([1], np.uint32), skip=np.array([1], np.uint32))
+    code = np.array([1 << 62], np.uint64)  # top digit 01: x bit set
+    mom = dict(cx=np.array([1.75]), cy=np.array([0.25]), mass=np.array([1.0]))
+    f = O.cell_features(np.array([1.75]), np.array([0.25]), None, tree, code, 32, 0.0, 0.0, 2.0,
+                        mom)[0]
+    assert np.all(f[:7] == 0.0) and f[9] == 0.0
+    assert f[7] == pytest.approx((1.75 - 1.5) / 1.0) and f[8] == pytest.approx((0.25 - 0.5) / 1.0)
+
+
+def _cluster_label(xs, ys, ms, dist, angle, l=1.0):
+    """Exact residual of a cell (particles xs, ys about their COM at the origin) seen from a sink
+    at distance dist in direction angle from the COM, and the quadrupole prediction."""
+    M = ms.sum()
+    px, py = -dist * np.cos(angle), -dist * np.sin(angle)  # d = com - p points along angle
+    lab = O.pair_label(xs, ys, ms, 0, xs.size, 0.0, 0.0, M, l, px, py)
+    feat = np.zeros(10)
+    feat[0] = np.sum(ms * xs * xs) / (M * l * l)
+    feat[1] = np.sum(ms * xs * ys) / (M * l * l)
+    feat[2] = np.sum(ms * ys * ys) / (M * l * l)
+    c = O.quadrupole_coefficients(feat)[0]
+    b = O.learned_basis(lab[2], lab[3], lab[4])
+    pr = np.sum(c[0::2] * b[0::2] - c[1::2] * b[1::2])
+    pi = np.sum(c[0::2] * b[1::2] + c[1::2] * b[0::2])
+    return lab, pr, pi
+
+
+def test_quadrupole_basis_reproduces_exact_residual():
+    """The basis with the quadrupole coefficients is the Taylor quadrupole term: on a cluster with
+    zero third moments (four masses at (+-a, 0), (0, +-b)) the remaining error falls as q^2 when
+    the distance doubles (hexadecapole), on an asymmetric cluster as q (octupole). A sign or
+    power slip in the basis leaves an O(1) error."""
+    a, b_ = 0.3, 0.12
+    xs, ys = np.array([a, -a, 0, 0]), np.array([0, 0, b_, -b_])
+    ms = np.ones(4)
+    for angle in (0.3, 1.1, 2.5, -2.0):
+        e = []
+        for dist in (4.0, 8.0, 16.0):
+            lab, pr, pi = _cluster_label(xs, ys, ms, dist, angle)
+            e.append(np.hypot(pr - lab[0], pi - lab[1]) / np.hypot(lab[0], lab[1]))
+        assert e[0] < 0.05 and e[1] / e[0] < 0.3 and e[2] / e[1] < 0.3, e
+    rng = np.random.default_rng(5)
+    xs, ys = rng.uniform(-0.5, 0.5, 12), rng.uniform(-0.5, 0.5, 12)
+    ms = rng.uniform(0.5, 1.5, 12)
+    xs -= np.sum(ms * xs) / ms.sum()
+    ys -= np.sum(ms * ys) / ms.sum()
+    e = []
+    for dist in (8.0, 16.0, 32.0):
+        lab, pr, pi = _cluster_label(xs, ys, ms, dist, 0.7)
+        e.append(np.hypot(pr - lab[0], pi - lab[1]) / np.hypot(lab[0], lab[1]))
+    assert e[0] < 0.2 and 0.3 < e[1] / e[0] < 0.7 and 0.3 < e[2] / e[1] < 0.7, e
+
+
+def test_pair_label_one_particle_and_self():
+    """A one-particle cell is its own monopole: residual 0. The sink inside the cell is left out
+    of the exact sum (and the label then holds minus its own monopole share)."""
+    lab = O.pair_label(np.array([0.2]), np.array([-0.1]), None, 0, 1, 0.2, -0.1, 1.0, 0.5, 3.0,
+                       1.0, eps2=1e-6)
+    assert abs(lab[0]) < 1e-15 and abs(lab[1]) < 1e-15
+    lab_self = O.pair_label(np.array([0.0, 1.0]), np.array([0.0, 0.0]), None, 0, 2, 0.5, 0.0, 2.0,
+                            1.0, 0.0, 0.0, self_index=0)
+    # exact sum: the particle at x = 1 alone (unit force +x); monopole: 2 / 0.5^2 = 8 along +x
+    assert lab_self[0] * lab_self[5] == pytest.approx(1.0 - 8.0, rel=1e-14)
+
+
+def test_learned_loss_gradient_finite_differences():
+    rng = np.random.default_rng(8)
+    h = 6
+    p = {"W1": rng.normal(0, 0.5, (10, h)), "b1": rng.normal(0, 0.1, h),
+         "W2": rng.normal(0, 0.5, (h, h)), "b2": rng.normal(0, 0.1, h),
+         "W3": rng.normal(0, 0.5, (h, 14)), "b3": rng.normal(0, 0.1, 14)}
+    X = rng.normal(0, 1, (7, 13))
+    ang = rng.uniform(0, 2 * np.pi, 7)
+    X[:, 10], X[:, 11], X[:, 12] = np.cos(ang), np.sin(ang), rng.uniform(0.1, 0.9, 7)
+    T = rng.normal(0, 1, (7, 2))
+    loss, g = O.learned_loss_grad(X, T, p)
+    for name in ("W1", "b2", "W3", "b3"):
+        flat = p[name].reshape(-1)
+        for idx in (0, flat.size // 2, flat.size - 1):
+            old = flat[idx]
+            flat[idx] = old + 1e-6
+            lp, _ = O.learned_loss_grad(X, T, p)
+            flat[idx] = old - 1e-6
+            lm, _ = O.learned_loss_grad(X, T, p)
+            flat[idx] = old
+            assert g[name].reshape(-1)[idx] == pytest.approx((lp - lm) / 2e-6, rel=1e-5, abs=1e-9)
+
+
+def test_learned_walk_zero_and_quadrupole_coefficients():
+    """Zero coefficients give exactly the single-precision-decision walk; the exact quadrupole
+    coefficients cut the error against the FP64 direct sum several times at theta = 0.9."""
+    from masterthesislucas_amd.workloads import plummer
+    p = plummer(20000, a=1.0, seed=9)
+    t = oracle_tree(p.x, p.y, p.m)
+    tr = dict(t["tree"])
+    tr["pp"] = t["pp"]
+    for k in ("cx", "cy", "mass"):
+        tr[k] = t["mom"][k].astype(np.float32)
+    xs, ys, ms = (t[k].astype(np.float32) for k in ("xs", "ys", "ms"))
+    side = np.float32(t["side"])
+    eps2 = float(np.float32(1e-3) ** 2)
+    s = np.arange(0, 20000, 97)
+    inv = O.inv_theta2_f32(0.9)
+    zero = np.zeros((tr["cx"].size, 14))
+    ax, ay, cnt = O.bh_traverse_learned(xs, ys, ms, tr, side, inv, zero, eps2=eps2, sinks=s)
+    bx, by, _, _, _, c3 = O.bh_traverse_f32(xs, ys, ms, tr, side, inv, O.KERNEL_GRAVITY,
+                                            eps2=eps2, sinks=s)
+    assert np.array_equal(ax, bx) and np.array_equal(ay, by)
+    assert int(cnt[0]) == int(c3[0]) and int(cnt[1]) == int(c3[1]) and int(cnt[2]) == int(cnt[0])
+    mom32 = dict(cx=tr["cx"].astype(np.float64), cy=tr["cy"].astype(np.float64),
+                 mass=tr["mass"].astype(np.float64))
+    f = O.cell_features(xs, ys, ms, tr, t["codes_sorted"], 32, t["bounds"][0], t["bounds"][1],
+                        float(side), mom32)
+    qx, qy, _ = O.bh_traverse_learned(xs, ys, ms, tr, side, inv, O.quadrupole_coefficients(f),
+                                      eps2=eps2, sinks=s)
+    dx, dy, _ = O.direct(xs, ys, ms, O.KERNEL_GRAVITY, 1.0, eps2, sinks=s)
+    e_plain = np.median(rel_err(bx, by, dx, dy))
+    e_quad = np.median(rel_err(qx, qy, dx, dy))
+    assert e_quad < e_plain / 3, (e_plain, e_quad)
