@@ -224,6 +224,11 @@ void bh_mlp_destroy(bh_mlp* mlp);
 const char* bh_mlp_last_error(const bh_mlp* mlp);
 bh_status bh_mlp_set_params(bh_mlp* mlp, const float* params, uint32_t count); /* resets Adam */
 bh_status bh_mlp_get_params(bh_mlp* mlp, float* params, uint32_t count);
+/* basis terms of the loss and of the walk's correction: 7 (default, quadrupole and octupole
+ * forms) or 3 (c_1 w + c_2 conj w + c_3 w^3 only: about half the correction's arithmetic);
+ * set before training, the walk uses the network's setting */
+bh_status bh_mlp_set_terms(bh_mlp* mlp, int32_t terms);
+int32_t bh_mlp_get_terms(const bh_mlp* mlp);
 /* coefficients Y[n x 14] of host cell features X[n x 10]: FP32 kernels (bf16_mfma = 0) or the
  * BF16 MFMA inference path the walk uses */
 bh_status bh_mlp_forward(bh_mlp* mlp, const float* X, uint32_t n, float* Y, int bf16_mfma);
@@ -273,6 +278,10 @@ bh_status bh_set_sort_spin_limit(bh_ctx* ctx, uint32_t polls);
 /* ---- introspection for parity checks (bit-exact against the reference tree) ---- */
 /* bounds used by the last build {minx, miny, maxx, maxy} and the root side (FP64) */
 bh_status bh_get_bounds(bh_ctx* ctx, double* bounds, double* root_side);
+/* the quantiser the last build prepared on the device from the root side S (exact emulation of
+ * floor(2^B * RN64(d / S)), masterthesislucas_amd/csrc/bh_softfp.h): S = M * 2^es and
+ * R = floor((2^116 - 1) / M) */
+bh_status bh_get_quantizer(bh_ctx* ctx, uint64_t* M, int32_t* es, uint64_t* R);
 /* sorted Morton codes (createMortonCode, nBodySolverFMM_SYM_MORTON.h:538-552; 64-bit slots
  * holding 32-bit codes in MORTON_FIXED mode) and the sort permutation (original indices in
  * sorted order, integer_sort at :89 with ties by original index) */

@@ -82,6 +82,8 @@ SIGNATURES = {
     "bh_mlp_last_error": (C.c_char_p, [_vp]),
     "bh_mlp_set_params": (_i32, [_vp, _vp, _u32]),
     "bh_mlp_get_params": (_i32, [_vp, _vp, _u32]),
+    "bh_mlp_set_terms": (_i32, [_vp, _i32]),
+    "bh_mlp_get_terms": (_i32, [_vp]),
     "bh_mlp_forward": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "bh_mlp_loss_grad": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "bh_mlp_adam_step": (_i32, [_vp, _vp, _f]),
@@ -98,6 +100,7 @@ SIGNATURES = {
     "bh_batch_info": (_i32, [_vp, _vp, _vp]),
     "bh_leapfrog_graph": (_i32, [_vp, _i32, _f, _d]),
     "bh_get_bounds": (_i32, [_vp, _vp, _vp]),
+    "bh_get_quantizer": (_i32, [_vp, _vp, _vp, _vp]),
     "bh_get_morton": (_i32, [_vp, _vp, _vp]),
     "bh_tree_info": (_i32, [_vp, _vp, _vp, _vp]),
     "bh_get_tree": (_i32, [_vp] + [_vp] * 8),

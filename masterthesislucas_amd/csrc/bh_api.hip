@@ -113,6 +113,7 @@ struct bh_ctx {
     bool learned = false;  // second moments in builds, learned correction available
     float* lfeat = nullptr;   // cell features of the last build (kMlpIn per node)
     float4* lcoef = nullptr;  // coefficient records (kCoefStride floats per node)
+    uint32_t* lbig = nullptr; // large cells listed by the feature pass (counts + lists)
     bool lfeat_ok = false;    // lfeat holds the features of the current tree
 
     // batched systems (bh_create_batched): particles grouped by system, one tree per system
@@ -1290,6 +1291,18 @@ bh_status bh_get_bounds(bh_ctx* c, double* bounds, double* root_side) {
     return BH_OK;
 }
 
+bh_status bh_get_quantizer(bh_ctx* c, uint64_t* M, int32_t* es, uint64_t* R) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
+    bh_status st = hip_check(c, hipMemcpy(c->h_prm, c->prm, sizeof(BuildParams), hipMemcpyDeviceToHost),
+                             "params");
+    if (st != BH_OK) return st;
+    if (M) *M = c->h_prm->quant.M;
+    if (es) *es = c->h_prm->quant.es;
+    if (R) *R = c->h_prm->quant.R;
+    return BH_OK;
+}
+
 bh_status bh_get_morton(bh_ctx* c, uint64_t* codes_sorted, uint32_t* perm) {
     if (!c) return BH_ERR_ARG;
     if (!c->built) return fail(c, BH_ERR_STATE, "no tree built");
@@ -1516,6 +1529,7 @@ struct bh_mlp {
     float *m1 = nullptr, *m2 = nullptr;
     MlpBf16* wbf = nullptr;
     int step = 0;
+    int terms = kLearnTerms;  // basis terms of the loss and of the walk's correction (3 or 7)
     float *X = nullptr, *T = nullptr;  // data set
     uint32_t cap = 0, n = 0;
     float* S = nullptr;  // training scratch
@@ -1661,6 +1675,16 @@ bh_status bh_mlp_set_params(bh_mlp* m, const float* params, uint32_t count) {
     return mlp_upload_params(m, params);
 }
 
+bh_status bh_mlp_set_terms(bh_mlp* m, int32_t terms) {
+    if (!m) return BH_ERR_ARG;
+    if (terms != 3 && terms != kLearnTerms)
+        return mfail(m, BH_ERR_ARG, "terms must be 3 or %d", kLearnTerms);
+    m->terms = terms;
+    return BH_OK;
+}
+
+int32_t bh_mlp_get_terms(const bh_mlp* m) { return m ? m->terms : 0; }
+
 bh_status bh_mlp_get_params(bh_mlp* m, float* params, uint32_t count) {
     if (!m || !params) return BH_ERR_ARG;
     if (count != (uint32_t)kMlpParams) return mfail(m, BH_ERR_ARG, "expected %d parameters", kMlpParams);
@@ -1696,7 +1720,8 @@ static bh_status mlp_grad_device(bh_mlp* m, const float* dX, const float* dT, co
     if (st == BH_OK) st = ensure_m(m, &m->partials, &m->part_cap, (size_t)mlp_grad_chunks(B) * kMlpParams);
     if (st == BH_OK) st = ensure_m(m, &m->loss_part, &m->loss_cap, mlp_loss_blocks(B));
     if (st != BH_OK) return st;
-    launch_mlp_grad(m->stream, dX, dT, order, B, m->params, m->S, m->loss_part, m->partials);
+    launch_mlp_grad(m->stream, dX, dT, order, B, m->params, m->S, m->loss_part, m->partials,
+                    m->terms);
     return mcheck(m, hipGetLastError(), "gradient");
 }
 
@@ -1844,7 +1869,7 @@ bh_status ensure_features(bh_ctx* c) {
     const void* codes = c->keys_in_b ? c->keys_b : c->keys_a;
     phase_begin(c, 4);
     launch_cell_features(c->stream, c->pos4, c->nodes, c->m2, codes, c->wide, c->B, c->prm,
-                         c->cap_nodes, c->lfeat);
+                         c->cap_nodes, c->lfeat, c->lbig, c->cfg.n_max);
     phase_end(c, 4);
     bh_status st = launch_status(c, "cell features");
     if (st == BH_OK) c->lfeat_ok = true;
@@ -1873,6 +1898,7 @@ bh_status bh_learned_enable(bh_ctx* c, int enable) {
         bh_status st = dalloc(c, &c->m2, c->cap_nodes);
         if (st == BH_OK) st = dalloc(c, &c->lfeat, (size_t)c->cap_nodes * kMlpIn);
         if (st == BH_OK) st = dalloc(c, &c->lcoef, (size_t)c->cap_nodes * (kCoefStride / 4));
+        if (st == BH_OK) st = dalloc(c, &c->lbig, cell_features_lists_words(c->cfg.n_max));
         if (st != BH_OK) return st;
     }
     c->learned = enable != 0;
@@ -1949,9 +1975,8 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
     phase_begin(c, 5);  // the per-cell inference is part of the learned force evaluation
     launch_cell_coefs(s, c->lfeat, c->prm, c->cap_nodes, m->wbf, c->lcoef);
     c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->lcoef, c->prm,
-                                             c->n, inv_t2, inv_b2, eps * eps, c->cfg.G, c->ax,
-                                             c->ay, total ? c->tot : nullptr, c->counters,
-                                             batch_of(c, variant_ns(c->variant) == 1 ? 0 : 1));
+                                             c->n, inv_t2, inv_b2, eps * eps, c->cfg.G, m->terms,
+                                             c->ax, c->ay, total ? c->tot : nullptr, c->counters);
     phase_end(c, 5);
     if (total) launch_reduce_f64(s, c->tot, nullptr, c->n, c->red_part, c->dsum);
     c->have_acc = true;

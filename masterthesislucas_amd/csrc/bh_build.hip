@@ -895,7 +895,7 @@ __global__ __launch_bounds__(kUpThreads) void k_up_chunk(const float4* __restric
                                                      float4* __restrict__ m2,
                                                      const BuildParams* __restrict__ prm,
                                                      uint32_t* __restrict__ deferred,
-                                                     uint32_t cap_nodes) {
+                                                     uint32_t cap_nodes, int big_done) {
     __shared__ float s_m[kUpChunk], s_x[kUpChunk], s_y[kUpChunk];
     __shared__ float s_xx[kUpChunk], s_xy[kUpChunk], s_yy[kUpChunk];
     __shared__ uint32_t s_maxl;
@@ -932,7 +932,10 @@ __global__ __launch_bounds__(kUpThreads) void k_up_chunk(const float4* __restric
         const uint32_t k = c0 + threadIdx.x + r * kUpThreads;
         if (k >= c1 || !(nd[r].meta & kMetaLeaf)) continue;
         const uint32_t s = k - c0;
-        const Moments mo = (!with_m2 && nd[r].count > kBigLeaf)
+        // leaves of more than kBigLeaf points hold their moments already when k_big_leaf_moments
+        // ran (fixed-depth trees); otherwise (coincident points at the depth limit of an
+        // adaptive tree, rare) this thread sums them
+        const Moments mo = (big_done && nd[r].count > kBigLeaf)
                                ? Moments{nd[r].mass, nd[r].cx, nd[r].cy, 0.f, 0.f, 0.f}
                                : leaf_moments(pos4, nd[r].first, nd[r].count, with_m2);
         s_m[s] = mo.m; s_x[s] = mo.x; s_y[s] = mo.y;
@@ -1419,11 +1422,12 @@ void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_
                    float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes,
                    bool big_leaves) {
     const uint32_t nch = up_chunks(cap_nodes);
-    if (big_leaves && !m2)
+    const bool big_done = big_leaves && !m2;
+    if (big_done)
         k_big_leaf_moments<<<div_up(cap_nodes, kWavesPerBlock), kBlock, 0, s>>>(pos4, nodes, prm,
                                                                                 cap_nodes);
     k_up_chunk<<<nch, kUpThreads, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
-                                      cap_nodes);
+                                      cap_nodes, big_done ? 1 : 0);
     k_up_top<<<1, kUpThreads, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, nch,
                                       cap_nodes);
 }

@@ -184,8 +184,10 @@ void launch_mlp_forward_f32(hipStream_t s, const float* X, uint32_t n, const flo
 size_t mlp_scratch_floats(uint32_t B);
 uint32_t mlp_grad_chunks(uint32_t B);
 uint32_t mlp_loss_blocks(uint32_t B);
+// (terms: basis terms the loss uses, 3 or 7)
 void launch_mlp_grad(hipStream_t s, const float* X, const float* T, const uint32_t* order,
-                     uint32_t B, const float* p, float* S, double* loss_part, float* partials);
+                     uint32_t B, const float* p, float* S, double* loss_part, float* partials,
+                     int terms);
 void launch_mlp_adam(hipStream_t s, float* p, const float* partials, uint32_t nchunks, float* m1,
                      float* m2, float lr, int step, float* grad_out, int apply);
 void launch_mlp_adam_grad(hipStream_t s, float* p, const float* g, float* m1, float* m2, float lr,
@@ -197,20 +199,24 @@ void launch_collect_pairs(hipStream_t s, const float4* pos4, const Node* nodes,
 void launch_sinks_sorted(hipStream_t s, const uint32_t* id, uint32_t n, const uint32_t* sinks_caller,
                          uint32_t ns, uint32_t* where, uint32_t* sinks_sorted);
 // the walk with the learned correction from the per-node coefficient records
+// (terms: 3 or 7 basis terms of the correction, as the network was trained)
 uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
                                  const float4* coef, const BuildParams* prm, uint32_t n,
                                  float inv_theta2, float inv_band2, float eps2, float scale,
-                                 float* ax, float* ay, float* tot,
-                                 unsigned long long* counters, TraverseBatch batch);
+                                 int terms, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters);
 void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, const float* feat,
                         const BuildParams* prm, const uint32_t* sinks, const uint32_t* pairs, const uint32_t* counts,
                         const uint32_t* offsets, uint32_t ns, uint32_t cap, double eps2, float* X,
                         float* T, uint32_t total);
 // cell features (kMlpIn per node, bh_mlp.h) of the last build and the network's coefficient
 // records (kCoefStride floats per node) from them
+// (lists: cell_features_lists_words(n) words of scratch, n = the particle capacity)
 void launch_cell_features(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
                           const void* codes, bool wide, int B, const BuildParams* prm,
-                          uint32_t cap_nodes, float* feat);
+                          uint32_t cap_nodes, float* feat, uint32_t* lists, uint32_t n);
+uint32_t cell_list_cap(uint32_t n, uint32_t c);
+size_t cell_features_lists_words(uint32_t n);
 void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
                        uint32_t cap_nodes, const MlpBf16* w, float4* coef);
 

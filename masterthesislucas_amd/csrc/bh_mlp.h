@@ -178,25 +178,31 @@ __host__ __device__ inline void learned_record(const float* y, float* r) {
 }
 
 // Correction (before G) of one accepted pair: d = (dx, dy) = c - p, d2 = |d|^2, l the cell
-// side, ml2 = M l^2, rec the cell's coefficient record (wave-uniform).
+// side, ml2 = M l^2, rec the cell's coefficient record (wave-uniform). TERMS = 7: the whole
+// basis; TERMS = 3: its quadrupole part c_1 w + c_2 conj w + c_3 w^3 only (a network trained
+// with three terms).
+template <int TERMS>
 __device__ __forceinline__ void learned_correction(float dx, float dy, float d2, float l, float ml2,
                                                    const float* rec, float& cx, float& cy) {
+    static_assert(TERMS == 3 || TERMS == 7, "3 or 7 basis terms");
     const float inv_r = __builtin_amdgcn_rsqf(d2);
     const float ux = dx * inv_r, uy = dy * inv_r;
-    const float q = l * inv_r;
     const float w2r = fmaf(ux, ux, -uy * uy), w2i = 2.0f * ux * uy;
     const float w3r = fmaf(w2r, ux, -w2i * uy), w3i = fmaf(w2r, uy, w2i * ux);
-    const float w4r = fmaf(w2r, w2r, -w2i * w2i), w4i = 2.0f * w2r * w2i;
-    // q^1 part: c_4 + c_5 w^2 + c_6 conj(w)^2 + c_7 w^4
-    float tr = fmaf(rec[8], w2r, fmaf(rec[9], w2i, rec[6]));
-    tr = fmaf(rec[12], w4r, fmaf(-rec[13], w4i, tr));
-    float ti = fmaf(rec[10], w2i, fmaf(rec[11], w2r, rec[7]));
-    ti = fmaf(rec[12], w4i, fmaf(rec[13], w4r, ti));
-    // q^0 part: c_1 w + c_2 conj w + c_3 w^3, plus q times the above
+    // q^0 part: c_1 w + c_2 conj w + c_3 w^3
     float re = fmaf(rec[0], ux, fmaf(rec[1], uy, fmaf(rec[4], w3r, -rec[5] * w3i)));
     float im = fmaf(rec[2], uy, fmaf(rec[3], ux, fmaf(rec[4], w3i, rec[5] * w3r)));
-    re = fmaf(q, tr, re);
-    im = fmaf(q, ti, im);
+    if constexpr (TERMS == 7) {
+        // q^1 part: c_4 + c_5 w^2 + c_6 conj(w)^2 + c_7 w^4, times q = l / r
+        const float q = l * inv_r;
+        const float w4r = fmaf(w2r, w2r, -w2i * w2i), w4i = 2.0f * w2r * w2i;
+        float tr = fmaf(rec[8], w2r, fmaf(rec[9], w2i, rec[6]));
+        tr = fmaf(rec[12], w4r, fmaf(-rec[13], w4i, tr));
+        float ti = fmaf(rec[10], w2i, fmaf(rec[11], w2r, rec[7]));
+        ti = fmaf(rec[12], w4i, fmaf(rec[13], w4r, ti));
+        re = fmaf(q, tr, re);
+        im = fmaf(q, ti, im);
+    }
     const float inv_r2 = inv_r * inv_r;
     const float s = ml2 * inv_r2 * inv_r2;
     cx = s * re;

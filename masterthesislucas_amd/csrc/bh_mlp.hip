@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t B, const float* __restrict__ p,
                                                        float* __restrict__ S,
-                                                       double* __restrict__ loss_part) {
+                                                       double* __restrict__ loss_part, int terms) {
     __shared__ double red[kWavesPerBlock];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     double sq = 0.0;
@@ -239,6 +239,11 @@ __global__ __launch_bounds__(kBlock) void k_mlp_fwdbwd(const float* __restrict__
         }
         float bas[kMlpOut];
         learned_basis_f(row[kMlpIn], row[kMlpIn + 1], row[kMlpIn + 2], bas);
+        // a network trained with fewer terms: the basis beyond them is zero (no prediction, no
+        // gradient)
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o)
+            if (o >= 2 * terms) bas[o] = 0.f;
         float pr = 0.f, pi = 0.f;
 #pragma unroll
         for (int j = 0; j < kLearnTerms; ++j) {
@@ -466,10 +471,55 @@ __device__ __forceinline__ uint64_t compact_even(uint64_t v) {
 
 __device__ __forceinline__ double f64_of_bits(uint64_t b) { return __longlong_as_double((long long)b); }
 
-// One wave per node: the cell features (see bh_mlp.h). Second moments come from the upward pass
-// (m2); third moments are summed here about the node's centre of mass over its particles (cells
-// of at most kLearnT3Max particles); the box centre follows from the Morton prefix of the
-// node's first particle (B code bits per axis) and the root square, in FP64.
+// third moments sum m s s s about (cx, cy) over particles [f, e), `stride` threads apart from
+// `t0`, kFeatLoads loads in flight per thread (pos4 is padded past n; the tail is masked)
+constexpr uint32_t kFeatLoads = 4;
+__device__ __forceinline__ void third_moments(const float4* __restrict__ pos4, uint32_t f,
+                                              uint32_t e, uint32_t t0, uint32_t stride, float cx,
+                                              float cy, float* t) {
+    for (uint32_t j0 = f + t0; j0 < e; j0 += kFeatLoads * stride) {
+        float4 q[kFeatLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kFeatLoads; ++u) q[u] = pos4[min(j0 + u * stride, e - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < kFeatLoads; ++u) {
+            const float m = j0 + u * stride < e ? q[u].z : 0.f;
+            const float x = q[u].x - cx, y = q[u].y - cy;
+            const float mx = m * x, my = m * y;
+            t[0] = fmaf(mx * x, x, t[0]);
+            t[1] = fmaf(mx * x, y, t[1]);
+            t[2] = fmaf(mx * y, y, t[2]);
+            t[3] = fmaf(my * y, y, t[3]);
+        }
+    }
+}
+
+// wave-aggregated append of the lanes with `want` set to list[1 + ...] (list[0] counts)
+__device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_t cap, bool want,
+                                            uint32_t v) {
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&list[0], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, kWave);
+    const uint32_t slot = base + (uint32_t)__popcll(m & lane_mask_lt());
+    if (want && slot < cap) list[1 + slot] = v;
+}
+
+__device__ __forceinline__ float cell_side_f(const BuildParams* __restrict__ prm, uint32_t meta) {
+    return (float)ldexp(f64_of_bits(prm->side64), -(int)(meta & kMetaLevelMask));
+}
+
+// One thread per node: the cell features (see bh_mlp.h). Second moments come from the upward
+// pass (m2); third moments are summed about the node's centre of mass over its particles, here
+// for cells of at most kFeatThread particles; larger cells are listed for one wave each
+// (k_cell_t3_wave, up to kFeatWave particles) or one block each (k_cell_t3_big, up to
+// kLearnT3Max). The box centre follows from the Morton prefix of the node's first particle (B
+// code bits per axis) and the root square, in FP64.
+constexpr uint32_t kFeatThread = 64;
+constexpr uint32_t kFeatWave = 2048;
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_cell_features(const float4* __restrict__ pos4,
                                                           const Node* __restrict__ nodes,
@@ -477,29 +527,21 @@ __global__ __launch_bounds__(kBlock) void k_cell_features(const float4* __restri
                                                           const K* __restrict__ codes, int B,
                                                           const BuildParams* __restrict__ prm,
                                                           uint32_t cap_nodes,
-                                                          float* __restrict__ feat) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (k >= min(prm->n_nodes, cap_nodes)) return;
-    const Node nd = nodes[k];
-    const float M = nd.mass, cx = nd.cx, cy = nd.cy;
-    float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
-    if (nd.count <= kLearnT3Max) {
-        for (uint32_t j = nd.first + lane; j < nd.first + nd.count; j += kWave) {
-            const float4 q = pos4[j];
-            const float u = q.x - cx, v = q.y - cy;
-            const float mu = q.z * u, mv = q.z * v;
-            t0 = fmaf(mu * u, u, t0);
-            t1 = fmaf(mu * u, v, t1);
-            t2 = fmaf(mu * v, v, t2);
-            t3 = fmaf(mv * v, v, t3);
-        }
-        t0 = wave_sum_f(t0);
-        t1 = wave_sum_f(t1);
-        t2 = wave_sum_f(t2);
-        t3 = wave_sum_f(t3);
-    }
-    if (lane != 0) return;
+                                                          float* __restrict__ feat,
+                                                          uint32_t* __restrict__ wlist,
+                                                          uint32_t wcap,
+                                                          uint32_t* __restrict__ blist,
+                                                          uint32_t bcap) {
+    const uint32_t nn = min(prm->n_nodes, cap_nodes);
+    if (blockIdx.x * kBlock >= nn) return;  // whole block
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k < nn;
+    const Node nd = nodes[live ? k : nn - 1];
+    list_append(wlist, wcap, live && nd.count > kFeatThread && nd.count <= kFeatWave, k);
+    list_append(blist, bcap, live && nd.count > kFeatWave && nd.count <= kLearnT3Max, k);
+    if (!live) return;
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    if (nd.count <= kFeatThread) third_moments(pos4, nd.first, nd.first + nd.count, 0, 1, nd.cx, nd.cy, t);
     const int L = (int)(nd.meta & kMetaLevelMask);
     const uint64_t pre = L > 0 ? (uint64_t)codes[nd.first] >> (2 * (B - L)) : 0ull;
     const double l64 = ldexp(f64_of_bits(prm->side64), -L);
@@ -507,18 +549,79 @@ __global__ __launch_bounds__(kBlock) void k_cell_features(const float4* __restri
     const double by = f64_of_bits(prm->miny64) + ((double)compact_even(pre >> 1) + 0.5) * l64;
     const float l = (float)l64;
     const float4 s2 = m2[k];
-    const float a = 1.0f / (M * l * l), b = a / l;
+    const float a = 1.0f / (nd.mass * l * l), b = a / l;
     float* o = feat + (size_t)k * kMlpIn;
     o[0] = s2.x * a;
     o[1] = s2.y * a;
     o[2] = s2.z * a;
-    o[3] = t0 * b;
-    o[4] = t1 * b;
-    o[5] = t2 * b;
-    o[6] = t3 * b;
-    o[7] = (float)(((double)cx - bx) / l64);
-    o[8] = (float)(((double)cy - by) / l64);
+    o[3] = t[0] * b;
+    o[4] = t[1] * b;
+    o[5] = t[2] * b;
+    o[6] = t[3] * b;
+    o[7] = (float)(((double)nd.cx - bx) / l64);
+    o[8] = (float)(((double)nd.cy - by) / l64);
     o[9] = (float)(log2((double)nd.count) / 16.0);
+}
+
+// Third moments of the cells listed for one wave each (grid-stride over the list).
+__global__ __launch_bounds__(kBlock) void k_cell_t3_wave(const float4* __restrict__ pos4,
+                                                         const Node* __restrict__ nodes,
+                                                         const BuildParams* __restrict__ prm,
+                                                         const uint32_t* __restrict__ wlist,
+                                                         uint32_t wcap,
+                                                         float* __restrict__ feat) {
+    const uint32_t nw = min(wlist[0], wcap);
+    const int lane = threadIdx.x & 63;
+    const uint32_t gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    for (uint32_t e = gw; e < nw; e += gridDim.x * kWavesPerBlock) {
+        const uint32_t k = wlist[1 + e];
+        const Node nd = nodes[k];
+        float t[4] = {0.f, 0.f, 0.f, 0.f};
+        third_moments(pos4, nd.first, nd.first + nd.count, lane, kWave, nd.cx, nd.cy, t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = wave_sum_f(t[i]);
+        if (lane < 4) {
+            const float l = cell_side_f(prm, nd.meta);
+            const float b = 1.0f / (nd.mass * l * l) / l;
+            const float v = lane == 0 ? t[0] : lane == 1 ? t[1] : lane == 2 ? t[2] : t[3];
+            feat[(size_t)k * kMlpIn + 3 + lane] = v * b;
+        }
+    }
+}
+
+// Third moments of the listed large cells: one block of kBigThreads per cell (grid-stride over
+// the list), reduced in a fixed order.
+constexpr int kBigThreads = 1024;
+__global__ __launch_bounds__(kBigThreads) void k_cell_t3_big(const float4* __restrict__ pos4,
+                                                             const Node* __restrict__ nodes,
+                                                             const BuildParams* __restrict__ prm,
+                                                             const uint32_t* __restrict__ big,
+                                                             uint32_t big_cap,
+                                                             float* __restrict__ feat) {
+    __shared__ float red[kBigThreads / kWave][4];
+    const uint32_t nb = min(big[0], big_cap);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
+        const uint32_t k = big[1 + e];
+        const Node nd = nodes[k];
+        float t[4] = {0.f, 0.f, 0.f, 0.f};
+        third_moments(pos4, nd.first, nd.first + nd.count, threadIdx.x, kBigThreads, nd.cx,
+                      nd.cy, t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = wave_sum_f(t[i]);
+        if (lane == 0)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[w][i] = t[i];
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            float v = 0.f;
+            for (int q = 0; q < kBigThreads / kWave; ++q) v += red[q][threadIdx.x];
+            const float l = cell_side_f(prm, nd.meta);
+            const float b = 1.0f / (nd.mass * l * l) / l;
+            feat[(size_t)k * kMlpIn + 3 + threadIdx.x] = v * b;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -535,8 +638,9 @@ size_t mlp_scratch_floats(uint32_t B) { return (size_t)kScratchRows * B; }
 uint32_t mlp_grad_chunks(uint32_t B) { return div_up_u(B, kGradChunk); }
 uint32_t mlp_loss_blocks(uint32_t B) { return div_up_u(B, kBlock); }
 void launch_mlp_grad(hipStream_t s, const float* X, const float* T, const uint32_t* order,
-                     uint32_t B, const float* p, float* S, double* loss_part, float* partials) {
-    k_mlp_fwdbwd<<<div_up_u(B, kBlock), kBlock, 0, s>>>(X, T, order, B, p, S, loss_part);
+                     uint32_t B, const float* p, float* S, double* loss_part, float* partials,
+                     int terms) {
+    k_mlp_fwdbwd<<<div_up_u(B, kBlock), kBlock, 0, s>>>(X, T, order, B, p, S, loss_part, terms);
     dim3 g(div_up_u(kMlpParams, kBlock), mlp_grad_chunks(B));
     k_mlp_wgrad<<<g, kBlock, 0, s>>>(S, B, partials);
 }
@@ -569,14 +673,29 @@ void launch_label_pairs(hipStream_t s, const float4* pos4, const Node* nodes, co
 }
 void launch_cell_features(hipStream_t s, const float4* pos4, const Node* nodes, const float4* m2,
                           const void* codes, bool wide, int B, const BuildParams* prm,
-                          uint32_t cap_nodes, float* feat) {
-    const uint32_t g = div_up_u(cap_nodes, kWavesPerBlock);
+                          uint32_t cap_nodes, float* feat, uint32_t* lists, uint32_t n) {
+    const uint32_t wcap = cell_list_cap(n, kFeatThread), bcap = cell_list_cap(n, kFeatWave);
+    uint32_t* wlist = lists;
+    uint32_t* blist = lists + 1 + wcap;
+    const uint32_t g = div_up_u(cap_nodes, kBlock);
+    launch_fill_u32(s, wlist, 1, 0u);
+    launch_fill_u32(s, blist, 1, 0u);
     if (wide)
         k_cell_features<uint64_t><<<g, kBlock, 0, s>>>(pos4, nodes, m2, (const uint64_t*)codes, B,
-                                                       prm, cap_nodes, feat);
+                                                       prm, cap_nodes, feat, wlist, wcap, blist,
+                                                       bcap);
     else
         k_cell_features<uint32_t><<<g, kBlock, 0, s>>>(pos4, nodes, m2, (const uint32_t*)codes, B,
-                                                       prm, cap_nodes, feat);
+                                                       prm, cap_nodes, feat, wlist, wcap, blist,
+                                                       bcap);
+    k_cell_t3_wave<<<2048, kBlock, 0, s>>>(pos4, nodes, prm, wlist, wcap, feat);
+    k_cell_t3_big<<<1024, kBigThreads, 0, s>>>(pos4, nodes, prm, blist, bcap, feat);
+}
+// cells of more than c particles are disjoint within a level: at most n / c of them per level,
+// over at most kMaxLevels + 1 levels
+uint32_t cell_list_cap(uint32_t n, uint32_t c) { return (n / c + 1) * (kMaxLevels + 1); }
+size_t cell_features_lists_words(uint32_t n) {
+    return 2 + (size_t)cell_list_cap(n, kFeatThread) + cell_list_cap(n, kFeatWave);
 }
 void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
                        uint32_t cap_nodes, const MlpBf16* w, float4* coef) {

@@ -3,10 +3,14 @@
 // The reference quantises a position with double arithmetic (nBodySolverFMM_SYM_MORTON.h:538-552):
 //     t = RN64(RN64(p + (-min)) / L);   q = (uint32) min(max(t * 65536, 0), 65535)
 // and sizes the QuadTree root with double adds (quadtree.h:45-47). To reproduce those codes bit
-// for bit on the GPU without FP64 arithmetic, this header restates the two IEEE-754 binary64
-// operations involved (round-to-nearest-even addition, and floor(2^B * RN64(d / S))) with 64-bit
-// integer arithmetic only. Compiles as host C++ and as HIP device code; the host build is checked
-// against native double arithmetic on millions of random and boundary cases.
+// for bit on the GPU without FP64 arithmetic on the per-particle path, this header restates the
+// two IEEE-754 binary64 operations involved (round-to-nearest-even addition, and
+// floor(2^B * RN64(d / S))) with 64-bit integer arithmetic. The one exception is the quantiser's
+// preparation (bh_sfp_quant_prepare, once per build by one thread): it starts from an FP64
+// estimate of the reciprocal of S's significand and makes it exact with a 128-bit integer
+// remainder, so its result does not depend on how that estimate rounds. Compiles as host C++ and
+// as HIP device code; the host build is checked against native double arithmetic on millions of
+// random and boundary cases, and the device-prepared quantiser against exact integer division.
 //
 // Values are carried as raw binary64 bit patterns (uint64_t). Inputs are finite; results of the
 // additions used here are zero or normal numbers.

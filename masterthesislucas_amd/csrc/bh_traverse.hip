@@ -108,6 +108,16 @@ __device__ __forceinline__ void wave_work(const TraverseBatch& batch, uint32_t g
     }
 }
 
+// Learned far-field correction inside the walk (LTERMS = 3 or 7 basis terms; 0: none): each
+// accepted cell whose l / r is at
+// least `band` (all accepted cells when inv_band2 = 0) gets its monopole corrected by
+// (M l^2 / r^4) sum_j c_j b_j(d / r, l / r), with the cell's coefficient record (kCoefStride
+// floats, computed once per cell and build by the network) read by the wave with scalar loads.
+struct LearnArgs {
+    const float4* coef = nullptr;
+    float inv_band2 = 0.f;
+};
+
 // Traversal variants (picked per workload by the autotuner):
 //   NS:      sinks per lane; a wave walks the tree once for 64 NS Morton-consecutive particles,
 //            so every node fetch feeds NS independent interactions per lane
@@ -129,7 +139,7 @@ __device__ __forceinline__ void wave_work(const TraverseBatch& batch, uint32_t g
 // output; every interaction is counted by exactly one wave. This multiplies the number of waves
 // when there are too few sinks to fill the GPU.
 template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED, bool SPLIT,
-          bool SMASK>
+          bool SMASK, int LTERMS = 0>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
                                                      const BuildParams* __restrict__ prm,
@@ -138,7 +148,11 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                                      float* __restrict__ ay_out,
                                                      float* __restrict__ tot_out,
                                                      unsigned long long* __restrict__ counters,
-                                                     TraverseBatch batch, uint32_t F) {
+                                                     TraverseBatch batch, uint32_t F,
+                                                     LearnArgs la) {
+    constexpr bool LEARN = LTERMS != 0;
+    static_assert(!LEARN || (KERNEL == kKernelGravity && !SMASK && !SPLIT && !BATCHED),
+                  "the learned walk is the vector-mask one-system gravity walk");
     __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
     __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
     const int lane = threadIdx.x & 63;
@@ -180,6 +194,9 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     }
     const uint32_t nn = prm->n_nodes;
     if constexpr (!BATCHED) end0 = nn;
+    // the root side as the node records take it (RN32(S)), for the learned correction's l
+    const float lside = LEARN ? prm->side : 0.f;
+    const bool use_band = LEARN && la.inv_band2 > 0.f;  // else every accepted cell is corrected
 
     // Compensated totals (ax, ay, at) and their Kahan terms; monopoles of consecutive visits are
     // first gathered in a short plain batch (bx, by, bt) that is folded into the compensated
@@ -210,6 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         // the next node is k+1 (descend) or `next` (skip): fetch both while this one is used
         const Node nd_down = nodes[min(k + 1, nn - 1)];
         const Node nd_skip = nodes[min(next, nn - 1)];
+
         bool owned = true;       // this wave adds node k's interactions
         uint64_t enter = ~0ull;  // this wave may open node k
         if constexpr (SPLIT) {
@@ -239,6 +257,24 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                 if (owned && (!BRANCHY || __ballot(acc))) {
                     accum<KERNEL, TOT>(dx, dy, d2, cur.mass, eps2, acc, bx[s], by[s], bt[s]);
                     npn[s] += acc ? 1u : 0u;
+                    if constexpr (LEARN) {
+                        const bool corr = use_band ? acc && !(d2 > cur.l2 * la.inv_band2) : acc;
+                        if (!use_band || __ballot(corr)) {
+                            const float4* cr = la.coef + (size_t)k * (kCoefStride / 4);
+                            const float4 c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
+                            if (corr) {
+                                const float rec[kCoefStride] = {
+                                    c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w,
+                                    c2.x, c2.y, c2.z, c2.w, c3.x, c3.y, c3.z, c3.w};
+                                const float l = ldexpf(lside, -(int)(cur.meta & kMetaLevelMask));
+                                float gx, gy;
+                                learned_correction<LTERMS>(dx, dy, d2, l, cur.mass * l * l, rec,
+                                                           gx, gy);
+                                bx[s] += gx;
+                                by[s] += gy;
+                            }
+                        }
+                    }
                 }
                 open[s] = __ballot(on && !acc) & enter;
             }
@@ -422,190 +458,6 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------
-// Traversal with the learned far-field correction (gravity). The walk and every acceptance
-// decision are those of k_traverse; in addition, each accepted cell whose l / r is at least
-// `band` (all accepted cells when band = 0) gets its monopole corrected by
-// (M l^2 / r^4) sum_j c_j b_j(d / r, l / r), with the cell's coefficient record c (computed once
-// per cell and build by the network, kCoefStride floats) read by the wave with scalar loads.
-// ------------------------------------------------------------------------------------------
-template <int NS, bool TOT>
-__global__ __launch_bounds__(kBlock) void k_traverse_learned(
-    const float4* __restrict__ pos4, const Node* __restrict__ nodes,
-    const float4* __restrict__ coef, const BuildParams* __restrict__ prm, uint32_t n,
-    float inv_theta2, float inv_band2, float eps2, float scale, float* __restrict__ ax_out,
-    float* __restrict__ ay_out, float* __restrict__ tot_out,
-    unsigned long long* __restrict__ counters, TraverseBatch batch) {
-    __shared__ uint64_t st_mask[kWavesPerBlock][kStackDepth][NS];
-    __shared__ uint32_t st_end[kWavesPerBlock][kStackDepth];
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const uint32_t wb = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t base, count, k0, end0;
-    wave_work(batch, wb * kWavesPerBlock + w, n, NS, nodes, prm, base, count, k0, end0);
-    uint32_t idx[NS];
-    uint64_t onm[NS];
-    float px[NS], py[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        idx[s] = base + s * kWave + lane;
-        const bool valid = (uint32_t)(s * kWave + lane) < count;
-        onm[s] = uniform_u64(__ballot(valid));
-        const float4 me = valid ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        px[s] = me.x;
-        py[s] = me.y;
-    }
-    float ax[NS], ay[NS], at[NS], cax[NS], cay[NS], cat[NS], bx[NS], by[NS], bt[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) ax[s] = ay[s] = at[s] = cax[s] = cay[s] = cat[s] = bx[s] =
-        by[s] = bt[s] = 0.f;
-    uint64_t cnt_pn = 0, cnt_pp = 0;
-    const float side = uniform_f32(prm->side);
-    const uint32_t last = uniform_u32(prm->n_nodes) - 1;
-    uint32_t k = k0, end_cur = end0;
-    int sp = 0, since_fold = 0;
-    for (;;) {
-        while (k >= end_cur && sp > 0) {
-            --sp;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) onm[s] = uniform_u64(st_mask[w][sp][s]);
-            end_cur = uniform_u32(st_end[w][sp]);
-        }
-        if (k >= end_cur) break;
-        uint64_t walking = 0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) walking |= onm[s];
-        if (!walking) {
-            k = end_cur;
-            continue;
-        }
-        const NodeBits cur = load_node(nodes, k);
-        const uint32_t next = uniform_u32(max(cur.skip(), k + 1));
-        const uint32_t meta = uniform_u32(cur.meta());
-        const float lim = cur.l2() * inv_theta2;
-        const float lim_band = cur.l2() * inv_band2;
-        uint64_t openm[NS];
-        uint64_t open_any = 0, acc_any = 0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const float dx = cur.cx() - px[s], dy = cur.cy() - py[s];
-            const float d2 = fmaf(dx, dx, dy * dy);
-            const uint64_t accm = uniform_u64(__ballot(d2 > lim) & onm[s]);
-            if (accm) {
-                const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
-                accum<kKernelGravity, TOT>(dx, dy, d2, cur.mass(), eps2, acc, bx[s], by[s], bt[s]);
-                cnt_pn += __popcll(accm);
-                const uint64_t bandm =
-                    uniform_u64(inv_band2 > 0.f ? (accm & __ballot(!(d2 > lim_band))) : accm);
-                if (bandm) {
-                    const float4* cr = coef + (size_t)k * (kCoefStride / 4);
-                    const float4 c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
-                    if (__builtin_amdgcn_inverse_ballot_w64(bandm)) {
-                        const float rec[kCoefStride] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y,
-                                                        c1.z, c1.w, c2.x, c2.y, c2.z, c2.w,
-                                                        c3.x, c3.y, c3.z, c3.w};
-                        const float l = ldexpf(side, -(int)(meta & kMetaLevelMask));
-                        float gx, gy;
-                        learned_correction(dx, dy, d2, l, cur.mass() * l * l, rec, gx, gy);
-                        bx[s] += gx;
-                        by[s] += gy;
-                    }
-                }
-            }
-            openm[s] = onm[s] & ~accm;
-            open_any |= openm[s];
-            acc_any |= accm;
-        }
-        const bool is_pp = (meta & kMetaPP) != 0;
-        const bool down = open_any && !is_pp;
-        const uint32_t knext = uniform_u32(down ? k + 1 : next);
-        const uint32_t touch = nodes[min(knext, last)].meta;
-        if (open_any) {
-            if (is_pp) {
-                float lx[NS], ly[NS], lt[NS];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) lx[s] = ly[s] = lt[s] = 0.f;
-                const uint32_t f0 = uniform_u32(cur.first()), cnt = uniform_u32(cur.count());
-                const uint32_t e0 = f0 + cnt;
-                bool mine[NS];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) mine[s] = __builtin_amdgcn_inverse_ballot_w64(uniform_u64(openm[s]));
-                for (uint32_t j = f0; j < e0; ++j) {
-                    const float4 q = pos4[uniform_u32(j)];
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        const float qx = q.x - px[s], qy = q.y - py[s];
-                        const float q2 = fmaf(qx, qx, qy * qy);
-                        const bool use = mine[s] && j != idx[s];
-                        accum<kKernelGravity, TOT>(qx, qy, q2, q.z, eps2, use, lx[s], ly[s], lt[s]);
-                    }
-                }
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    const uint64_t self = __ballot(idx[s] >= f0 && idx[s] < e0) & openm[s];
-                    cnt_pp += (uint64_t)__popcll(openm[s]) * cnt - (uint64_t)__popcll(self);
-                    kahan_add(ax[s], cax[s], lx[s] + bx[s]);
-                    kahan_add(ay[s], cay[s], ly[s] + by[s]);
-                    if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
-                    bx[s] = by[s] = bt[s] = 0.f;
-                }
-                since_fold = 0;
-            } else {
-                if (acc_any && sp < kStackDepth) {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (lane == 0) st_mask[w][sp][s] = onm[s];
-                        onm[s] = openm[s];
-                    }
-                    if (lane == 0) st_end[w][sp] = end_cur;
-                    ++sp;
-                    end_cur = next;
-                }
-            }
-        }
-        if (++since_fold >= kFoldEvery) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                kahan_add(ax[s], cax[s], bx[s]);
-                kahan_add(ay[s], cay[s], by[s]);
-                if (TOT) kahan_add(at[s], cat[s], bt[s]);
-                bx[s] = by[s] = bt[s] = 0.f;
-            }
-            since_fold = 0;
-        }
-        asm volatile("; prefetched %0" ::"s"(touch));
-        k = knext;
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        kahan_add(ax[s], cax[s], bx[s]);
-        kahan_add(ay[s], cay[s], by[s]);
-        if (TOT) kahan_add(at[s], cat[s], bt[s]);
-        if ((uint32_t)(s * kWave + lane) < count) {
-            ax_out[idx[s]] = scale * (ax[s] - cax[s]);
-            ay_out[idx[s]] = scale * (ay[s] - cay[s]);
-            if (TOT) tot_out[idx[s]] = scale * (at[s] - cat[s]);
-        }
-    }
-    if (counters) {
-        __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
-        if (lane == 0) {
-            s_cnt[w][0] = cnt_pn;
-            s_cnt[w][1] = cnt_pp;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long a = 0, b = 0;
-            for (int k2 = 0; k2 < kWavesPerBlock; ++k2) {
-                a += s_cnt[k2][0];
-                b += s_cnt[k2][1];
-            }
-            counters[2 * (size_t)wb] = a;
-            counters[2 * (size_t)wb + 1] = b;
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------------
 // FP64 reduction of per-particle totals (t-SNE Z, potential energy): fixed partition into
 // block partials, then one block sums them in order, so the result is deterministic.
 // ------------------------------------------------------------------------------------------
@@ -731,7 +583,7 @@ static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, 
 #define BH_TRAV(NS, BR, U) BH_TRAV_M(NS, BR, U, false)
 #define BH_TRAV_M(NS, BR, U, M)                                                                   \
     k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false, M><<<g, kBlock, 0, s>>>(                   \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch, 1u)
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch, 1u, LearnArgs{})
     if (variant >= kTraverseVariantsVector) {
         // the same 18 shapes with scalar lane masks
         const int v = variant - kTraverseVariantsVector;
@@ -782,7 +634,7 @@ static uint32_t traverse_split(hipStream_t s, int variant, uint32_t F, uint32_t 
     TraverseBatch none;
 #define BH_TRAVS(NS, BR, U)                                                                       \
     k_traverse<KERNEL, TOT, NS, BR, U, false, true, false><<<g, kBlock, 0, s>>>(                  \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F)
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F, LearnArgs{})
 #define BH_TRAVS_U(NS, BR)                                                                        \
     if (lane_pp) BH_TRAVS(NS, BR, 0); else BH_TRAVS(NS, BR, 1)
     const bool br = variant_branchy(variant);
@@ -841,20 +693,28 @@ uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* p
 uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
                                  const float4* coef, const BuildParams* prm, uint32_t n,
                                  float inv_theta2, float inv_band2, float eps2, float scale,
-                                 float* ax, float* ay, float* tot,
-                                 unsigned long long* counters, TraverseBatch batch) {
-    const int ns = variant_ns(variant) == 1 ? 1 : 2;
-    const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
-                                  : div_up(n, (uint64_t)kBlock * ns);
-#define BH_TRAVL(NS, TOT)                                                                         \
-    k_traverse_learned<NS, TOT><<<g, kBlock, 0, s>>>(pos4, nodes, coef, prm, n, inv_theta2,      \
-                                                      inv_band2, eps2, scale, ax, ay, tot,        \
-                                                      counters, batch)
+                                 int terms, float* ax, float* ay, float* tot,
+                                 unsigned long long* counters) {
+    // the variant's sinks per lane; branchy monopoles, one source per particle-particle fetch
+    const int ns = variant_ns(variant);
+    const uint32_t g = div_up(n, (uint64_t)kBlock * ns);
+    LearnArgs la;
+    la.coef = coef;
+    la.inv_band2 = inv_band2;
+    TraverseBatch none;
+#define BH_TRAVL(NS, TOT, T)                                                                      \
+    k_traverse<kKernelGravity, TOT, NS, true, 1, false, false, false, T><<<g, kBlock, 0, s>>>(    \
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, 1u, la)
+#define BH_TRAVL_T(NS, TOT) \
+    if (terms == 3) BH_TRAVL(NS, TOT, 3); else BH_TRAVL(NS, TOT, 7)
     if (ns == 1) {
-        if (tot) BH_TRAVL(1, true); else BH_TRAVL(1, false);
+        if (tot) BH_TRAVL_T(1, true); else BH_TRAVL_T(1, false);
+    } else if (ns == 2) {
+        if (tot) BH_TRAVL_T(2, true); else BH_TRAVL_T(2, false);
     } else {
-        if (tot) BH_TRAVL(2, true); else BH_TRAVL(2, false);
+        if (tot) BH_TRAVL_T(4, true); else BH_TRAVL_T(4, false);
     }
+#undef BH_TRAVL_T
 #undef BH_TRAVL
     return g;
 }

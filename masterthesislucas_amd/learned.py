@@ -88,6 +88,15 @@ class FarFieldMLP:
         p = np.ascontiguousarray(flat, np.float32)
         self._c(_lib.load().bh_mlp_set_params(self._h, _ptr(p), int(p.size)))
 
+    @property
+    def terms(self):
+        """Basis terms of the loss and of the walk's correction (7, or 3: quadrupole forms)."""
+        return int(_lib.load().bh_mlp_get_terms(self._h))
+
+    @terms.setter
+    def terms(self, t):
+        self._c(_lib.load().bh_mlp_set_terms(self._h, int(t)))
+
     # compute ----------------------------------------------------------------------------
     def forward(self, X, mfma=False):
         """Coefficients [n x 14] of cell features X [n x 10]."""

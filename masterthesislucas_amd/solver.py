@@ -192,6 +192,12 @@ class BHContext:
         self._c(_lib.load().bh_get_bounds(self._h, _ptr(b), C.byref(side)))
         return b, side.value
 
+    def quantizer(self):
+        """(M, es, R) of the device-prepared quantiser of the last build (bh_get_quantizer)."""
+        M, R, es = C.c_uint64(0), C.c_uint64(0), C.c_int32(0)
+        self._c(_lib.load().bh_get_quantizer(self._h, C.byref(M), C.byref(es), C.byref(R)))
+        return M.value, es.value, R.value
+
     def morton(self):
         codes = np.empty(self.n, np.uint64)
         perm = np.empty(self.n, np.uint32)

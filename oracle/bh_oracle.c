@@ -1755,7 +1755,7 @@ void orc_pair_label(const double* xs, const double* ys, const double* ms, uint32
     out[5] = scale;
 }
 
-double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h,
+double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h, int terms,
                              const double* W1, const double* b1, const double* W2,
                              const double* b2, const double* W3, const double* b3, double* gW1,
                              double* gb1, double* gW2, double* gb2, double* gW3, double* gb3) {
@@ -1774,6 +1774,7 @@ double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h,
         const double* xin = X + s * dx_;
         mlp_fwd_one(xin, din, h, dout, W1, b1, W2, b2, W3, b3, z1, a1, z2, a2, y);
         orc_learned_basis(xin[din], xin[din + 1], xin[din + 2], bas);
+        for (int j = terms; j < ORC_LEARNED_TERMS; ++j) bas[2 * j] = bas[2 * j + 1] = 0.0;
         double pr, pi;
         learned_apply(y, bas, &pr, &pi);
         const double ex = pr - T[2 * s], ey = pi - T[2 * s + 1];

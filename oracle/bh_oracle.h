@@ -212,9 +212,10 @@ void orc_learned_basis(double ux, double uy, double q, double* b);
 void orc_pair_label(const double* xs, const double* ys, const double* ms, uint32_t first,
                     uint32_t count, double cx, double cy, double mass, double l, double px,
                     double py, int64_t self, double eps2, double* out);
-/* loss sum |sum_j c_j b_j - T|^2 / (2 B) over samples X[B x 13] (10 cell features, w_x, w_y, q)
- * and labels T[B x 2], with c = MLP(10 -> h -> h -> 14) of the cell features; gradients */
-double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h,
+/* loss sum |sum_{j < terms} c_j b_j - T|^2 / (2 B) over samples X[B x 13] (10 cell features,
+ * w_x, w_y, q) and labels T[B x 2], with c = MLP(10 -> h -> h -> 14) of the cell features;
+ * gradients. terms: 7, or 3 (the quadrupole forms only; the other outputs get no gradient) */
+double orc_learned_loss_grad(const double* X, const double* T, int64_t B, int h, int terms,
                              const double* W1, const double* b1, const double* W2,
                              const double* b2, const double* W3, const double* b3, double* gW1,
                              double* gb1, double* gW2, double* gb2, double* gW3, double* gb3);

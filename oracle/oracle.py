@@ -68,7 +68,7 @@ def lib():
             "orc_learned_basis": (None, [D, D, D, _vp]),
             "orc_pair_label": (None, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, D, D, D, D, D, D,
                                       I64, D, _vp]),
-            "orc_learned_loss_grad": (D, [_vp, _vp, I64, I32] + [_vp] * 12),
+            "orc_learned_loss_grad": (D, [_vp, _vp, I64, I32, I32] + [_vp] * 12),
             "orc_bh_traverse_learned": (None, [_vp, _vp, _vp, I64] + [_vp] * 8
                                         + [I64, C.c_float, C.c_float, D, D, _vp, C.c_float, _vp,
                                            I64, _vp, _vp, _vp]),
@@ -505,14 +505,14 @@ def pair_label(xs, ys, ms, first, count, cx, cy, mass, l, px, py, self_index=-1,
     return out
 
 
-def learned_loss_grad(X, T, params):
+def learned_loss_grad(X, T, params, terms=LEARNED_TERMS):
     X, T = _f64(X), _f64(T)
     B = X.shape[0]
     W1, b1, W2, b2, W3, b3 = (_f64(params[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3"))
     g = {k: np.empty_like(v) for k, v in (("W1", W1), ("b1", b1), ("W2", W2), ("b2", b2),
                                           ("W3", W3), ("b3", b3))}
-    loss = lib().orc_learned_loss_grad(_p(X), _p(T), B, W1.shape[1], _p(W1), _p(b1), _p(W2),
-                                       _p(b2), _p(W3), _p(b3), _p(g["W1"]), _p(g["b1"]),
+    loss = lib().orc_learned_loss_grad(_p(X), _p(T), B, W1.shape[1], int(terms), _p(W1), _p(b1),
+                                       _p(W2), _p(b2), _p(W3), _p(b3), _p(g["W1"]), _p(g["b1"]),
                                        _p(g["W2"]), _p(g["b2"]), _p(g["W3"]), _p(g["b3"]))
     return float(loss), g
 

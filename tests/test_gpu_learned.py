@@ -85,8 +85,11 @@ def test_mlp_forward_mfma_bf16_within_bf16_bounds(gpu):
     net.close()
 
 
-def test_learned_loss_gradients_match_oracle(gpu):
+@pytest.mark.parametrize("terms", [3, 7])
+def test_learned_loss_gradients_match_oracle(gpu, terms):
     net = FarFieldMLP(max_samples=16)
+    net.terms = terms
+    assert net.terms == terms
     p = _params(5)
     net.set_params(flatten(p))
     X = _X(3000, 6, cols=13)
@@ -95,7 +98,8 @@ def test_learned_loss_gradients_match_oracle(gpu):
     X[:, 12] = np.random.default_rng(7).uniform(0.05, 1.0, 3000).astype(np.float32)
     T = np.random.default_rng(8).normal(0, 0.5, (3000, 2)).astype(np.float32)
     loss, g = net.loss_grad(X, T)
-    ref_loss, ref_g = O.learned_loss_grad(X.astype(np.float64), T.astype(np.float64), _f64(p))
+    ref_loss, ref_g = O.learned_loss_grad(X.astype(np.float64), T.astype(np.float64), _f64(p),
+                                          terms=terms)
     assert abs(loss - ref_loss) < 1e-5 * ref_loss
     gg = unflatten(g)
     for k in ref_g:
@@ -285,15 +289,18 @@ def test_collect_labels_match_oracle(gpu):
     ctx.close()
 
 
-@pytest.mark.parametrize("variant,band", [(0, 0.0), (4, 0.0), (4, 0.5)])
-def test_learned_walk_matches_oracle(gpu, variant, band):
+@pytest.mark.parametrize("variant,band,terms", [(0, 0.0, 7), (4, 0.0, 7), (4, 0.5, 7),
+                                                (4, 0.0, 3), (8, 0.5, 3)])
+def test_learned_walk_matches_oracle(gpu, variant, band, terms):
     """A network whose last layer is zero outputs its bias for every cell: the corrected walk
-    with those constant coefficients against orc_bh_traverse_learned on the GPU's tree."""
+    with those constant coefficients against orc_bh_traverse_learned on the GPU's tree (with the
+    coefficients of unused terms zero for a three-term network)."""
     n, theta = 30000, 0.9
     ctx, p = _plummer_ctx(n, seed=34)
     ctx.set_variant(variant)
     t = _gpu_tree(ctx, p)
     net = FarFieldMLP(max_samples=16)
+    net.terms = terms
     prm = _params(35)
     prm["W3"][:] = 0.0
     prm["b3"][:] = np.random.default_rng(36).uniform(-1, 1, N_OUT).astype(np.float32)
@@ -302,7 +309,9 @@ def test_learned_walk_matches_oracle(gpu, variant, band):
     assert np.array_equal(rec, np.broadcast_to(_records(prm["b3"][None, :])[0], rec.shape))
     ax, ay, _ = ctx.forces_learned(net, theta, band=band)
     pn, pp = ctx.counters()
-    coef = np.broadcast_to(prm["b3"].astype(np.float64), (t["tree"]["first"].size, N_OUT))
+    c = prm["b3"].astype(np.float64)
+    c[2 * terms:] = 0.0
+    coef = np.broadcast_to(c, (t["tree"]["first"].size, N_OUT))
     inv_b2 = 0.0 if band == 0 else 1.0 / (band * band)
     bx, by, cnt = O.bh_traverse_learned(t["xs"], t["ys"], t["ms"], t["tree"], t["side"],
                                         O.inv_theta2_f32(theta), coef, inv_band2=inv_b2,
@@ -350,12 +359,14 @@ def test_learned_requires_quadtree_gravity(gpu):
     ctx.close()
 
 
-def test_trained_network_beats_plain_monopole(gpu):
+@pytest.mark.parametrize("terms", [7, 3])
+def test_trained_network_beats_plain_monopole(gpu, terms):
     """Trained on one Plummer realisation, evaluated on another: at theta = 0.9 the learned walk
     must cut the plain walk's median error against the FP64 direct sum at least threefold."""
     theta = 0.9
     ctx, _ = _plummer_ctx(150000, seed=23)
     net = FarFieldMLP(max_samples=1 << 21, seed=24)
+    net.terms = terms
     added = net.collect(ctx, theta, sample_indices(150000, 6000, seed=25).astype(np.uint32))
     assert added > 100000
     loss = net.train(epochs=12, batch=4096, lr=2e-3, seed=26)
@@ -371,7 +382,7 @@ def test_trained_network_beats_plain_monopole(gpu):
     plain = err(*ctx.forces(theta, want_total=False)[:2])
     plain05 = err(*ctx.forces(0.5, want_total=False)[:2])
     learned = err(*ctx.forces_learned(net, theta)[:2])
-    print(f"[learned] theta={theta} pairs={added} loss={loss:.3e} plain={plain:.3e} "
+    print(f"[learned] terms={terms} theta={theta} pairs={added} loss={loss:.3e} plain={plain:.3e} "
           f"learned={learned:.3e} plain(0.5)={plain05:.3e}", flush=True)
     assert learned < plain / 3, (plain, learned)
     net.close()

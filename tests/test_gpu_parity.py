@@ -114,6 +114,31 @@ def test_sort_nearly_sorted_input(gpu, mode, n, order):
     ctx.close()
 
 
+@pytest.mark.parametrize("mode", ["quadtree", "morton_fixed"])
+def test_device_quantizer_exact(gpu, mode):
+    """The quantiser each build prepares on the device from the root side S (an FP64 estimate of
+    1 / M made exact with a 128-bit remainder) against exact integer division: S = M 2^es and
+    R = floor((2^116 - 1) / M), for sides over many magnitudes and significands, powers of two
+    (M = 2^52) included."""
+    rng = np.random.default_rng(77)
+    spans = [2.0 ** k for k in range(-20, 21, 5)]
+    spans += list(rng.uniform(0.5, 2.0, 40) * 10.0 ** rng.integers(-6, 7, 40))
+    spans += [np.nextafter(np.float32(1.0), np.float32(2.0)), np.float32(3.0), np.float32(0.1)]
+    ctx = _ctx(2, kernel="gravity", tree_mode=mode)
+    for w in spans:
+        x = np.array([-0.25, np.float32(w)], np.float32)
+        y = np.array([0.5, np.float32(0.5 + 0.3 * w)], np.float32)
+        ctx.set_particles(x, y)
+        ctx.build()
+        _, side = ctx.bounds()
+        bits = int(np.array(side, np.float64).view(np.uint64))
+        M, es, R = ctx.quantizer()
+        assert M == (bits & ((1 << 52) - 1)) | (1 << 52)
+        assert es == ((bits >> 52) & 0x7FF) - 1075
+        assert R == ((1 << 116) - 1) // M, (w, side, M, R)
+    ctx.close()
+
+
 def test_morton32_caller_bounds(gpu):
     """updateTree honours the caller's minPos/maxPos in the Morton solvers."""
     p = uniform_disk(4096, seed=3)
@@ -353,6 +378,12 @@ def test_coincident_points_depth_limit(gpu):
     ax, ay, _ = ctx.forces(0.5)
     assert np.all(np.isfinite(ax)) and np.all(np.isfinite(ay))
     t = oracle_tree(x, y, None)
+    # every node's moments, the overfull leaves (100 and 50 points) and their ancestors included
+    g = ctx.tree()
+    assert nn == t["tree"]["first"].size
+    assert np.array_equal(g["mass"], t["mom"]["mass"].astype(np.float32))
+    assert np.allclose(g["cx"], t["mom"]["cx"], rtol=0, atol=1e-6)
+    assert np.allclose(g["cy"], t["mom"]["cy"], rtol=0, atol=1e-6)
     bx, by, _, _ = oracle_forces(t, O.KERNEL_GRAVITY, 0.5, eps=EPS)
     r = rel_err(ax, ay, bx, by)
     assert np.median(r) <= MEDIAN_TOL

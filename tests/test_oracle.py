@@ -773,7 +773,8 @@ def test_pair_label_one_particle_and_self():
     assert lab_self[0] * lab_self[5] == pytest.approx(1.0 - 8.0, rel=1e-14)
 
 
-def test_learned_loss_gradient_finite_differences():
+@pytest.mark.parametrize("terms", [3, 7])
+def test_learned_loss_gradient_finite_differences(terms):
     rng = np.random.default_rng(8)
     h = 6
     p = {"W1": rng.normal(0, 0.5, (10, h)), "b1": rng.normal(0, 0.1, h),
@@ -783,17 +784,24 @@ def test_learned_loss_gradient_finite_differences():
     ang = rng.uniform(0, 2 * np.pi, 7)
     X[:, 10], X[:, 11], X[:, 12] = np.cos(ang), np.sin(ang), rng.uniform(0.1, 0.9, 7)
     T = rng.normal(0, 1, (7, 2))
-    loss, g = O.learned_loss_grad(X, T, p)
+    loss, g = O.learned_loss_grad(X, T, p, terms=terms)
     for name in ("W1", "b2", "W3", "b3"):
         flat = p[name].reshape(-1)
-        for idx in (0, flat.size // 2, flat.size - 1):
+        for idx in (0, 5, flat.size // 2, flat.size - 1):
             old = flat[idx]
             flat[idx] = old + 1e-6
-            lp, _ = O.learned_loss_grad(X, T, p)
+            lp, _ = O.learned_loss_grad(X, T, p, terms=terms)
             flat[idx] = old - 1e-6
-            lm, _ = O.learned_loss_grad(X, T, p)
+            lm, _ = O.learned_loss_grad(X, T, p, terms=terms)
             flat[idx] = old
             assert g[name].reshape(-1)[idx] == pytest.approx((lp - lm) / 2e-6, rel=1e-5, abs=1e-9)
+    if terms == 3:
+        # the outputs of the unused terms get no gradient, and the loss does not see them
+        assert np.all(g["W3"][:, 6:] == 0) and np.all(g["b3"][6:] == 0)
+        q = {k: v.copy() for k, v in p.items()}
+        q["W3"][:, 6:] = 0.0
+        q["b3"][6:] = 0.0
+        assert O.learned_loss_grad(X, T, q, terms=7)[0] == pytest.approx(loss, rel=1e-14)
 
 
 def test_learned_walk_zero_and_quadrupole_coefficients():
