@@ -202,6 +202,21 @@ bh_status bh_tsne_step(bh_ctx* ctx, const bh_tsne_params* p, int32_t iteration, 
                        float* dy, double* z);
 /* iterations first .. first+count-1 without host synchronisation in between */
 bh_status bh_tsne_run(bh_ctx* ctx, const bh_tsne_params* p, int32_t first, int32_t count);
+/* Input affinities of t-SNE from points X[n x dim] (FP32, row-major, dim <= 1024): exact k
+ * nearest neighbours by Euclidean distance (self excluded, ties by index; FP16 MFMA candidates
+ * with rigorous rounding bounds, FP64 distances decide), per-point Gaussian bandwidths beta_i
+ * bisected until the entropy (natural log) of p_{j|i} = exp(-beta_i (d2_ij - d2_i0)) / sum is
+ * within 1e-5 of log(perplexity) (at most 200 steps), and P = (P_cond + P_cond^T) / (2n) as CSR
+ * sorted by (row, column): row_ptr[n + 1], col and val with room for 2 n k entries, *nnz set.
+ * The reference computes P offline with openTSNE (MnistLoader.ipynb:539-547) and reads it as a
+ * Matrix Market file (tsne.h:94-98, setupEfficientPmatrix tsne.h:462-520). Optional outputs:
+ * knn_idx[n x k], knn_d2[n x k] (ascending), beta[n]. 1 <= k <= 128, k < n. Returns
+ * BH_ERR_CAPACITY when some point has more than 192 candidates within the FP16 error bound of
+ * its k-th neighbour (hundreds of exact or near duplicates). */
+bh_status bh_tsne_affinities(bh_ctx* ctx, const float* X, uint32_t n, uint32_t dim,
+                             double perplexity, uint32_t k, uint32_t* row_ptr, uint32_t* col,
+                             float* val, uint32_t* nnz, uint32_t* knn_idx, double* knn_d2,
+                             double* beta);
 /* KL divergence of the current embedding with the exact FP64 Z (costFunction,
  * tsne.h:383-460; O(N^2) on the GPU in FP64) */
 bh_status bh_tsne_kl(bh_ctx* ctx, double* kl, double* z_exact);

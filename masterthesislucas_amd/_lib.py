@@ -76,6 +76,8 @@ SIGNATURES = {
     "bh_tsne_step": (_i32, [_vp, C.POINTER(TsneParams), _i32, _vp, _vp, _vp]),
     "bh_tsne_run": (_i32, [_vp, C.POINTER(TsneParams), _i32, _i32]),
     "bh_tsne_kl": (_i32, [_vp, _vp, _vp]),
+    "bh_tsne_affinities": (_i32, [_vp, _vp, _u32, _u32, _d, _u32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp]),
     "bh_mlp_param_count": (_u32, []),
     "bh_mlp_create": (_i32, [_vp, _u32, _u64]),
     "bh_mlp_destroy": (None, [_vp]),

@@ -1217,6 +1217,122 @@ bh_status bh_tsne_run(bh_ctx* c, const bh_tsne_params* p, int32_t first, int32_t
     return finish_build_call(c, "t-SNE run");
 }
 
+bh_status bh_tsne_affinities(bh_ctx* c, const float* X, uint32_t n, uint32_t dim,
+                             double perplexity, uint32_t k, uint32_t* row_ptr, uint32_t* col,
+                             float* val, uint32_t* nnz, uint32_t* knn_idx, double* knn_d2,
+                             double* beta) {
+    if (!c || !X || !row_ptr || !col || !val || !nnz) return BH_ERR_ARG;
+    if (n < 2 || k < 1 || k >= n || k > 128)
+        return fail(c, BH_ERR_ARG, "need 1 <= k <= 128 and k < n (k = %u, n = %u)", k, n);
+    if (dim < 1 || dim > 1024) return fail(c, BH_ERR_ARG, "dimension %u outside [1, 1024]", dim);
+    if (!(perplexity > 0.0)) return fail(c, BH_ERR_ARG, "perplexity must be positive");
+    const uint64_t nk = (uint64_t)n * k, m64 = 2 * nk;
+    if (m64 >= (1ull << 31) || (uint64_t)n * n >= (1ull << 62))
+        return fail(c, BH_ERR_ARG, "%u points x %u neighbours is too large", n, k);
+    const uint32_t m = (uint32_t)m64;
+    // scale by a power of two so that the largest |x| lands in [2^13, 2^14): no FP16 overflow
+    double maxabs = 0.0;
+    for (uint64_t e = 0; e < (uint64_t)n * dim; ++e) {
+        const double v = std::fabs((double)X[e]);
+        if (!(v <= 3.4e38)) return fail(c, BH_ERR_ARG, "input coordinate %llu is not finite",
+                                        (unsigned long long)e);
+        maxabs = std::max(maxabs, v);
+    }
+    int ex = 0;
+    if (maxabs > 0.0) (void)std::frexp(maxabs, &ex);
+    const float scale = maxabs > 0.0 ? std::ldexp(1.0f, 14 - ex) : 1.0f;
+    const uint32_t dp = aff_dim_padded(dim), cap = aff_cap();
+    hipStream_t s = c->stream;
+    std::vector<void*> tmp;
+    auto alloc = [&](void** p, size_t bytes) -> bh_status {
+        *p = nullptr;
+        hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+        if (e != hipSuccess) return hip_check(c, e, "hipMalloc (affinities)");
+        tmp.push_back(*p);
+        return BH_OK;
+    };
+    float *dX = nullptr, *nh = nullptr, *ah = nullptr, *eb = nullptr, *clo = nullptr, *chi = nullptr;
+    float* dval = nullptr;
+    _Float16* Xh = nullptr;
+    uint32_t *cj = nullptr, *ccount = nullptr, *flags = nullptr, *idx = nullptr, *vals_a = nullptr,
+             *vals_b = nullptr, *head = nullptr, *pos = nullptr, *spart = nullptr,
+             *scratch = nullptr, *dcol = nullptr, *drp = nullptr;
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    double *d2 = nullptr, *P = nullptr, *dbeta = nullptr;
+    const size_t nsp = scan_partials_size(m);
+    bh_status st = BH_OK;
+    st = alloc((void**)&dX, (size_t)n * dim * 4);
+    if (st == BH_OK) st = alloc((void**)&Xh, (size_t)n * dp * 2);
+    if (st == BH_OK) st = alloc((void**)&nh, (size_t)n * 4);
+    if (st == BH_OK) st = alloc((void**)&ah, (size_t)n * 4);
+    if (st == BH_OK) st = alloc((void**)&eb, (size_t)n * 4);
+    if (st == BH_OK) st = alloc((void**)&cj, (size_t)n * cap * 4);
+    if (st == BH_OK) st = alloc((void**)&clo, (size_t)n * cap * 4);
+    if (st == BH_OK) st = alloc((void**)&chi, (size_t)n * cap * 4);
+    if (st == BH_OK) st = alloc((void**)&ccount, (size_t)n * 4);
+    if (st == BH_OK) st = alloc((void**)&flags, 4 * 4);  // [0] rows over capacity, [1] sort error, [2] nnz
+    if (st == BH_OK) st = alloc((void**)&idx, nk * 4);
+    if (st == BH_OK) st = alloc((void**)&d2, nk * 8);
+    if (st == BH_OK) st = alloc((void**)&P, nk * 8);
+    if (st == BH_OK) st = alloc((void**)&dbeta, (size_t)n * 8);
+    if (st == BH_OK) st = alloc((void**)&keys_a, (size_t)m * 8);
+    if (st == BH_OK) st = alloc((void**)&keys_b, (size_t)m * 8);
+    if (st == BH_OK) st = alloc((void**)&vals_a, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&vals_b, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&head, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&pos, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&spart, (nsp + 1) * 4);
+    if (st == BH_OK) st = alloc((void**)&scratch, sort_scratch_words(m) * 4);
+    if (st == BH_OK) st = alloc((void**)&dcol, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&dval, (size_t)m * 4);
+    if (st == BH_OK) st = alloc((void**)&drp, ((size_t)n + 1) * 4);
+    uint32_t h_flags[4] = {0, 0, 0, 0};
+    if (st == BH_OK) {
+        st = hip_check(c, hipMemcpyAsync(dX, X, (size_t)n * dim * 4, hipMemcpyHostToDevice, s), "X");
+    }
+    if (st == BH_OK) {
+        launch_fill_u32(s, flags, 4, 0u);
+        launch_aff_pack(s, dX, n, dim, scale, Xh, nh, ah, eb);
+        launch_aff_candidates(s, Xh, nh, ah, eb, n, dim, k, cj, clo, chi, ccount, flags);
+        launch_aff_refine(s, dX, n, dim, k, cj, ccount, idx, d2);
+        launch_aff_bandwidth(s, d2, n, k, std::log(perplexity), 1e-5, 200, P, dbeta);
+        launch_sym_entries(s, idx, n, k, keys_a, vals_a);
+        int key_bits = 1;
+        while (key_bits < 64 && ((uint64_t)n * n - 1) >> key_bits) ++key_bits;
+        bool in_b = false;
+        launch_radix_sort(s, keys_a, vals_a, keys_b, vals_b, m, key_bits, true, scratch, flags + 1,
+                          kSortSpinLimit, &in_b);
+        const uint64_t* ks = in_b ? keys_b : keys_a;
+        const uint32_t* vs = in_b ? vals_b : vals_a;
+        launch_sym_heads(s, ks, m, head);
+        launch_exclusive_scan(s, head, pos, m, spart, flags + 2);
+        launch_sym_reduce(s, ks, vs, pos, m, n, (uint32_t)nk, P, dcol, dval, drp, flags + 2);
+        st = launch_status(c, "affinities launch");
+    }
+    if (st == BH_OK)
+        st = hip_check(c, hipMemcpyAsync(h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost, s), "flags");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(s), "affinities");
+    if (st == BH_OK && h_flags[0])
+        st = fail(c, BH_ERR_CAPACITY,
+                  "%u points have more than %u candidates within the FP16 error bound of their "
+                  "k-th neighbour (many exact or near duplicates)", h_flags[0], cap - 64);
+    if (st == BH_OK && h_flags[1]) st = fail(c, BH_ERR_HIP, "the radix sort gave up waiting");
+    const uint32_t nz = h_flags[2];
+    if (st == BH_OK && (nz == 0 || nz > m)) st = fail(c, BH_ERR_HIP, "bad nonzero count %u", nz);
+    if (st == BH_OK) {
+        *nnz = nz;
+        st = hip_check(c, hipMemcpy(row_ptr, drp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost), "row_ptr");
+        if (st == BH_OK) st = hip_check(c, hipMemcpy(col, dcol, (size_t)nz * 4, hipMemcpyDeviceToHost), "col");
+        if (st == BH_OK) st = hip_check(c, hipMemcpy(val, dval, (size_t)nz * 4, hipMemcpyDeviceToHost), "val");
+        if (st == BH_OK && knn_idx) st = hip_check(c, hipMemcpy(knn_idx, idx, nk * 4, hipMemcpyDeviceToHost), "idx");
+        if (st == BH_OK && knn_d2) st = hip_check(c, hipMemcpy(knn_d2, d2, nk * 8, hipMemcpyDeviceToHost), "d2");
+        if (st == BH_OK && beta) st = hip_check(c, hipMemcpy(beta, dbeta, (size_t)n * 8, hipMemcpyDeviceToHost), "beta");
+    }
+    (void)hipStreamSynchronize(s);
+    for (void* p : tmp) (void)hipFree(p);
+    return st;
+}
+
 bh_status bh_tsne_kl(bh_ctx* c, double* kl, double* z_exact) {
     if (!c) return BH_ERR_ARG;
     if (!c->ty_set) return fail(c, BH_ERR_STATE, "set P and the embedding first");

@@ -220,4 +220,25 @@ size_t cell_features_lists_words(uint32_t n);
 void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
                        uint32_t cap_nodes, const MlpBf16* w, float4* coef);
 
+// t-SNE input affinities (bh_affinity.hip): FP16 copy padded to aff_dim_padded(dim) halfs per
+// point, candidate buffers of aff_cap() slots per point, exact k nearest by FP64 distance,
+// bandwidths by bisection, symmetrisation keyed by row * n + column
+uint32_t aff_dim_padded(uint32_t dim);
+uint32_t aff_cap();
+void launch_aff_pack(hipStream_t s, const float* X, uint32_t n, uint32_t dim, float scale,
+                     _Float16* Xh, float* nh, float* ah, float* eb);
+void launch_aff_candidates(hipStream_t s, const _Float16* Xh, const float* nh, const float* ah,
+                           const float* eb, uint32_t n, uint32_t dim, uint32_t k, uint32_t* cj,
+                           float* clo, float* chi, uint32_t* ccount, uint32_t* overflow);
+void launch_aff_refine(hipStream_t s, const float* X, uint32_t n, uint32_t dim, uint32_t k,
+                       const uint32_t* cj, const uint32_t* ccount, uint32_t* idx, double* d2);
+void launch_aff_bandwidth(hipStream_t s, const double* d2, uint32_t n, uint32_t k, double target,
+                          double tol, int max_iter, double* P, double* beta);
+void launch_sym_entries(hipStream_t s, const uint32_t* idx, uint32_t n, uint32_t k,
+                        uint64_t* keys, uint32_t* vals);
+void launch_sym_heads(hipStream_t s, const uint64_t* keys, uint32_t m, uint32_t* head);
+void launch_sym_reduce(hipStream_t s, const uint64_t* keys, const uint32_t* vals,
+                       const uint32_t* pos, uint32_t m, uint32_t n, uint32_t nk, const double* P,
+                       uint32_t* col, float* val, uint32_t* row_ptr, const uint32_t* nnz);
+
 }  // namespace bh

@@ -8,17 +8,16 @@ theta schedule (tsne.h:208-233), the derivative (4/Z) * BH repulsion + sparse at
 (tsne.h:383-460). All of an iteration runs on the GPU inside the library (include/bh.h,
 bh_tsne_*); this class only holds parameters and the context.
 
-The reference loads its P matrix from a Matrix Market file prepared offline (tsne.h:94-98,
-Eigen loadMarket). Here the affinities come from `joint_probabilities`: exact k-nearest
-neighbours, a per-point Gaussian bandwidth found by bisection to the requested perplexity, and
-the symmetrisation P = (P_cond + P_cond^T) / (2N), returned in CSR sorted by (row, col) as
-setupEfficientPmatrix builds it (tsne.h:462-520).
+The reference loads its P matrix from a Matrix Market file prepared offline with openTSNE
+(MnistLoader.ipynb:539-547; tsne.h:94-98, Eigen loadMarket). Here the affinities come from
+`affinities` / `joint_probabilities`, computed on the GPU by the library (bh_tsne_affinities):
+exact k-nearest neighbours, a per-point Gaussian bandwidth found by bisection to the requested
+perplexity, and the symmetrisation P = (P_cond + P_cond^T) / (2N), returned in CSR sorted by
+(row, col) as setupEfficientPmatrix builds it (tsne.h:462-520).
 """
 from __future__ import annotations
 
 import ctypes as C
-import math
-
 import numpy as np
 
 from . import _lib
@@ -38,91 +37,38 @@ def gaussian_mixture(n, dim=784, components=10, seed=4523, mean_std=2.0, sigma=1
     return X.astype(np.float32), labels
 
 
-def knn_exact(X, k, device="cpu", chunk=4096):
-    """Exact k nearest neighbours (self excluded) by squared Euclidean distance.
-    Returns (idx int64 [n, k], d2 float64 [n, k]) with neighbours in ascending distance."""
-    import torch
-
-    Xt = torch.as_tensor(np.asarray(X), dtype=torch.float32, device=device)
-    n = Xt.shape[0]
-    if k >= n:
-        raise ValueError(f"k={k} needs more than {n} points")
-    sq = (Xt * Xt).sum(1)
-    idx_out = torch.empty((n, k), dtype=torch.int64, device=device)
-    d_out = torch.empty((n, k), dtype=torch.float64, device=device)
-    ar = torch.arange(n, device=device)
-    for s in range(0, n, chunk):
-        e = min(s + chunk, n)
-        d = sq[s:e, None] + sq[None, :] - 2.0 * (Xt[s:e] @ Xt.T)
-        d[torch.arange(e - s, device=device), ar[s:e]] = float("inf")  # exclude self
-        _, idx = torch.topk(d, k, dim=1, largest=False, sorted=True)
-        idx_out[s:e] = idx
-        # recompute the selected distances in FP64 from the differences (no cancellation)
-        for a in range(s, e, 128):
-            b = min(a + 128, e)
-            diff = Xt[a:b, None, :].double() - Xt[idx[a - s:b - s]].double()
-            d_out[a:b] = (diff * diff).sum(-1)
-    order = torch.argsort(d_out, dim=1, stable=True)
-    return (torch.gather(idx_out, 1, order).cpu().numpy(),
-            torch.gather(d_out, 1, order).cpu().numpy())
+def affinities(X, perplexity=30.0, k=90):
+    """The t-SNE input affinities on the GPU (bh_tsne_affinities): exact k nearest neighbours,
+    per-point bandwidths to the perplexity, and P = (P_cond + P_cond^T) / (2n).
+    Returns dict(row_ptr uint32 [n+1], col uint32 [nnz], val float32 [nnz], idx uint32 [n, k],
+    d2 float64 [n, k], beta float64 [n])."""
+    X = np.ascontiguousarray(X, np.float32)
+    n, dim = X.shape
+    ctx = BHContext(max(int(n), 2), kernel="tsne")
+    try:
+        rp = np.empty(n + 1, np.uint32)
+        col = np.empty(2 * n * k, np.uint32)
+        val = np.empty(2 * n * k, np.float32)
+        idx = np.empty((n, k), np.uint32)
+        d2 = np.empty((n, k), np.float64)
+        beta = np.empty(n, np.float64)
+        nnz = C.c_uint32(0)
+        ctx._c(_lib.load().bh_tsne_affinities(ctx.handle, _ptr(X), int(n), int(dim),
+                                              float(perplexity), int(k), _ptr(rp), _ptr(col),
+                                              _ptr(val), C.byref(nnz), _ptr(idx), _ptr(d2),
+                                              _ptr(beta)))
+    finally:
+        ctx.close()
+    m = nnz.value
+    return dict(row_ptr=rp, col=col[:m].copy(), val=val[:m].copy(), idx=idx, d2=d2, beta=beta)
 
 
-def conditional_probabilities(d2, perplexity, tol=1e-5, max_iter=200):
-    """Row-wise p_{j|i} = exp(-beta_i d2_ij) / sum_j, with beta_i bisected so that the entropy
-    (natural log) equals log(perplexity). d2: float64 [n, k]. Returns (P [n, k], beta [n])."""
-    d2 = np.asarray(d2, dtype=np.float64)
-    n = d2.shape[0]
-    target = math.log(perplexity)
-    dr = d2 - d2[:, :1]  # shift by the row minimum: same P, no underflow
-    beta = np.ones(n)
-    lo = np.full(n, -np.inf)
-    hi = np.full(n, np.inf)
-    done = np.zeros(n, bool)
-    P = None
-    for _ in range(max_iter):
-        E = np.exp(-beta[:, None] * dr)
-        S = E.sum(1)
-        P = E / S[:, None]
-        H = np.log(S) + beta * (dr * P).sum(1)
-        diff = H - target
-        done |= np.abs(diff) < tol
-        if done.all():
-            break
-        up = (diff > 0) & ~done  # entropy too high: sharpen
-        dn = (diff <= 0) & ~done
-        lo = np.where(up, beta, lo)
-        hi = np.where(dn, beta, hi)
-        beta = np.where(up, np.where(np.isinf(hi), beta * 2.0, 0.5 * (beta + hi)), beta)
-        beta = np.where(dn, np.where(np.isinf(lo), beta * 0.5, 0.5 * (beta + lo)), beta)
-    return P, beta
-
-
-def joint_probabilities(X, perplexity=30.0, k=90, device="cpu"):
-    """Symmetrised sparse affinities from exact kNN. Returns CSR (row_ptr uint32 [n+1],
-    col uint32 [nnz], val float32 [nnz]) sorted by (row, col), summing to 1."""
-    idx, d2 = knn_exact(X, k, device=device)
-    Pc, _ = conditional_probabilities(d2, perplexity)
-    return symmetrize(idx, Pc)
-
-
-def symmetrize(idx, Pc):
-    """P = (P_cond + P_cond^T) / (2 n) from neighbour lists, as CSR sorted by (row, col)."""
-    n, k = idx.shape
-    rows = np.repeat(np.arange(n, dtype=np.int64), k)
-    cols = idx.reshape(-1).astype(np.int64)
-    vals = Pc.reshape(-1)
-    r = np.concatenate([rows, cols])
-    c = np.concatenate([cols, rows])
-    v = np.concatenate([vals, vals]) / (2.0 * n)
-    key = r * n + c
-    order = np.argsort(key, kind="stable")
-    key, v = key[order], v[order]
-    uniq, start = np.unique(key, return_index=True)
-    vsum = np.add.reduceat(v, start)
-    ur, uc = uniq // n, uniq % n
-    row_ptr = np.zeros(n + 1, np.uint32)
-    np.cumsum(np.bincount(ur, minlength=n), out=row_ptr[1:])
-    return row_ptr, uc.astype(np.uint32), vsum.astype(np.float32)
+def joint_probabilities(X, perplexity=30.0, k=90, device=None):
+    """Symmetrised sparse affinities of X computed on the GPU. Returns CSR (row_ptr uint32
+    [n+1], col uint32 [nnz], val float32 [nnz]) sorted by (row, col), summing to 1. `device` is
+    accepted for compatibility and ignored: the work runs on the library's device."""
+    a = affinities(X, perplexity, k)
+    return a["row_ptr"], a["col"], a["val"]
 
 
 def initial_embedding(n, seed=4523, std=1e-2):

@@ -556,3 +556,99 @@ def quadrupole_coefficients(features):
     c[:, 2], c[:, 3] = 0.375 * qr, 0.375 * qi
     c[:, 4], c[:, 5] = 1.875 * qr, -1.875 * qi
     return c
+
+
+# ---------------------------------------------------------------------------------------------
+# t-SNE input affinities (NumPy, FP64): the reference computes P offline with openTSNE 1.x
+# (MnistLoader.ipynb:539-547: exact PerplexityBasedNN, k = 3 * perplexity, and
+# joint_probabilities_nn) and reads it as a Matrix Market file (tsne.h:94-98); openTSNE is not
+# importable here, so its published algorithm is restated: exact Euclidean kNN, a per-point
+# bisection on the Gaussian precision beta to the perplexity's entropy, P = (P_c + P_c^T) / (2n).
+# Pinned in tests/test_tsne_host.py: kNN against a full sort, the bisection against the
+# two-point closed form and the entropy property, the symmetrisation against a dense matrix.
+# ---------------------------------------------------------------------------------------------
+def knn_exact(X, k, rows=None, chunk=256):
+    """Exact k nearest neighbours (self excluded) of `rows` (all by default) by FP64 Euclidean
+    distance: candidates from the FP64 Gram identity (the 4k nearest), final distances from the
+    differences, ascending, ties by index. Returns (idx int64 [r, k], d2 float64 [r, k])."""
+    X64 = np.asarray(X, np.float64)
+    n = X64.shape[0]
+    if k >= n:
+        raise ValueError(f"k={k} needs more than {n} points")
+    rows = np.arange(n) if rows is None else np.asarray(rows, np.int64)
+    sq = (X64 * X64).sum(1)
+    m = min(n - 1, 4 * k)
+    idx = np.empty((rows.size, k), np.int64)
+    d2 = np.empty((rows.size, k))
+    for s in range(0, rows.size, chunk):
+        r = rows[s:s + chunk]
+        G = sq[r, None] + sq[None, :] - 2.0 * (X64[r] @ X64.T)
+        G[np.arange(r.size), r] = np.inf
+        cand = np.argpartition(G, m - 1, axis=1)[:, :m]
+        diff = X64[r, None, :] - X64[cand]
+        dc = np.einsum("ijk,ijk->ij", diff, diff)
+        order = np.lexsort((cand, dc), axis=1)[:, :k]
+        idx[s:s + r.size] = np.take_along_axis(cand, order, 1)
+        d2[s:s + r.size] = np.take_along_axis(dc, order, 1)
+    return idx, d2
+
+
+def conditional_probabilities(d2, perplexity, tol=1e-5, max_iter=200):
+    """Row-wise p_{j|i} = exp(-beta_i (d2_ij - d2_i0)) / sum_j, beta_i bisected (doubling /
+    halving until bracketed, then midpoints) until the entropy (natural log) is within tol of
+    log(perplexity). d2: float64 [n, k], ascending per row. Returns (P [n, k], beta [n])."""
+    import math
+
+    d2 = np.asarray(d2, dtype=np.float64)
+    n = d2.shape[0]
+    target = math.log(perplexity)
+    dr = d2 - d2[:, :1]
+    beta = np.ones(n)
+    lo = np.full(n, -np.inf)
+    hi = np.full(n, np.inf)
+    done = np.zeros(n, bool)
+    P = None
+    for _ in range(max_iter):
+        E = np.exp(-beta[:, None] * dr)
+        S = E.sum(1)
+        P = E / S[:, None]
+        H = np.log(S) + beta * (dr * P).sum(1)
+        diff = H - target
+        done |= np.abs(diff) < tol
+        if done.all():
+            break
+        up = (diff > 0) & ~done  # entropy too high: sharpen
+        dn = (diff <= 0) & ~done
+        lo = np.where(up, beta, lo)
+        hi = np.where(dn, beta, hi)
+        beta = np.where(up, np.where(np.isinf(hi), beta * 2.0, 0.5 * (beta + hi)), beta)
+        beta = np.where(dn, np.where(np.isinf(lo), beta * 0.5, 0.5 * (beta + lo)), beta)
+    return P, beta
+
+
+def symmetrize(idx, Pc):
+    """P = (P_c + P_c^T) / (2 n) from neighbour lists, as CSR sorted by (row, col): each value
+    divided by 2n in FP64, equal (row, col) entries summed in FP64, rounded to FP32."""
+    n, k = idx.shape
+    rows = np.repeat(np.arange(n, dtype=np.int64), k)
+    cols = np.asarray(idx).reshape(-1).astype(np.int64)
+    vals = np.asarray(Pc, np.float64).reshape(-1)
+    r = np.concatenate([rows, cols])
+    c = np.concatenate([cols, rows])
+    v = np.concatenate([vals, vals]) / (2.0 * n)
+    key = r * n + c
+    order = np.argsort(key, kind="stable")
+    key, v = key[order], v[order]
+    uniq, start = np.unique(key, return_index=True)
+    vsum = np.add.reduceat(v, start)
+    ur, uc = uniq // n, uniq % n
+    row_ptr = np.zeros(n + 1, np.uint32)
+    np.cumsum(np.bincount(ur, minlength=n), out=row_ptr[1:])
+    return row_ptr, uc.astype(np.uint32), vsum.astype(np.float32)
+
+
+def joint_probabilities(X, perplexity=30.0, k=90):
+    """The whole CPU pipeline: (row_ptr, col, val) of the symmetrised P."""
+    idx, d2 = knn_exact(X, k)
+    Pc, _ = conditional_probabilities(d2, perplexity)
+    return symmetrize(idx, Pc)

@@ -1,5 +1,6 @@
-"""Host-side t-SNE logic: affinities (kNN, perplexity bisection, symmetrisation) and the theta
-schedule of the library against the oracle. No GPU needed."""
+"""Host-side t-SNE logic: the oracle's affinities (kNN, perplexity bisection, symmetrisation)
+pinned by a full sort, closed forms and a dense matrix, and the library's theta schedule against
+the oracle. No GPU needed."""
 import ctypes as C
 import math
 
@@ -14,19 +15,27 @@ from oracle import oracle as orc
 def test_knn_exact_matches_brute_force():
     rng = np.random.default_rng(3)
     X = rng.normal(size=(300, 20)).astype(np.float32)
-    idx, d2 = tsne.knn_exact(X, 12, chunk=64)
+    idx, d2 = orc.knn_exact(X, 12, chunk=64)
     X64 = X.astype(np.float64)
     D = ((X64[:, None, :] - X64[None, :, :]) ** 2).sum(-1)
     np.fill_diagonal(D, np.inf)
     ref = np.argsort(D, axis=1, kind="stable")[:, :12]
     assert np.array_equal(idx, ref)
     assert np.allclose(d2, np.take_along_axis(D, ref, 1), rtol=1e-12, atol=0)
+    # a subset of rows gives the same lists; duplicated points tie at distance 0, by index
+    sub = np.array([5, 0, 299])
+    si, sd = orc.knn_exact(X, 12, rows=sub)
+    assert np.array_equal(si, idx[sub]) and np.array_equal(sd, d2[sub])
+    Xd = np.concatenate([X, X[:3], X[:3]])
+    di, dd = orc.knn_exact(Xd, 3, rows=[0, 300])
+    assert di[0].tolist() == [300, 303, int(ref[0, 0])] and dd[0, 0] == dd[0, 1] == 0.0
+    assert di[1].tolist()[:2] == [0, 303]
 
 
 def test_perplexity_bisection_reaches_target():
     rng = np.random.default_rng(4)
     d2 = np.sort(rng.uniform(0.5, 40.0, size=(200, 90)), axis=1)
-    P, beta = tsne.conditional_probabilities(d2, 30.0)
+    P, beta = orc.conditional_probabilities(d2, 30.0)
     assert np.allclose(P.sum(1), 1.0, atol=1e-12)
     H = -(P * np.log(P)).sum(1)
     assert np.allclose(np.exp(H), 30.0, rtol=1e-4)
@@ -43,7 +52,7 @@ def test_perplexity_two_point_closed_form():
         return -(p * math.log(p) + (1 - p) * math.log(1 - p)) - target
 
     b_star = brentq(h, 1e-9, 10.0, xtol=1e-14)
-    _, beta = tsne.conditional_probabilities(np.array([[0.0, a]]), perp, tol=1e-12)
+    _, beta = orc.conditional_probabilities(np.array([[0.0, a]]), perp, tol=1e-12)
     assert abs(beta[0] - b_star) < 1e-6 * b_star
 
 
@@ -53,7 +62,7 @@ def test_symmetrize_matches_dense():
     idx = np.stack([rng.choice(np.delete(np.arange(n), i), k, replace=False) for i in range(n)])
     Pc = rng.uniform(size=(n, k))
     Pc /= Pc.sum(1, keepdims=True)
-    rp, col, val = tsne.symmetrize(idx, Pc)
+    rp, col, val = orc.symmetrize(idx, Pc)
     dense = np.zeros((n, n))
     for i in range(n):
         dense[i, idx[i]] = Pc[i]
@@ -72,7 +81,7 @@ def test_symmetrize_matches_dense():
 def test_joint_probabilities_mixture():
     X, labels = tsne.gaussian_mixture(600, dim=30, seed=7)
     assert labels.max() < 10
-    rp, col, val = tsne.joint_probabilities(X, perplexity=10.0, k=30)
+    rp, col, val = orc.joint_probabilities(X, perplexity=10.0, k=30)
     assert rp.size == 601 and rp[-1] == col.size == val.size
     assert (val > 0).all()
     assert abs(val.sum(dtype=np.float64) - 1.0) < 1e-5
