@@ -222,17 +222,17 @@ def learned_cpu_baseline(p, eps, theta=0.5):
                       f"particles of the evaluation system, FP64"}
 
 
-def batched_comparison(args, eps, theta, dt):
+def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
     """Batched systems: `args.batched_systems` independent systems with N log-uniform in
-    [1,024, 262,144] (even: uniform disks, odd: Plummer a = 0.1 truncated at R = 3), all in one
-    context (one set of
-    build launches, one traversal launch per step). Leapfrog steps timed eagerly and replayed
-    from captured HIP graphs; the traversal variant is autotuned on the batched tree."""
+    [1,024, 262,144] (even: uniform disks, odd: Plummer a = 0.1, untruncated unless rmax > 0),
+    all in one context (one set of build launches, one traversal launch per step). Leapfrog
+    steps timed eagerly and replayed from captured HIP graphs; the traversal variant is
+    autotuned on the batched tree unless given."""
     from masterthesislucas_amd import BHContext
     from masterthesislucas_amd.autotune import tune
     from masterthesislucas_amd.workloads import batched_systems
 
-    ps = batched_systems(args.batched_systems)
+    ps = batched_systems(args.batched_systems, plummer_rmax=rmax)
     sizes = [len(p.x) for p in ps]
     cat = [np.concatenate([getattr(p, f) for p in ps]) for f in ("x", "y", "m", "vx", "vy")]
     n = int(sum(sizes))
@@ -240,7 +240,11 @@ def batched_comparison(args, eps, theta, dt):
     ctx = BHContext(n, kernel="gravity", softening=eps, sizes=sizes)
     ctx.set_particles(*cat)
     ctx.build()
-    variant, times = tune(ctx, theta)
+    times = {}
+    if variant is None:
+        variant, times = tune(ctx, theta)
+    else:
+        ctx.set_variant(variant)
     ctx.set_particles(*cat)
     K = args.batched_steps
     ctx.leapfrog(2, dt, theta)
@@ -255,9 +259,9 @@ def batched_comparison(args, eps, theta, dt):
     graph = (time.perf_counter() - t0) / K
     pn, pp = ctx.counters()
     ctx.close()
+    plum = f"Plummer a=0.1 truncated at R={rmax}" if rmax > 0 else "Plummer a=0.1 untruncated"
     return {"workload": f"{len(sizes)} systems, N log-uniform in [1024, 262144] (seeded), "
-                        f"even uniform disk / odd Plummer a=0.1 truncated at R=3, eps={eps}, "
-                        f"theta={theta}",
+                        f"even uniform disk / odd {plum}, eps={eps}, theta={theta}",
             "systems": len(sizes), "particles": n, "n_min": int(min(sizes)),
             "n_max": int(max(sizes)), "steps": K, "traverse_variant": variant,
             "autotune_ms": {str(k): round(v, 4) for k, v in times.items()},
@@ -736,6 +740,8 @@ def main():
     ctx.close()
     if not args.no_batched:
         result["batched"] = batched_comparison(args, eps, theta, dt)
+        result["batched"]["truncated_r3"] = batched_comparison(
+            args, eps, theta, dt, rmax=3.0, variant=result["batched"]["traverse_variant"])
     if not args.no_learned:
         result["learned"] = learned_comparison(args, eps)
     if not args.no_tsne:

@@ -98,12 +98,14 @@ def batched_sizes(systems: int = 256, nmin: int = 1024, nmax: int = 262144,
     return np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), systems))).astype(np.int64)
 
 
-def batched_systems(systems: int = 256, seed: int = SEED_BATCHED, plummer_rmax: float = 3.0):
-    """Independent systems: even-numbered ones uniform disks, odd-numbered Plummer (a = 0.1)
-    truncated at R = plummer_rmax (30 a). Untruncated, a few far outliers per system set the
-    QuadTree root square; the cluster then sits under a chain of one-child cells, which the
-    reference rule (a cell with at most one child is summed particle by particle over all its
-    occupants, nBodySolverBH.h:68) turns into an all-pairs sum over the cluster."""
+def batched_systems(systems: int = 256, seed: int = SEED_BATCHED, plummer_rmax: float = 0.0):
+    """Independent systems: even-numbered ones uniform disks, odd-numbered Plummer (a = 0.1),
+    untruncated by default (plummer_rmax = 0) or truncated at R = plummer_rmax. Untruncated, a
+    few far outliers per system set the QuadTree root square; the cluster then sits under a
+    chain of one-child cells, which the reference rule (a cell with at most one child is summed
+    particle by particle over all its occupants, nBodySolverBH.h:68) turns into large
+    particle-particle sums: for the 256-system workload about 2,960 particle-particle
+    interactions per particle against 136 truncated at R = 3 (30 a)."""
     sizes = batched_sizes(systems, seed=seed)
     out = []
     for k, nk in enumerate(sizes):
