@@ -27,13 +27,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ static inline uint32_t div_up_a(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-constexpr int kAffRows = 64;                  // query rows per workgroup (4 waves x 16)
 constexpr int kAffCols = 64;                  // candidate columns per tile
 constexpr int kAffKC = 128;                   // K halfs per staged chunk
 constexpr int kAffPad = 8;                    // LDS row padding (conflict-free 16-byte reads)
 constexpr uint32_t kAffCap = 256;             // candidate slots per row
 constexpr uint32_t kAffKeep = kAffCap - kAffCols;  // compact above this many (a tile adds <= 64)
-constexpr int kAffMaxDimPad = 1024;           // the query block stays in LDS: dim <= 1024
+constexpr int kAffMaxDimPad = 1024;           // dim <= 1024 (8 chunks of operands in registers)
 
 // ---------------------------------------------------------------------------------------------
 // scaled FP16 copy, squared norms and rounding bounds (one wave per point)
@@ -126,33 +125,45 @@ __device__ __forceinline__ void aff_compact(uint32_t* __restrict__ cj, float* __
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_aff_candidates(const _Float16* __restrict__ Xh,
-                                                           const float* __restrict__ nh,
-                                                           const float* __restrict__ ah,
-                                                           const float* __restrict__ eb,
-                                                           uint32_t n, uint32_t dp, uint32_t k,
-                                                           uint32_t* __restrict__ cj,
-                                                           float* __restrict__ clo,
-                                                           float* __restrict__ chi,
-                                                           uint32_t* __restrict__ ccount,
-                                                           uint32_t* __restrict__ overflow) {
-    __shared__ __attribute__((aligned(16))) _Float16 As[kAffRows * (kAffMaxDimPad + kAffPad)];
-    __shared__ __attribute__((aligned(16))) _Float16 Bs[kAffCols * (kAffKC + kAffPad)];
-    __shared__ uint32_t s_cnt[kAffRows], s_over[kAffRows];
-    __shared__ float s_T[kAffRows], s_nh[kAffRows], s_ah[kAffRows], s_eb[kAffRows];
-    const uint32_t lda = dp + kAffPad, ldb = kAffKC + kAffPad;
+// Workgroup: kAffWaves waves, 16 query rows each, whose FP16 operands for the whole K stay in
+// registers (NCH chunks of 128 halfs: 16 NCH VGPRs); the columns stream through LDS in tiles of
+// 64 x 128 halfs, double-buffered: the next chunk's global loads are in flight while the
+// current one feeds the MFMAs.
+constexpr int kAffWaves = 8;
+constexpr int kAffThreads = kAffWaves * kWave;
+constexpr int kAffRowsWG = 16 * kAffWaves;
+constexpr int kAffChunkLoads = kAffCols * (kAffKC / 8) / kAffThreads;  // 16-byte loads per thread
+static_assert(kAffChunkLoads * kAffThreads == kAffCols * (kAffKC / 8), "chunk loads");
+
+template <int NCH>
+__global__ __launch_bounds__(kAffThreads) void k_aff_candidates(const _Float16* __restrict__ Xh,
+                                                                const float* __restrict__ nh,
+                                                                const float* __restrict__ ah,
+                                                                const float* __restrict__ eb,
+                                                                uint32_t n, uint32_t k,
+                                                                uint32_t* __restrict__ cj,
+                                                                float* __restrict__ clo,
+                                                                float* __restrict__ chi,
+                                                                uint32_t* __restrict__ ccount,
+                                                                uint32_t* __restrict__ overflow) {
+    constexpr uint32_t dp = NCH * kAffKC;
+    constexpr uint32_t ldb = kAffKC + kAffPad;
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kAffCols * ldb];
+    __shared__ uint32_t s_cnt[kAffRowsWG], s_over[kAffRowsWG];
+    __shared__ float s_T[kAffRowsWG], s_nh[kAffRowsWG], s_ah[kAffRowsWG], s_eb[kAffRowsWG];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int r16 = lane & 15, q4 = lane >> 4;
-    const uint32_t row0 = blockIdx.x * kAffRows;
-    const uint32_t d8 = dp / 8;
-    for (uint32_t e = tid; e < kAffRows * d8; e += kBlock) {
-        const uint32_t r = e / d8, c8 = e % d8;
-        half8 v = {};
-        if (row0 + r < n) v = *(const half8*)(Xh + (size_t)(row0 + r) * dp + c8 * 8);
-        *(half8*)(As + r * lda + c8 * 8) = v;
+    const uint32_t row0 = blockIdx.x * kAffRowsWG;
+    // this lane's A operands: row 16 w + r16, k = 32 s + 8 q4 .. + 7 for every step s
+    half8 A[NCH * (kAffKC / 32)];
+    {
+        const uint32_t ar = row0 + 16 * w + r16;
+#pragma unroll
+        for (int s = 0; s < NCH * (kAffKC / 32); ++s)
+            A[s] = ar < n ? *(const half8*)(Xh + (size_t)ar * dp + 32 * s + 8 * q4) : half8{};
     }
-    if (tid < kAffRows) {
+    if (tid < kAffRowsWG) {
         const uint32_t i = row0 + tid;
         s_cnt[tid] = 0u;
         s_over[tid] = 0u;
@@ -161,38 +172,57 @@ __global__ __launch_bounds__(kBlock) void k_aff_candidates(const _Float16* __res
         s_ah[tid] = i < n ? ah[i] : 0.f;
         s_eb[tid] = i < n ? eb[i] : 0.f;
     }
-    __syncthreads();
     const float kdelta = 0x1p-24f * (float)(2 * dp + 4);
-    const uint32_t ntiles = div_up_a(n, kAffCols), nch = dp / kAffKC;
+    const uint32_t ntiles = div_up_a(n, kAffCols);
+    const uint32_t nchunks = ntiles * NCH;  // (tile, chunk) in order
+    // chunk c = (tile c / NCH, K chunk c % NCH): 64 columns x 16 loads of 16 bytes
+    half8 pre[kAffChunkLoads];
+    auto fetch = [&](uint32_t c) {
+        const uint32_t c0 = (c / NCH) * kAffCols, ch = c % NCH;
+#pragma unroll
+        for (int u = 0; u < kAffChunkLoads; ++u) {
+            const int e = tid + u * kAffThreads;
+            const int col = e >> 4, q = e & 15;
+            pre[u] = c0 + col < n
+                         ? *(const half8*)(Xh + (size_t)(c0 + col) * dp + ch * kAffKC + q * 8)
+                         : half8{};
+        }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < kAffChunkLoads; ++u) {
+            const int e = tid + u * kAffThreads;
+            *(half8*)(&Bs[buf][(e >> 4) * ldb + (e & 15) * 8]) = pre[u];
+        }
+    };
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    f32x4 acc[4];
     for (uint32_t t = 0; t < ntiles; ++t) {
-        const uint32_t c0 = t * kAffCols;
-        f32x4 acc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (uint32_t ch = 0; ch < nch; ++ch) {
-            __syncthreads();  // the previous chunk's reads are done
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {  // 64 columns x 16 half8 = 1024 loads of 16 bytes
-                const int e = tid + u * kBlock;
-                const int c = e >> 4, q = e & 15;
-                half8 v = {};
-                if (c0 + c < n)
-                    v = *(const half8*)(Xh + (size_t)(c0 + c) * dp + ch * kAffKC + q * 8);
-                *(half8*)(Bs + c * ldb + q * 8) = v;
-            }
-            __syncthreads();
+        for (int ch = 0; ch < NCH; ++ch) {
+            const uint32_t c = t * NCH + ch;
+            const int buf = c & 1;
+            if (c + 1 < nchunks) fetch(c + 1);  // in flight during the MFMAs below
 #pragma unroll
             for (int s = 0; s < kAffKC / 32; ++s) {
-                const half8 a =
-                    *(const half8*)(As + (16 * w + r16) * lda + ch * kAffKC + s * 32 + 8 * q4);
+                const half8 a = A[ch * (kAffKC / 32) + s];
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb) {
-                    const half8 b = *(const half8*)(Bs + (16 * cb + r16) * ldb + s * 32 + 8 * q4);
+                    const half8 b =
+                        *(const half8*)(&Bs[buf][(16 * cb + r16) * ldb + s * 32 + 8 * q4]);
                     acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[cb], 0, 0, 0);
                 }
             }
+            // the other buffer was last read in the previous chunk, before its barrier
+            if (c + 1 < nchunks) stage(buf ^ 1);
+            __syncthreads();
         }
-        // acc[cb][e] = x^_i . x^_j for i = row0 + 16 w + 4 q4 + e, j = c0 + 16 cb + r16
+        // tile done: acc[cb][e] = x^_i . x^_j, i = row0 + 16 w + 4 q4 + e, j = c0 + 16 cb + r16
+        const uint32_t c0 = t * kAffCols;
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
             const uint32_t j = c0 + 16 * cb + r16;
@@ -221,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_aff_candidates(const _Float16* __res
         }
         __syncthreads();
         // rows whose buffer could overflow with the next tile are compacted, a wave per row
-        for (int r = w; r < kAffRows; r += kWavesPerBlock) {
+        for (int r = w; r < kAffRowsWG; r += kAffWaves) {
             const uint32_t i = row0 + r;
             if (i < n && s_cnt[r] > kAffKeep)
                 aff_compact(cj + (size_t)i * kAffCap, clo + (size_t)i * kAffCap,
@@ -229,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_aff_candidates(const _Float16* __res
         }
         __syncthreads();
     }
-    for (int r = w; r < kAffRows; r += kWavesPerBlock) {
+    for (int r = w; r < kAffRowsWG; r += kAffWaves) {
         const uint32_t i = row0 + r;
         if (i >= n) continue;
         aff_compact(cj + (size_t)i * kAffCap, clo + (size_t)i * kAffCap, chi + (size_t)i * kAffCap,
@@ -416,8 +446,18 @@ void launch_aff_pack(hipStream_t s, const float* X, uint32_t n, uint32_t dim, fl
 void launch_aff_candidates(hipStream_t s, const _Float16* Xh, const float* nh, const float* ah,
                            const float* eb, uint32_t n, uint32_t dim, uint32_t k, uint32_t* cj,
                            float* clo, float* chi, uint32_t* ccount, uint32_t* overflow) {
-    k_aff_candidates<<<div_up_a(n, kAffRows), kBlock, 0, s>>>(
-        Xh, nh, ah, eb, n, aff_dim_padded(dim), k, cj, clo, chi, ccount, overflow);
+    const uint32_t g = div_up_a(n, kAffRowsWG);
+    switch (aff_dim_padded(dim) / kAffKC) {
+#define BH_AFFC(C)                                                                                \
+    case C:                                                                                       \
+        k_aff_candidates<C><<<g, kAffThreads, 0, s>>>(Xh, nh, ah, eb, n, k, cj, clo, chi, ccount,  \
+                                                      overflow);                                  \
+        break
+        BH_AFFC(1); BH_AFFC(2); BH_AFFC(3); BH_AFFC(4);
+        BH_AFFC(5); BH_AFFC(6); BH_AFFC(7); BH_AFFC(8);
+#undef BH_AFFC
+        default: break;  // dim > 1024 is rejected by the caller
+    }
 }
 void launch_aff_refine(hipStream_t s, const float* X, uint32_t n, uint32_t dim, uint32_t k,
                        const uint32_t* cj, const uint32_t* ccount, uint32_t* idx, double* d2) {

@@ -1230,14 +1230,18 @@ bh_status bh_tsne_affinities(bh_ctx* c, const float* X, uint32_t n, uint32_t dim
     if (m64 >= (1ull << 31) || (uint64_t)n * n >= (1ull << 62))
         return fail(c, BH_ERR_ARG, "%u points x %u neighbours is too large", n, k);
     const uint32_t m = (uint32_t)m64;
-    // scale by a power of two so that the largest |x| lands in [2^13, 2^14): no FP16 overflow
-    double maxabs = 0.0;
-    for (uint64_t e = 0; e < (uint64_t)n * dim; ++e) {
-        const double v = std::fabs((double)X[e]);
-        if (!(v <= 3.4e38)) return fail(c, BH_ERR_ARG, "input coordinate %llu is not finite",
-                                        (unsigned long long)e);
-        maxabs = std::max(maxabs, v);
+    // scale by a power of two so that the largest |x| lands in [2^13, 2^14): no FP16 overflow.
+    // The magnitude bits of a float order its |x|; infinities and NaN have all exponent bits set
+    uint32_t maxbits = 0;
+    {
+        const uint32_t* xb = reinterpret_cast<const uint32_t*>(X);
+        const uint64_t cnt = (uint64_t)n * dim;
+        for (uint64_t e = 0; e < cnt; ++e) maxbits = std::max(maxbits, xb[e] & 0x7FFFFFFFu);
     }
+    if (maxbits >= 0x7F800000u) return fail(c, BH_ERR_ARG, "an input coordinate is not finite");
+    float maxf;
+    std::memcpy(&maxf, &maxbits, 4);
+    const double maxabs = maxf;
     int ex = 0;
     if (maxabs > 0.0) (void)std::frexp(maxabs, &ex);
     const float scale = maxabs > 0.0 ? std::ldexp(1.0f, 14 - ex) : 1.0f;
