@@ -275,7 +275,8 @@ bh_status bh_forces_learned(bh_ctx* ctx, const bh_mlp* mlp, double theta, double
                             float* ay, double* total);
 /* the current tree's cell features (features[n_nodes x 10]) and, with mlp, the walk's
  * coefficient records (coef[n_nodes x 16]: for c_j = a_j + i b_j, a1+a2, b2-b1, a1-a2, b1+b2,
- * a3, b3, a4, b4, a5+a6, b6-b5, a5-a6, b5+b6, a7, b7, 0, 0); n_nodes must equal the node count
+ * a3, b3, a4, b4, a5+a6, b6-b5, a5-a6, b5+b6, a7, b7, then the cell's M l^2 and side l as the
+ * walk uses them); n_nodes must equal the node count
  * of the tree (bh_tree_info). Either output may be NULL. */
 bh_status bh_learned_export(bh_ctx* ctx, const bh_mlp* mlp, float* features, float* coef,
                             uint32_t n_nodes);

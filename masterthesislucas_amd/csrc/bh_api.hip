@@ -2000,7 +2000,7 @@ bh_status ensure_features(bh_ctx* c) {
 bh_status ensure_coefs(bh_ctx* c, const bh_mlp* m) {
     bh_status st = ensure_features(c);
     if (st != BH_OK) return st;
-    launch_cell_coefs(c->stream, c->lfeat, c->prm, c->cap_nodes, m->wbf, c->lcoef);
+    launch_cell_coefs(c->stream, c->lfeat, c->nodes, c->prm, c->cap_nodes, m->wbf, c->lcoef);
     return launch_status(c, "cell coefficients");
 }
 
@@ -2093,7 +2093,7 @@ bh_status bh_forces_learned(bh_ctx* c, const bh_mlp* m, double theta, double ban
     const float inv_b2 = band > 0.0 ? (float)(1.0 / (band * band)) : 0.f;
     const float eps = c->cfg.softening;
     phase_begin(c, 5);  // the per-cell inference is part of the learned force evaluation
-    launch_cell_coefs(s, c->lfeat, c->prm, c->cap_nodes, m->wbf, c->lcoef);
+    launch_cell_coefs(s, c->lfeat, c->nodes, c->prm, c->cap_nodes, m->wbf, c->lcoef);
     c->trav_blocks = launch_traverse_learned(s, c->variant, c->pos4, c->nodes, c->lcoef, c->prm,
                                              c->n, inv_t2, inv_b2, eps * eps, c->cfg.G, m->terms,
                                              c->ax, c->ay, total ? c->tot : nullptr, c->counters);

@@ -217,8 +217,9 @@ void launch_cell_features(hipStream_t s, const float4* pos4, const Node* nodes, 
                           uint32_t cap_nodes, float* feat, uint32_t* lists, uint32_t n);
 uint32_t cell_list_cap(uint32_t n, uint32_t c);
 size_t cell_features_lists_words(uint32_t n);
-void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
-                       uint32_t cap_nodes, const MlpBf16* w, float4* coef);
+void launch_cell_coefs(hipStream_t s, const float* feat, const Node* nodes,
+                       const BuildParams* prm, uint32_t cap_nodes, const MlpBf16* w,
+                       float4* coef);
 
 // t-SNE input affinities (bh_affinity.hip): FP16 copy padded to aff_dim_padded(dim) halfs per
 // point, candidate buffers of aff_cap() slots per point, exact k nearest by FP64 distance,

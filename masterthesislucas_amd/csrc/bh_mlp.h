@@ -155,10 +155,10 @@ __device__ __forceinline__ void mlp_wave_infer(const MlpBf16* __restrict__ W,
 }
 
 // The walk's coefficient record (kCoefStride floats) from the 14 network outputs y (Re, Im of
-// c_1..c_7): [A1, B1, C1, D1, a3, b3, a4, b4, A5, B5, C5, D5, a7, b7, 0, 0] with
-// A1 = a1 + a2, B1 = b2 - b1, C1 = a1 - a2, D1 = b1 + b2 (c_1 w + c_2 conj w) and A5..D5 the
-// same for c_5 w^2 + c_6 conj(w)^2.
-__host__ __device__ inline void learned_record(const float* y, float* r) {
+// c_1..c_7) and the cell's M l^2 and side l: [A1, B1, C1, D1, a3, b3, a4, b4, A5, B5, C5, D5,
+// a7, b7, M l^2, l] with A1 = a1 + a2, B1 = b2 - b1, C1 = a1 - a2, D1 = b1 + b2
+// (c_1 w + c_2 conj w) and A5..D5 the same for c_5 w^2 + c_6 conj(w)^2.
+__host__ __device__ inline void learned_record(const float* y, float ml2, float l, float* r) {
     r[0] = y[0] + y[2];
     r[1] = y[3] - y[1];
     r[2] = y[0] - y[2];
@@ -173,17 +173,18 @@ __host__ __device__ inline void learned_record(const float* y, float* r) {
     r[11] = y[9] + y[11];
     r[12] = y[12];
     r[13] = y[13];
-    r[14] = 0.f;
-    r[15] = 0.f;
+    r[14] = ml2;
+    r[15] = l;
 }
 
-// Correction (before G) of one accepted pair: d = (dx, dy) = c - p, d2 = |d|^2, l the cell
-// side, ml2 = M l^2, rec the cell's coefficient record (wave-uniform). TERMS = 7: the whole
+// Correction (before G) of one accepted pair: d = (dx, dy) = c - p, d2 = |d|^2, rec the cell's
+// coefficient record (wave-uniform; M l^2 and l in its last two words). TERMS = 7: the whole
 // basis; TERMS = 3: its quadrupole part c_1 w + c_2 conj w + c_3 w^3 only (a network trained
 // with three terms).
 template <int TERMS>
-__device__ __forceinline__ void learned_correction(float dx, float dy, float d2, float l, float ml2,
-                                                   const float* rec, float& cx, float& cy) {
+__device__ __forceinline__ void learned_correction(float dx, float dy, float d2, const float* rec,
+                                                   float& cx, float& cy) {
+    const float ml2 = rec[14], l = rec[15];
     static_assert(TERMS == 3 || TERMS == 7, "3 or 7 basis terms");
     const float inv_r = __builtin_amdgcn_rsqf(d2);
     const float ux = dx * inv_r, uy = dy * inv_r;

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer_bf16(const float* __restri
 
 // per node: the network on its features -> the walk's coefficient record
 __global__ __launch_bounds__(kBlock) void k_cell_coefs(const float* __restrict__ feat,
+                                                       const Node* __restrict__ nodes,
                                                        const BuildParams* __restrict__ prm,
                                                        uint32_t cap_nodes,
                                                        const MlpBf16* __restrict__ W,
@@ -127,8 +128,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_coefs(const float* __restrict__
     infer_rows(&ws, feat, base, nn, kMlpIn, xs[w], ys[w]);
     const uint32_t k = base + lane;
     if (k >= nn) return;
+    // the cell side as the walk takes it (the FP32 root side scaled by 2^-level, exact) and
+    // M l^2 with the walk's rounding
+    const Node nd = nodes[k];
+    const float l = ldexpf(prm->side, -(int)(nd.meta & kMetaLevelMask));
     float r[kCoefStride];
-    learned_record(&ys[w][lane * 16], r);
+    learned_record(&ys[w][lane * 16], nd.mass * l * l, l, r);
     float4* dst = coef + (size_t)k * (kCoefStride / 4);
 #pragma unroll
     for (int q = 0; q < kCoefStride / 4; ++q)
@@ -697,10 +702,11 @@ uint32_t cell_list_cap(uint32_t n, uint32_t c) { return (n / c + 1) * (kMaxLevel
 size_t cell_features_lists_words(uint32_t n) {
     return 2 + (size_t)cell_list_cap(n, kFeatThread) + cell_list_cap(n, kFeatWave);
 }
-void launch_cell_coefs(hipStream_t s, const float* feat, const BuildParams* prm,
-                       uint32_t cap_nodes, const MlpBf16* w, float4* coef) {
-    k_cell_coefs<<<div_up_u(cap_nodes, 64 * kWavesPerBlock), kBlock, 0, s>>>(feat, prm, cap_nodes,
-                                                                            w, coef);
+void launch_cell_coefs(hipStream_t s, const float* feat, const Node* nodes,
+                       const BuildParams* prm, uint32_t cap_nodes, const MlpBf16* w,
+                       float4* coef) {
+    k_cell_coefs<<<div_up_u(cap_nodes, 64 * kWavesPerBlock), kBlock, 0, s>>>(feat, nodes, prm,
+                                                                            cap_nodes, w, coef);
 }
 
 }  // namespace bh

@@ -194,8 +194,6 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     }
     const uint32_t nn = prm->n_nodes;
     if constexpr (!BATCHED) end0 = nn;
-    // the root side as the node records take it (RN32(S)), for the learned correction's l
-    const float lside = LEARN ? prm->side : 0.f;
     const bool use_band = LEARN && la.inv_band2 > 0.f;  // else every accepted cell is corrected
 
     // Compensated totals (ax, ay, at) and their Kahan terms; monopoles of consecutive visits are
@@ -266,10 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                 const float rec[kCoefStride] = {
                                     c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w,
                                     c2.x, c2.y, c2.z, c2.w, c3.x, c3.y, c3.z, c3.w};
-                                const float l = ldexpf(lside, -(int)(cur.meta & kMetaLevelMask));
                                 float gx, gy;
-                                learned_correction<LTERMS>(dx, dy, d2, l, cur.mass * l * l, rec,
-                                                           gx, gy);
+                                learned_correction<LTERMS>(dx, dy, d2, rec, gx, gy);
                                 bx[s] += gx;
                                 by[s] += gy;
                             }

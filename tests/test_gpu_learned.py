@@ -206,15 +206,22 @@ def test_cell_coefficients_match_network(gpu):
     prm = _params(33)
     net.set_params(flatten(prm))
     f, rec = ctx.learned_export(net)
-    ref = _records(O.mlp_forward(f.astype(np.float64), _f64(prm)))
+    ref = _records(O.mlp_forward(f.astype(np.float64), _f64(prm)))[:, :14]
     scale = np.abs(ref).max()
+    # the record's last two words: the cell's M l^2 and side l as the walk takes them (the FP32
+    # root side scaled by 2^-level; M l^2 rounded as (M l) l)
+    g = ctx.tree()
+    _, side = ctx.bounds()
+    l = np.ldexp(np.float32(side), -g["level"]).astype(np.float32)
+    assert np.array_equal(rec[:, 15], l)
+    assert np.array_equal(rec[:, 14], (g["mass"] * l).astype(np.float32) * l)
+    rec = rec[:, :14]
     err = np.abs(rec - ref)
-    assert np.all(rec[:, 14:] == 0)
     # BF16 MFMA inference against the FP64 forward of the same features
     assert np.median(err) < 1e-2 * scale and err.max() < 6e-2 * scale, (np.median(err), err.max())
     # and the same network evaluated through the FP32 kernels on the exported features agrees
     y32 = net.forward(f)
-    assert np.max(np.abs(_records(y32) - ref)) < 1e-4 * scale
+    assert np.max(np.abs(_records(y32)[:, :14] - ref)) < 1e-4 * scale
     net.close()
     ctx.close()
 
@@ -306,7 +313,8 @@ def test_learned_walk_matches_oracle(gpu, variant, band, terms):
     prm["b3"][:] = np.random.default_rng(36).uniform(-1, 1, N_OUT).astype(np.float32)
     net.set_params(flatten(prm))
     _, rec = ctx.learned_export(net)
-    assert np.array_equal(rec, np.broadcast_to(_records(prm["b3"][None, :])[0], rec.shape))
+    assert np.array_equal(rec[:, :14],
+                          np.broadcast_to(_records(prm["b3"][None, :])[0, :14], (rec.shape[0], 14)))
     ax, ay, _ = ctx.forces_learned(net, theta, band=band)
     pn, pp = ctx.counters()
     c = prm["b3"].astype(np.float64)
