@@ -259,6 +259,11 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
     graph = (time.perf_counter() - t0) / K
     pn, pp = ctx.counters()
     ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline and rmax == 0.0:
+        cpu = batched_cpu_baseline(ps, eps, theta)
+        log(f"[bench] batched CPU baseline: {cpu['seconds']:.2f} s for {cpu['particles']} "
+            f"particles")
     plum = f"Plummer a=0.1 truncated at R={rmax}" if rmax > 0 else "Plummer a=0.1 untruncated"
     return {"workload": f"{len(sizes)} systems, N log-uniform in [1024, 262144] (seeded), "
                         f"even uniform disk / odd {plum}, eps={eps}, theta={theta}",
@@ -266,7 +271,34 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
             "n_max": int(max(sizes)), "steps": K, "traverse_variant": variant,
             "autotune_ms": {str(k): round(v, 4) for k, v in times.items()},
             "eager_ms_per_step": eager * 1e3, "graph_ms_per_step": graph * 1e3,
-            "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n}
+            "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n,
+            "cpu_baseline": cpu}
+
+
+def batched_cpu_baseline(ps, eps, theta, budget=1_000_000):
+    """Plain Barnes-Hut on this host for the batched workload, bounded: the first systems in
+    the seeded order until `budget` particles, each with the oracle's C restatement of the
+    reference QuadTree and walk (FP64, tree build serial, walk threaded over all host cores up
+    to 16); particle-steps/s over the sample."""
+    from oracle import oracle as O
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    tot, secs, used = 0, 0.0, 0
+    for p in ps:
+        if tot >= budget:
+            break
+        x, y = p.x.astype(np.float64), p.y.astype(np.float64)
+        t0 = time.perf_counter()
+        qt = O.QuadTree(x, y, 16)
+        qt.forces(O.KERNEL_GRAVITY, G=1.0, m=1.0 / p.n, eps2=eps * eps, theta=theta,
+                  nthreads=cores)
+        secs += time.perf_counter() - t0
+        tot += p.n
+        used += 1
+    return {"value": tot / secs, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "seconds": secs, "particles": tot, "systems": used,
+            "sample": f"one plain step (QuadTree build + walk, theta={theta}) of the first {used} "
+                      f"systems ({tot} particles), FP64"}
 
 
 def tsne_run(args):
