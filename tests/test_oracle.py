@@ -716,6 +716,21 @@ def test_cell_features_closed_forms():
                         mom)[0]
     assert np.all(f[:7] == 0.0) and f[9] == 0.0
     assert f[7] == pytest.approx((1.75 - 1.5) / 1.0) and f[8] == pytest.approx((0.25 - 0.5) / 1.0)
+    # third moments: masses 1 at (0, 0) and 3 at (1, 1/2) about the COM (3/4, 3/8):
+    # offsets u = (-3/4, -3/8) and (1/4, 1/8); sum m u^3 = -27/64 + 3/64 = -3/8, and each mixed
+    # moment is the same sum scaled by (1/2) per y factor
+    xs, ys, ms = np.array([0.0, 1.0]), np.array([0.0, 0.5]), np.array([1.0, 3.0])
+    mom = dict(cx=np.array([0.75]), cy=np.array([0.375]), mass=np.array([4.0]))
+    f = O.cell_features(xs, ys, ms, _one_cell_tree(2), np.zeros(2, np.uint64), 32, 0.0, 0.0,
+                        2.0, mom)[0]
+    M, l = 4.0, 2.0
+    b = 1.0 / (M * l ** 3)
+    assert f[3] == pytest.approx(-3 / 8 * b, rel=1e-14)
+    assert f[4] == pytest.approx(-3 / 16 * b, rel=1e-14)
+    assert f[5] == pytest.approx(-3 / 32 * b, rel=1e-14)
+    assert f[6] == pytest.approx(-3 / 64 * b, rel=1e-14)
+    assert f[0] == pytest.approx((9 / 16 + 3 / 16) / (M * l * l), rel=1e-14)
+    assert f[1] == pytest.approx((9 / 32 + 3 / 32) / (M * l * l), rel=1e-14)
 
 
 def _cluster_label(xs, ys, ms, dist, angle, l=1.0):
