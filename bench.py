@@ -258,6 +258,7 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
     ctx.leapfrog_graph(K, dt, theta)
     graph = (time.perf_counter() - t0) / K
     pn, pp = ctx.counters()
+    prio = ctx.priority
     ctx.close()
     cpu = None
     if not args.no_cpu_baseline and rmax == 0.0:
@@ -269,6 +270,7 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
                         f"even uniform disk / odd {plum}, eps={eps}, theta={theta}",
             "systems": len(sizes), "particles": n, "n_min": int(min(sizes)),
             "n_max": int(max(sizes)), "steps": K, "traverse_variant": variant,
+            "walk_priority": prio,
             "autotune_ms": {str(k): round(v, 4) for k, v in times.items()},
             "eager_ms_per_step": eager * 1e3, "graph_ms_per_step": graph * 1e3,
             "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n,
@@ -754,6 +756,7 @@ def main():
         "interactions_per_particle": (pn + pp) / n,
         "tree": {"nodes": nodes, "max_level": max_level, "overfull_leaves": overfull},
         "traverse_variant": variant,
+        "walk_priority": ctx.priority,
         "autotune_ms": tuning,
         "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
         "roofline": {"bound": "valu", "pipe": "FP32 VALU (traversal)", "kernel": "k_traverse",

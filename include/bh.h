@@ -352,6 +352,15 @@ bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
  * Results equal split 1 up to FP32 summation order; interaction counts are identical.
  * Batched contexts accept only 1; bh_forces_learned always walks with split 1. */
 bh_status bh_set_traverse_split(bh_ctx* ctx, int split);
+/* Issue priority of the walk's waves (no reference counterpart; on by default): each wave
+ * starts at priority 3 and steps down a level every ~1/3 of a wave's node visits (a step set
+ * from theta), so that waves that have made less progress are issued first and the waves of a
+ * SIMD finish together instead of in dispatch order. Applied to one-system walks (not split,
+ * not learned) whose waves all fit on the device at once (at most 8 per SIMD); other launches
+ * walk without it. Scheduling only: results and interaction counts are bit-identical either
+ * way. */
+bh_status bh_set_walk_priority(bh_ctx* ctx, int on);
+bh_status bh_get_walk_priority(bh_ctx* ctx, int* on);
 
 /* ---- force solver (NBodySolverFMM_SYM_MORTON, nBodySolverFMM_SYM_MORTON.h:28-316, 482-536,
  * registered as "FMM_SYM_MORTON" in tsne.h:121) ----

@@ -115,6 +115,8 @@ SIGNATURES = {
     "bh_traverse_variant_count": (_i32, []),
     "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
     "bh_set_traverse_split": (_i32, [_vp, C.c_int]),
+    "bh_set_walk_priority": (_i32, [_vp, C.c_int]),
+    "bh_get_walk_priority": (_i32, [_vp, C.POINTER(C.c_int)]),
     "bh_set_solver": (_i32, [_vp, C.c_int]),
     "bh_set_sort_spin_limit": (_i32, [_vp, _u32]),
     "bh_direct_fp64_noalloc_async": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),

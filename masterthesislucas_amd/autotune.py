@@ -50,15 +50,26 @@ def time_variant(ctx, variant, theta, repeats=3):
 
 def tune(ctx, theta, repeats=3, save=False, path=TABLE):
     """Time all variants on ctx's current tree (ctx must have been built), select the fastest,
-    and return (best_variant, {variant: ms})."""
+    and return (best_variant, {variant: ms}).
+
+    The variants are timed with the walk's progress-ordered issue priority on
+    (bh_set_walk_priority); the winner is timed once more without it, under the key
+    "<variant>:no-priority", and the context keeps whichever was faster. The priority pays when
+    all waves of a launch are resident at once and can cost a few percent when they are not."""
     times = {}
+    ctx.set_priority(True)
     for v in range(ctx.variant_count()):
         times[v] = time_variant(ctx, v, theta, repeats)
     best = min(times, key=times.get)
+    ctx.set_priority(False)
+    off = time_variant(ctx, best, theta, repeats)
+    times[f"{best}:no-priority"] = off
+    ctx.set_priority(off >= times[best])
     ctx.set_variant(best)
     if save:
         table = load_table(path)
-        table[_key(ctx, theta)] = {"variant": best, "ms": times}
+        table[_key(ctx, theta)] = {"variant": best, "priority": ctx.priority,
+                                   "ms": {str(k): v for k, v in times.items()}}
         with open(path, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     return best, times

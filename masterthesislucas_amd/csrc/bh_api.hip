@@ -140,6 +140,7 @@ struct bh_ctx {
     int g_variant = -1;
 
     int variant = 4;  // traversal kernel variant (see launch_traverse)
+    bool walk_prio = true;  // waves step their issue priority down as they progress
     uint32_t split = 1;         // waves per sink group in the traversal (one-system contexts)
     int solver = BH_SOLVER_BH;  // force solver (BH_SOLVER_FMM_SYM: symmetric dual-tree FMM)
     void* fmm_buf = nullptr;    // FMM workspace (fmm_workspace_bytes: per-slot moments,
@@ -412,7 +413,7 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
                                      grav ? c->cfg.G : 1.0f, ox, oy, want_total ? ot : nullptr,
                                      c->counters,
                                      batch_of(c, variant_ns_index(c->variant)),
-                                     F);
+                                     F, c->walk_prio);
     if (F > 1) {
         launch_sum_parts(s, ox, F, c->n, c->ax);
         launch_sum_parts(s, oy, F, c->n, c->ay);
@@ -1603,6 +1604,19 @@ bh_status bh_set_solver(bh_ctx* c, int solver) {
     if (solver != c->solver) c->g_valid = false;
     c->solver = solver;
     c->have_acc = false;
+    return BH_OK;
+}
+
+bh_status bh_set_walk_priority(bh_ctx* c, int on) {
+    if (!c) return BH_ERR_ARG;
+    if ((on != 0) != c->walk_prio) c->g_valid = false;  // captured steps hold the old walk
+    c->walk_prio = on != 0;
+    return BH_OK;
+}
+
+bh_status bh_get_walk_priority(bh_ctx* c, int* on) {
+    if (!c || !on) return BH_ERR_ARG;
+    *on = c->walk_prio ? 1 : 0;
     return BH_OK;
 }
 

@@ -94,7 +94,9 @@ inline int variant_ppu(int v) {
     v = variant_shape(v);
     return v < 12 ? (v % 2 ? 4 : 1) : 0;
 }
-// returns the number of blocks launched (one counter partial each)
+// returns the number of blocks launched (one counter partial each); prio: the waves of a
+// one-system walk that fits on the device at once lower their issue priority as they progress
+// (scheduling only, results are identical either way)
 // batched systems: one work entry (first sink, count, system, -) per wave and the systems'
 // roots; work == nullptr for a one-system context
 struct TraverseBatch {
@@ -105,7 +107,8 @@ struct TraverseBatch {
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                      const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                      float eps2, float scale, float* ax, float* ay, float* tot,
-                     unsigned long long* counters, TraverseBatch batch, uint32_t F = 1);
+                     unsigned long long* counters, TraverseBatch batch, uint32_t F = 1,
+                     bool prio = true);
 // F > 1 (one-system contexts): F waves per sink group, outputs are F parts of n each
 constexpr uint32_t kMaxSplit = 8;
 void launch_sum_parts(hipStream_t s, const float* part, uint32_t F, uint32_t n, float* out);

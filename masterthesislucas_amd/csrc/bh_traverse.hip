@@ -13,6 +13,8 @@
 #include "bh_launch.h"
 #include "bh_mlp.h"
 
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 namespace bh {
@@ -138,8 +140,21 @@ struct LearnArgs {
 // particle-particle sums only of nodes it owns, and writes partial results to part f of the
 // output; every interaction is counted by exactly one wave. This multiplies the number of waves
 // when there are too few sinks to fill the GPU.
+//
+// Issue priority by progress. A SIMD's arbiter issues for the oldest ready wave first, so of
+// the waves dispatched together the early ones run ahead: measured with per-wave timers on the
+// two-disk walk (1M sinks, 7 to 8 waves per SIMD), wave durations grew linearly with the
+// dispatch index from 330 to 780 us for the same work, and for the second half of the launch
+// the SIMDs held ever fewer waves. Each wave therefore starts at priority 3 and steps down one
+// level every `prio_step` node visits (about a third of a wave's visits), so the waves that have
+// made the least progress are issued first and the waves of a SIMD finish close together: the
+// 1M-sink walk (2 sinks per lane, 8 waves per SIMD) 0.893 -> 0.826 ms. One scalar register
+// carries the level and the countdown: at 76 scalar registers instead of 72 the hardware held 7
+// waves per SIMD instead of 8, and the priority then made the walk slower, not faster.
+// Scheduling only: every result is bit-identical with or without it (prio_step = 0: off).
+constexpr uint32_t kPrioLeft = (1u << 30) - 1u;
 template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED, bool SPLIT,
-          bool SMASK, int LTERMS = 0>
+          bool SMASK, int LTERMS = 0, bool PRIO = false>
 __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ pos4,
                                                      const Node* __restrict__ nodes,
                                                      const BuildParams* __restrict__ prm,
@@ -149,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                                      float* __restrict__ tot_out,
                                                      unsigned long long* __restrict__ counters,
                                                      TraverseBatch batch, uint32_t F,
-                                                     LearnArgs la) {
+                                                     LearnArgs la, uint32_t prio_step) {
     constexpr bool LEARN = LTERMS != 0;
     static_assert(!LEARN || (KERNEL == kKernelGravity && !SMASK && !SPLIT && !BATCHED),
                   "the learned walk is the vector-mask one-system gravity walk");
@@ -212,6 +227,10 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     uint32_t k = k0, end_cur = end0;
     int sp = 0, since_fold = 0;
     Node cur = nodes[min(k0, nn - 1)];
+    // priority level (top 2 bits) and the visits left at it (one scalar register: every
+    // further register of walk state cost the 1M-sink walk a wave per SIMD)
+    uint32_t prio = (3u << 30) | min(max(prio_step, 1u), kPrioLeft);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     while (any) {
         while (k == end_cur && sp > 0) {
             --sp;
@@ -401,6 +420,15 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             }
             since_fold = 0;
         }
+        if constexpr (PRIO) {
+            if ((--prio & kPrioLeft) == 0u) {  // s_setprio takes an immediate: a branch per level
+                const uint32_t lvl = prio >> 30;
+                if (lvl == 3) __builtin_amdgcn_s_setprio(2);
+                else if (lvl == 2) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+                prio = ((lvl ? lvl - 1u : 0u) << 30) | min(max(prio_step, 1u), kPrioLeft);
+            }
+        }
         k = down ? k + 1 : next;
         // select the next record word by word on the integer bit patterns: a uniform select
         // of uniform values, kept in scalar registers
@@ -417,7 +445,6 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         cur.count = down ? nd_down.count : nd_skip.count;
         cur.meta = down ? nd_down.meta : nd_skip.meta;
     }
-    unsigned long long s_pn = 0, s_pp = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         kahan_add(ax[s], cax[s], bx[s]);
@@ -429,11 +456,15 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             ay_out[o] = scale * (ay[s] - cay[s]);
             if (TOT) tot_out[o] = scale * (at[s] - cat[s]);
         }
-        s_pn += valid[s] ? npn[s] : 0u;
-        s_pp += valid[s] ? npp[s] : 0u;
     }
     if (counters) {  // one partial per block (no same-address device atomics)
         __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
+        unsigned long long s_pn = 0, s_pp = 0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            s_pn += valid[s] ? npn[s] : 0u;
+            s_pp += valid[s] ? npp[s] : 0u;
+        }
         s_pn = wave_sum_u64(s_pn) + cnt_pn;
         s_pp = wave_sum_u64(s_pp);
         if (lane == 0) {
@@ -569,17 +600,53 @@ __global__ __launch_bounds__(kBlock) void k_kick(float* __restrict__ vx, float* 
 // ------------------------------------------------------------------------------------------
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Node visits per priority level of the walk. The best step measured on the two-disk walk
+// (1M sinks) was about a third of a wave's mean visits: 512-768 at theta 0.3, 256 at 0.5,
+// 128-192 at 0.7 and 96-128 at 1.0, while the visits per wave scale as about theta^-1.45.
+static uint32_t walk_prio_step(float inv_theta2) {
+    const double s = 261.0 * std::pow(0.25 * (double)inv_theta2, 0.725);
+    return (uint32_t)std::min(65536.0, std::max(16.0, std::round(s)));
+}
+
+// Wave slots of the device at the largest occupancy (8 waves per SIMD, 4 SIMDs per CU). The
+// priority steps only pay when every wave of a launch is resident from the start: with several
+// rounds of waves, each round then finishes together and the next one starts on an idle SIMD
+// (measured: the 4M-sink walk, 4 rounds, 9% slower with them; the 1M-sink walk with one sink
+// per lane, 2 rounds, 4% slower), so launches with more waves than this walk without them.
+static uint64_t device_wave_slots() {
+    static const uint64_t slots = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 0;
+        return (uint64_t)cus * 4u * 8u;
+    }();
+    return slots;
+}
+
+static uint32_t launch_prio_step(bool prio, uint64_t waves, float inv_theta2) {
+    return prio && waves <= device_wave_slots() ? walk_prio_step(inv_theta2) : 0u;
+}
+
 // variant v: sinks per lane {1, 2, 4}[v / 4]; monopoles branchy (v % 4 < 2) or masked;
 // particle-particle fetch of 1 (v even) or 4 (v odd) sources
 template <int KERNEL, bool TOT, bool BATCHED>
 static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, const float4* pos4,
                             const Node* nodes, const BuildParams* prm, float inv_theta2,
                             float eps2, float scale, float* ax, float* ay, float* tot,
-                            unsigned long long* counters, TraverseBatch batch) {
+                            unsigned long long* counters, TraverseBatch batch, uint32_t pstep) {
 #define BH_TRAV(NS, BR, U) BH_TRAV_M(NS, BR, U, false)
 #define BH_TRAV_M(NS, BR, U, M)                                                                   \
-    k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false, M><<<g, kBlock, 0, s>>>(                   \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch, 1u, LearnArgs{})
+    do {                                                                                          \
+        if (!BATCHED && pstep)                                                                    \
+            k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false, M, 0, !BATCHED>                    \
+                <<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot,  \
+                                      counters, batch, 1u, LearnArgs{}, pstep);                   \
+        else                                                                                      \
+            k_traverse<KERNEL, TOT, NS, BR, U, BATCHED, false, M>                                 \
+                <<<g, kBlock, 0, s>>>(pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot,  \
+                                      counters, batch, 1u, LearnArgs{}, 0u);                      \
+    } while (0)
     if (variant >= kTraverseVariantsVector) {
         // the same 18 shapes with scalar lane masks
         const int v = variant - kTraverseVariantsVector;
@@ -630,7 +697,8 @@ static uint32_t traverse_split(hipStream_t s, int variant, uint32_t F, uint32_t 
     TraverseBatch none;
 #define BH_TRAVS(NS, BR, U)                                                                       \
     k_traverse<KERNEL, TOT, NS, BR, U, false, true, false><<<g, kBlock, 0, s>>>(                  \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F, LearnArgs{})
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F, LearnArgs{}, \
+        0u)
 #define BH_TRAVS_U(NS, BR)                                                                        \
     if (lane_pp) BH_TRAVS(NS, BR, 0); else BH_TRAVS(NS, BR, 1)
     const bool br = variant_branchy(variant);
@@ -651,39 +719,44 @@ template <int KERNEL, bool TOT>
 static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const float4* pos4,
                                  const Node* nodes, const BuildParams* prm, float inv_theta2,
                                  float eps2, float scale, float* ax, float* ay, float* tot,
-                                 unsigned long long* counters, TraverseBatch batch, uint32_t F) {
-    if (F > 1 && !batch.work)
+                                 unsigned long long* counters, TraverseBatch batch, uint32_t F,
+                                 bool prio) {
+    const int ns = variant_ns(variant);
+    if (F > 1 && !batch.work)  // split walks: no priority steps
         return traverse_split<KERNEL, TOT>(s, variant, F, n, pos4, nodes, prm, inv_theta2, eps2,
                                            scale, ax, ay, tot, counters);
-    const int ns = variant_ns(variant);
     const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
                                   : div_up(n, (uint64_t)kBlock * ns);
+    const uint32_t pstep = launch_prio_step(prio, (uint64_t)g * kWavesPerBlock, inv_theta2);
     if (batch.work)
         traverse_launch<KERNEL, TOT, true>(s, variant, g, n, pos4, nodes, prm, inv_theta2, eps2,
-                                           scale, ax, ay, tot, counters, batch);
+                                           scale, ax, ay, tot, counters, batch, pstep);
     else
         traverse_launch<KERNEL, TOT, false>(s, variant, g, n, pos4, nodes, prm, inv_theta2, eps2,
-                                            scale, ax, ay, tot, counters, batch);
+                                            scale, ax, ay, tot, counters, batch, pstep);
     return g;
 }
 
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                          const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                          float eps2, float scale, float* ax, float* ay, float* tot,
-                         unsigned long long* counters, TraverseBatch batch, uint32_t F) {
+                         unsigned long long* counters, TraverseBatch batch, uint32_t F,
+                         bool prio) {
     if (kernel == kKernelGravity) {
         if (tot)
             return traverse_variant<kKernelGravity, true>(s, variant, n, pos4, nodes, prm,
                                                           inv_theta2, eps2, scale, ax, ay, tot,
-                                                          counters, batch, F);
+                                                          counters, batch, F, prio);
         return traverse_variant<kKernelGravity, false>(s, variant, n, pos4, nodes, prm, inv_theta2,
-                                                       eps2, scale, ax, ay, tot, counters, batch, F);
+                                                       eps2, scale, ax, ay, tot, counters, batch, F,
+                                                       prio);
     }
     if (tot)
         return traverse_variant<kKernelTsne, true>(s, variant, n, pos4, nodes, prm, inv_theta2,
-                                                   eps2, scale, ax, ay, tot, counters, batch, F);
+                                                   eps2, scale, ax, ay, tot, counters, batch, F,
+                                                   prio);
     return traverse_variant<kKernelTsne, false>(s, variant, n, pos4, nodes, prm, inv_theta2, eps2,
-                                                scale, ax, ay, tot, counters, batch, F);
+                                                scale, ax, ay, tot, counters, batch, F, prio);
 }
 
 uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4, const Node* nodes,
@@ -691,7 +764,8 @@ uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4,
                                  float inv_theta2, float inv_band2, float eps2, float scale,
                                  int terms, float* ax, float* ay, float* tot,
                                  unsigned long long* counters) {
-    // the variant's sinks per lane; branchy monopoles, one source per particle-particle fetch
+    // the variant's sinks per lane; branchy monopoles, one source per particle-particle fetch;
+    // no priority steps (the learned walks of interest span several rounds of waves)
     const int ns = variant_ns(variant);
     const uint32_t g = div_up(n, (uint64_t)kBlock * ns);
     LearnArgs la;
@@ -700,7 +774,7 @@ uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4,
     TraverseBatch none;
 #define BH_TRAVL(NS, TOT, T)                                                                      \
     k_traverse<kKernelGravity, TOT, NS, true, 1, false, false, false, T><<<g, kBlock, 0, s>>>(    \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, 1u, la)
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, 1u, la, 0u)
 #define BH_TRAVL_T(NS, TOT) \
     if (terms == 3) BH_TRAVL(NS, TOT, 3); else BH_TRAVL(NS, TOT, 7)
     if (ns == 1) {

@@ -270,6 +270,16 @@ class BHContext:
         self._c(_lib.load().bh_set_solver(self._h, self.SOLVERS[solver]))
         self._solver = solver
 
+    def set_priority(self, on=True):
+        """Progress-ordered issue priority of the walk's waves (bh_set_walk_priority)."""
+        self._c(_lib.load().bh_set_walk_priority(self._h, 1 if on else 0))
+
+    @property
+    def priority(self):
+        v = C.c_int(0)
+        self._c(_lib.load().bh_get_walk_priority(self._h, C.byref(v)))
+        return bool(v.value)
+
     def set_split(self, split):
         """Waves per sink group in the traversal (1..8); see bh_set_traverse_split."""
         self._c(_lib.load().bh_set_traverse_split(self._h, int(split)))

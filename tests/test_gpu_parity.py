@@ -435,6 +435,31 @@ def test_every_traversal_variant(gpu, variant, kernel):
                      tot)
 
 
+@pytest.mark.parametrize("variant", [4, 6, 8, 14, 22, 26])
+def test_walk_priority_is_scheduling_only(gpu, variant):
+    """bh_set_walk_priority changes only the order in which waves are issued: forces, potential
+    and interaction counts are bit-identical with it on and off (1M sinks with 2 or 4 per lane:
+    4 to 8 waves per SIMD, all resident, so the priority steps are applied and reorder the
+    issue)."""
+    p = two_plummer_disks(1 << 20, seed=21)
+    ctx = _ctx(p.n, kernel="gravity", tree_mode="quadtree", softening=EPS)
+    ctx.set_particles(p.x, p.y, p.m)
+    ctx.set_variant(variant)
+    ctx.build()
+    assert ctx.priority  # on by default
+    res = {}
+    for on in (True, False, True):
+        ctx.set_priority(on)
+        ax, ay, tot = ctx.forces(0.5)
+        res.setdefault(on, []).append((ax, ay, tot, ctx.counters()))
+    for ax, ay, tot, cnt in res[True] + res[False][1:]:
+        ax0, ay0, tot0, cnt0 = res[False][0]
+        assert np.array_equal(ax.view(np.uint32), ax0.view(np.uint32))
+        assert np.array_equal(ay.view(np.uint32), ay0.view(np.uint32))
+        assert tot == tot0 and cnt == cnt0
+    ctx.close()
+
+
 def test_phase_timers_by_mask(gpu):
     """bh_profile_phases: only the selected phases record events and launches; a mask with bits
     beyond the last phase is refused; bh_profile_enable(1) times all of them again."""
