@@ -324,10 +324,9 @@ def tsne_run(args):
     y0 = T.initial_embedding(n, seed=SEED_TSNE, std=1e-2)
     run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
     kl0, _ = run.costFunction()
-    # warm, then tune the walk variant on the trees of three snapshots of the embedding (end of
-    # the exaggeration phase, mid-run, end): the walks get more expensive as the embedding
-    # spreads, so the choice minimises their summed time. The split walk is left out: it wins
-    # on repeated walks of one tree, but loses inside the iteration loop (measured)
+    # warm, then tune the walk variant and split on the trees of three snapshots of the
+    # embedding (end of the exaggeration phase, mid-run, end): the walks get more expensive as
+    # the embedding spreads, so the choice minimises their summed time
     if args.tsne_split > 0:
         run.run(args.tsne_iters)
         best, tuning = (args.variant if args.variant >= 0 else 4, args.tsne_split), None
@@ -337,7 +336,7 @@ def tsne_run(args):
         for stop in sorted({min(260, args.tsne_iters), args.tsne_iters // 2, args.tsne_iters}):
             run.run(stop - done)
             done = stop
-            _, times = tune_split(run.ctx, 0.5, splits=(1,))
+            _, times = tune_split(run.ctx, 0.5, splits=(1, 2, 4, 8))
             for key, ms in times.items():
                 tuning[key] = tuning.get(key, 0.0) + ms
         v, s = min(tuning, key=tuning.get).split("/")

@@ -345,10 +345,12 @@ bh_status bh_synchronize(bh_ctx* ctx);
  * are identical up to FP32 summation order. */
 int bh_traverse_variant_count(void);
 bh_status bh_set_traverse_variant(bh_ctx* ctx, int variant);
-/* Tree-split walk for small N (no reference counterpart): `split` waves (1..8) share each sink
- * group, wave f owning the pre-order node range [f nodes / split, (f+1) nodes / split); every
- * interaction is evaluated by exactly one wave and the partial results are summed in wave
- * order. Multiplies the waves of a launch when the sinks alone cannot fill the device.
+/* Split walk for small N (no reference counterpart): `split` waves (1..8) share each sink
+ * group and take the same path through the tree; wave f adds the monopoles of every split-th
+ * node visit and, in each particle-particle cell, the sources of the f-th of `split` equal
+ * slices of the cell. Every interaction is evaluated by exactly one wave and the partial
+ * results are summed in wave order. Multiplies the waves of a launch when the sinks alone
+ * cannot fill the device and divides the long particle-particle loops of crowded leaves.
  * Results equal split 1 up to FP32 summation order; interaction counts are identical.
  * Batched contexts accept only 1; bh_forces_learned always walks with split 1. */
 bh_status bh_set_traverse_split(bh_ctx* ctx, int split);

@@ -135,11 +135,14 @@ struct LearnArgs {
 //   wave popcount, and 1/theta^2 sits in a vector register so the limit costs one multiply.
 //   Fewer vector instructions per visit; which form is faster depends on the variant and the
 //   workload (measured both ways), so both are variants.
-// SPLIT: F waves share one sink group; wave f of the group owns the pre-order node range
-// [f nn / F, (f+1) nn / F). A wave skips subtrees that miss its range, adds the monopoles and
-// particle-particle sums only of nodes it owns, and writes partial results to part f of the
-// output; every interaction is counted by exactly one wave. This multiplies the number of waves
-// when there are too few sinks to fill the GPU.
+// SPLIT: F waves share one sink group and take the same path through the tree (the same sinks
+// make the same decisions); wave f adds the monopoles of every F-th node visit (visit index mod
+// F = f) and, in every particle-particle cell, the sources of the f-th of F equal slices of the
+// cell, and writes partial results to part f of the output; every interaction is counted by
+// exactly one wave. This multiplies the waves of a launch when there are too few sinks to fill
+// the GPU, and splits the long particle-particle loops of crowded leaves (t-SNE embeddings
+// gather thousands of points in a leaf of the fixed-depth tree and hundreds in a QuadTree leaf)
+// which a split by node range left to one wave.
 //
 // Issue priority by progress. A SIMD's arbiter issues for the oldest ready wave first, so of
 // the waves dispatched together the early ones run ahead: measured with per-wave timers on the
@@ -179,16 +182,13 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     // XCD a contiguous range of sinks: the balance matters more than L2 locality here)
     const uint32_t wb = blockIdx.x;
     uint32_t base, count = 0, k0 = 0, end0 = 0;
-    uint32_t part = 0, pstart = 0, pend = 0;
+    uint32_t part = 0;
     if constexpr (BATCHED) {
         wave_work(batch, wb * kWavesPerBlock + w, n, NS, nodes, prm, base, count, k0, end0);
     } else if constexpr (SPLIT) {
         const uint32_t wv = wb * kWavesPerBlock + w;
         part = wv % F;
         base = (wv / F) * (uint32_t)(kWave * NS);
-        const uint32_t nnodes = prm->n_nodes;
-        pstart = (uint32_t)((uint64_t)nnodes * part / F);
-        pend = (uint32_t)((uint64_t)nnodes * (part + 1) / F);
     } else {
         base = (wb * kWavesPerBlock + w) * (uint32_t)(kWave * NS);
     }
@@ -227,6 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     uint32_t k = k0, end_cur = end0;
     int sp = 0, since_fold = 0;
     Node cur = nodes[min(k0, nn - 1)];
+    uint32_t vmod = 0;  // SPLIT: node visits mod F
     // priority level (top 2 bits) and the visits left at it (one scalar register: every
     // further register of walk state cost the 1M-sink walk a wave per SIMD)
     uint32_t prio = (3u << 30) | min(max(prio_step, 1u), kPrioLeft);
@@ -245,12 +246,10 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         const Node nd_down = nodes[min(k + 1, nn - 1)];
         const Node nd_skip = nodes[min(next, nn - 1)];
 
-        bool owned = true;       // this wave adds node k's interactions
-        uint64_t enter = ~0ull;  // this wave may open node k
+        bool owned = true;  // this wave adds node k's monopoles
         if constexpr (SPLIT) {
-            if (k >= pend) break;  // every node still to come lies past this wave's range
-            owned = k >= pstart;
-            enter = next > pstart ? ~0ull : 0ull;  // the subtree reaches this wave's range
+            owned = vmod == part;
+            vmod = vmod + 1u == F ? 0u : vmod + 1u;
         }
         const float lim = cur.l2 * inv_t2;
         uint64_t open[NS];
@@ -267,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                        bt[s]);
                     cnt_pn += (uint64_t)__popcll(accm);
                 }
-                open[s] = active[s] & ~accm & enter;
+                open[s] = active[s] & ~accm;
             } else {
                 const bool on = (active[s] >> lane) & 1ull;
                 const bool acc = on && (d2 > lim);
@@ -291,14 +290,14 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         }
                     }
                 }
-                open[s] = __ballot(on && !acc) & enter;
+                open[s] = __ballot(on && !acc);
             }
             open_any |= open[s];
         }
         bool down = false;
         if (open_any) {
             if (cur.meta & kMetaPP) {
-                if (owned) {
+                {  // particle-particle sums (SPLIT: of this wave's slice of the sources)
                     bool mine[NS];
                     float lx[NS], ly[NS], lt[NS];
 #pragma unroll
@@ -307,7 +306,11 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                         : ((open[s] >> lane) & 1ull) != 0ull;
                         lx[s] = ly[s] = lt[s] = 0.f;
                     }
-                    const uint32_t f = cur.first, e = f + cur.count;
+                    uint32_t f = cur.first, e = f + cur.count;  // SPLIT: this wave's slice
+                    if constexpr (SPLIT) {
+                        f = cur.first + (uint32_t)((uint64_t)cur.count * part / F);
+                        e = cur.first + (uint32_t)((uint64_t)cur.count * (part + 1u) / F);
+                    }
                     if constexpr (PPU == 0) {
                         // lane l loads source b + l; the sources are broadcast with readlane,
                         // kLaneUnroll at a time so that their dependency chains interleave
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                     }
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        npp[s] += mine[s] ? cur.count - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
+                        npp[s] += mine[s] ? (e - f) - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
                         kahan_add(ax[s], cax[s], lx[s] + bx[s]);
                         kahan_add(ay[s], cay[s], ly[s] + by[s]);
                         if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
