@@ -339,9 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 
 // ------------------------------------------------------------------------------------------
 // LSD radix sort, 8-bit digits, one launch per pass after one counting launch ("one-sweep").
-//   k_radix_counts: one block per (tile, pass); the tile's digit counts for the pass (only
-//     their totals over tiles are used: the multiset of digits of a pass does not depend on the
-//     order) and the clearing of the tile's look-back words of the pass.
+//   k_radix_counts: one block per 4096-key tile; the tile's digit counts for every pass (the
+//     multiset of digits of a pass does not depend on the order) and the clearing of the tile's
+//     look-back words of every pass.
 //   k_radix_totals: per (pass, digit) the total over tiles.
 //   k_radix_pass: a block takes the next tile ticket, ranks its keys inside each wave with ballot
 //     match masks, publishes the tile's digit counts, and gets the tile's global digit offsets
@@ -367,17 +367,13 @@ static_assert(kBlock == 256, "one thread per digit");
 
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_radix_counts(const K* __restrict__ keys, uint32_t n,
-                                                          uint32_t ntiles,
+                                                          int passes, uint32_t ntiles,
                                                           uint32_t* __restrict__ counts,
                                                           uint32_t* __restrict__ status) {
-    // block (tile, pass): one pass's digit counts of one tile (a 70k-key sort has only 35
-    // tiles; one block per pass gives the device eight times the blocks)
-    const int p = blockIdx.y;
-    __shared__ uint32_t h[256];
-    for (int i = threadIdx.x; i < 256; i += THREADS) {
-        h[i] = 0u;
-        status[((size_t)p * ntiles + blockIdx.x) * 256 + i] = 0u;
-    }
+    __shared__ uint32_t h[kSortMaxPasses][256];
+    for (int i = threadIdx.x; i < kSortMaxPasses * 256; i += THREADS) (&h[0][0])[i] = 0u;
+    for (int i = threadIdx.x; i < passes * 256; i += THREADS)
+        status[((size_t)(i >> 8) * ntiles + blockIdx.x) * 256 + (i & 255)] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * (THREADS * ITEMS);
     K k[ITEMS];  // all loads issued before the first use
@@ -397,15 +393,17 @@ __global__ __launch_bounds__(THREADS) void k_radix_counts(const K* __restrict__ 
         const bool valid = base + r * THREADS + threadIdx.x < n;  // invalid lanes are a suffix
         const K prev = (K)__shfl_up((unsigned long long)k[r], 1);
         const uint64_t stops = __ballot(!valid);
-        const uint32_t d = (uint32_t)(k[r] >> (8 * p)) & 0xFFu;
-        const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> (8 * p)) & 0xFFu));
-        const uint64_t after = (__ballot(head) | stops) & ~le;
-        const uint32_t end = after ? (uint32_t)__builtin_ctzll(after) : 64u;
-        if (head) atomicAdd(&h[d], end - (uint32_t)lane);
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t d = (uint32_t)(k[r] >> (8 * p)) & 0xFFu;
+            const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> (8 * p)) & 0xFFu));
+            const uint64_t after = (__ballot(head) | stops) & ~le;
+            const uint32_t end = after ? (uint32_t)__builtin_ctzll(after) : 64u;
+            if (head) atomicAdd(&h[p][d], end - (uint32_t)lane);
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 256; i += THREADS)
-        counts[((size_t)p * 256 + i) * ntiles + blockIdx.x] = h[i];
+    for (int i = threadIdx.x; i < passes * 256; i += THREADS)
+        counts[(size_t)i * ntiles + blockIdx.x] = h[i >> 8][i & 255];
 }
 
 // one wave per (pass, digit) row of counts; block 0 also clears the pass tickets
@@ -1332,7 +1330,7 @@ void launch_radix_sort(hipStream_t s, void* keys_a, uint32_t* vals_a, void* keys
     uint32_t* tickets = totals + kSortMaxPasses * 256;
     const bool small = sort_items(n) == kSortItemsSmall;
 #define BH_COUNTS(K, T, I)                                                                        \
-    k_radix_counts<K, T, I><<<dim3(nt, passes), T, 0, s>>>((const K*)keys_a, n, nt, counts, status)
+    k_radix_counts<K, T, I><<<nt, T, 0, s>>>((const K*)keys_a, n, passes, nt, counts, status)
     if (wide) {
         if (small) BH_COUNTS(uint64_t, 512, 4); else BH_COUNTS(uint64_t, 1024, 4);
     } else {
