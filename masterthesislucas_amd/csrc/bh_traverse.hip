@@ -154,7 +154,8 @@ struct LearnArgs {
 // 1M-sink walk (2 sinks per lane, 8 waves per SIMD) 0.893 -> 0.826 ms. One scalar register
 // carries the level and the countdown: at 76 scalar registers instead of 72 the hardware held 7
 // waves per SIMD instead of 8, and the priority then made the walk slower, not faster.
-// Scheduling only: every result is bit-identical with or without it (prio_step = 0: off).
+// PRIO compiles the steps in; launches without them run an instantiation without that code.
+// Scheduling only: every result is bit-identical with or without it.
 constexpr uint32_t kPrioLeft = (1u << 30) - 1u;
 template <int KERNEL, bool TOT, int NS, bool BRANCHY, int PPU, bool BATCHED, bool SPLIT,
           bool SMASK, int LTERMS = 0, bool PRIO = false>
