@@ -222,7 +222,7 @@ def learned_cpu_baseline(p, eps, theta=0.5):
                       f"particles of the evaluation system, FP64"}
 
 
-def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
+def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None, split=None):
     """Batched systems: `args.batched_systems` independent systems with N log-uniform in
     [1,024, 262,144] (even: uniform disks, odd: Plummer a = 0.1, untruncated unless rmax > 0),
     all in one context (one set of build launches, one traversal launch per step). Leapfrog
@@ -242,9 +242,26 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
     ctx.build()
     times = {}
     if variant is None:
+        from masterthesislucas_amd.autotune import time_variant
+
         variant, times = tune(ctx, theta)
+        # then the split (each work entry walked by F waves, each taking a slice of every
+        # particle-particle cell) for the chosen variant
+        best_ms = min(times[variant], times.get(f"{variant}:no-priority", times[variant]))
+        split = 1
+        for F in (2, 4, 8):
+            ctx.set_split(F)
+            ms = time_variant(ctx, variant, theta)
+            times[f"{variant}/split{F}"] = ms
+            if ms < best_ms:
+                split, best_ms = F, ms
     else:
         ctx.set_variant(variant)
+    ctx.set_split(1)
+    # walk the heaviest work entries first (bh_batch_order_by_cost, from one walk's counters)
+    ctx.forces_device_only(theta)
+    ctx.order_by_cost()
+    ctx.set_split(split or 1)
     ctx.set_particles(*cat)
     K = args.batched_steps
     ctx.leapfrog(2, dt, theta)
@@ -270,7 +287,7 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None):
                         f"even uniform disk / odd {plum}, eps={eps}, theta={theta}",
             "systems": len(sizes), "particles": n, "n_min": int(min(sizes)),
             "n_max": int(max(sizes)), "steps": K, "traverse_variant": variant,
-            "walk_priority": prio,
+            "traverse_split": split or 1, "walk_priority": prio, "ordered_by_cost": True,
             "autotune_ms": {str(k): round(v, 4) for k, v in times.items()},
             "eager_ms_per_step": eager * 1e3, "graph_ms_per_step": graph * 1e3,
             "particle_steps_per_s": n / graph, "interactions_per_particle": (pn + pp) / n,
@@ -775,7 +792,8 @@ def main():
     if not args.no_batched:
         result["batched"] = batched_comparison(args, eps, theta, dt)
         result["batched"]["truncated_r3"] = batched_comparison(
-            args, eps, theta, dt, rmax=3.0, variant=result["batched"]["traverse_variant"])
+            args, eps, theta, dt, rmax=3.0, variant=result["batched"]["traverse_variant"],
+            split=result["batched"]["traverse_split"])
     if not args.no_learned:
         result["learned"] = learned_comparison(args, eps)
     if not args.no_tsne:

@@ -116,6 +116,7 @@ SIGNATURES = {
     "bh_set_traverse_variant": (_i32, [_vp, C.c_int]),
     "bh_set_traverse_split": (_i32, [_vp, C.c_int]),
     "bh_set_walk_priority": (_i32, [_vp, C.c_int]),
+    "bh_batch_order_by_cost": (_i32, [_vp]),
     "bh_get_walk_priority": (_i32, [_vp, C.POINTER(C.c_int)]),
     "bh_set_solver": (_i32, [_vp, C.c_int]),
     "bh_set_sort_spin_limit": (_i32, [_vp, _u32]),

@@ -400,7 +400,7 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
     const float eps = c->cfg.softening;
     const bool grav = c->cfg.kernel == BH_KERNEL_GRAVITY;
     // a split walk writes F parts of n per output, summed in part order afterwards
-    const uint32_t F = c->nsys ? 1u : c->split;
+    const uint32_t F = c->split;
     const size_t pn = (size_t)F * c->cfg.n_max;
     float* ox = F > 1 ? c->part : c->ax;
     float* oy = F > 1 ? c->part + pn : c->ay;
@@ -1607,6 +1607,40 @@ bh_status bh_set_solver(bh_ctx* c, int solver) {
     return BH_OK;
 }
 
+bh_status bh_batch_order_by_cost(bh_ctx* c) {
+    if (!c) return BH_ERR_ARG;
+    if (!c->nsys) return fail(c, BH_ERR_STATE, "not a batched context");
+    if (!c->have_acc) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    if (c->split != 1) return fail(c, BH_ERR_STATE, "order by cost after a walk with split 1");
+    const int v = variant_ns_index(c->variant);
+    const uint32_t nw = c->n_work[v], nb = (nw + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (c->trav_blocks != nb)
+        return fail(c, BH_ERR_STATE, "the last walk used another variant's work table");
+    std::vector<unsigned long long> cnt(2ull * nb);
+    std::vector<uint4> wk(nw), out;
+    bh_status st = hip_check(c, hipMemcpyAsync(cnt.data(), c->counters, cnt.size() * 8,
+                                               hipMemcpyDeviceToHost, c->stream), "counters");
+    if (st == BH_OK)
+        st = hip_check(c, hipMemcpyAsync(wk.data(), c->work[v], (size_t)nw * sizeof(uint4),
+                                         hipMemcpyDeviceToHost, c->stream), "work");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "order by cost");
+    if (st != BH_OK) return st;
+    // whole blocks of kWavesPerBlock entries, heaviest first; a last, partial block stays last
+    const uint32_t full = nw / kWavesPerBlock;
+    std::vector<std::pair<unsigned long long, uint32_t>> order(full);  // (~cost, block)
+    for (uint32_t b = 0; b < full; ++b) order[b] = {~(cnt[2 * b] + cnt[2 * b + 1]), b};
+    std::sort(order.begin(), order.end());  // descending cost, ties by block
+    out.reserve(nw);
+    for (const auto& ob : order)
+        for (uint32_t e = 0; e < (uint32_t)kWavesPerBlock; ++e)
+            out.push_back(wk[(size_t)ob.second * kWavesPerBlock + e]);
+    for (uint32_t e = full * kWavesPerBlock; e < nw; ++e) out.push_back(wk[e]);
+    st = hip_check(c, hipMemcpyAsync(c->work[v], out.data(), (size_t)nw * sizeof(uint4),
+                                     hipMemcpyHostToDevice, c->stream), "work");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "order by cost");
+    return st;
+}
+
 bh_status bh_set_walk_priority(bh_ctx* c, int on) {
     if (!c) return BH_ERR_ARG;
     if ((on != 0) != c->walk_prio) c->g_valid = false;  // captured steps hold the old walk
@@ -1624,8 +1658,6 @@ bh_status bh_set_traverse_split(bh_ctx* c, int split) {
     if (!c) return BH_ERR_ARG;
     if (split < 1 || split > (int)kMaxSplit)
         return fail(c, BH_ERR_ARG, "traversal split %d outside [1, %u]", split, kMaxSplit);
-    if (c->nsys && split != 1)
-        return fail(c, BH_ERR_ARG, "batched contexts traverse with split 1");
     if (split > 1 && (uint32_t)split > c->part_split) {
         // grown in place of the old buffer: 3 outputs x F parts x n_max
         bh_status st = hip_check(c, hipStreamSynchronize(c->stream), "split sync");

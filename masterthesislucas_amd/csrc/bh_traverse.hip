@@ -185,7 +185,12 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     uint32_t base, count = 0, k0 = 0, end0 = 0;
     uint32_t part = 0;
     if constexpr (BATCHED) {
-        wave_work(batch, wb * kWavesPerBlock + w, n, NS, nodes, prm, base, count, k0, end0);
+        uint32_t wv = wb * kWavesPerBlock + w;
+        if constexpr (SPLIT) {  // F waves per work entry
+            part = wv % F;
+            wv /= F;
+        }
+        wave_work(batch, wv, n, NS, nodes, prm, base, count, k0, end0);
     } else if constexpr (SPLIT) {
         const uint32_t wv = wb * kWavesPerBlock + w;
         part = wv % F;
@@ -689,20 +694,24 @@ static void traverse_launch(hipStream_t s, int variant, uint32_t g, uint32_t n, 
 }
 
 // split traversal: the variant's NS and branchiness, particle-particle sources by scalar load
-// (PPU 1, for PPU 1 and 4) or lane-loaded (PPU 0); F partial outputs of n each
+// (PPU 1, for PPU 1 and 4) or lane-loaded (PPU 0); F partial outputs of n each; one-system
+// sink groups or the work entries of a batched context (F waves each)
 template <int KERNEL, bool TOT>
 static uint32_t traverse_split(hipStream_t s, int variant, uint32_t F, uint32_t n,
                                const float4* pos4, const Node* nodes, const BuildParams* prm,
                                float inv_theta2, float eps2, float scale, float* ax, float* ay,
-                               float* tot, unsigned long long* counters) {
+                               float* tot, unsigned long long* counters, TraverseBatch batch) {
     const int ns = variant_ns(variant);
-    const uint64_t waves = (uint64_t)div_up(n, (uint64_t)kWave * ns) * F;
-    const uint32_t g = div_up(waves, kWavesPerBlock);
-    TraverseBatch none;
-#define BH_TRAVS(NS, BR, U)                                                                       \
-    k_traverse<KERNEL, TOT, NS, BR, U, false, true, false><<<g, kBlock, 0, s>>>(                  \
-        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, none, F, LearnArgs{}, \
+    const uint64_t groups = batch.work ? batch.n_work : div_up(n, (uint64_t)kWave * ns);
+    const uint32_t g = div_up(groups * F, kWavesPerBlock);
+#define BH_TRAVS_B(NS, BR, U, B)                                                                  \
+    k_traverse<KERNEL, TOT, NS, BR, U, B, true, false><<<g, kBlock, 0, s>>>(                      \
+        pos4, nodes, prm, n, inv_theta2, eps2, scale, ax, ay, tot, counters, batch, F, LearnArgs{}, \
         0u)
+#define BH_TRAVS(NS, BR, U)                                                                       \
+    do {                                                                                          \
+        if (batch.work) BH_TRAVS_B(NS, BR, U, true); else BH_TRAVS_B(NS, BR, U, false);           \
+    } while (0)
 #define BH_TRAVS_U(NS, BR)                                                                        \
     if (lane_pp) BH_TRAVS(NS, BR, 0); else BH_TRAVS(NS, BR, 1)
     const bool br = variant_branchy(variant);
@@ -716,6 +725,7 @@ static uint32_t traverse_split(hipStream_t s, int variant, uint32_t F, uint32_t 
     }
 #undef BH_TRAVS_U
 #undef BH_TRAVS
+#undef BH_TRAVS_B
     return g;
 }
 
@@ -726,9 +736,9 @@ static uint32_t traverse_variant(hipStream_t s, int variant, uint32_t n, const f
                                  unsigned long long* counters, TraverseBatch batch, uint32_t F,
                                  bool prio) {
     const int ns = variant_ns(variant);
-    if (F > 1 && !batch.work)  // split walks: no priority steps
+    if (F > 1)  // split walks: no priority steps
         return traverse_split<KERNEL, TOT>(s, variant, F, n, pos4, nodes, prm, inv_theta2, eps2,
-                                           scale, ax, ay, tot, counters);
+                                           scale, ax, ay, tot, counters, batch);
     const uint32_t g = batch.work ? div_up(batch.n_work, kWavesPerBlock)
                                   : div_up(n, (uint64_t)kBlock * ns);
     const uint32_t pstep = launch_prio_step(prio, (uint64_t)g * kWavesPerBlock, inv_theta2);

@@ -270,6 +270,10 @@ class BHContext:
         self._c(_lib.load().bh_set_solver(self._h, self.SOLVERS[solver]))
         self._solver = solver
 
+    def order_by_cost(self):
+        """Batched contexts: heaviest work entries of the last walk first (bh_batch_order_by_cost)."""
+        self._c(_lib.load().bh_batch_order_by_cost(self._h))
+
     def set_priority(self, on=True):
         """Progress-ordered issue priority of the walk's waves (bh_set_walk_priority)."""
         self._c(_lib.load().bh_set_walk_priority(self._h, 1 if on else 0))

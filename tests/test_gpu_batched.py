@@ -125,3 +125,36 @@ def test_graph_steps_equal_eager_steps(gpu, batched):
             assert np.array_equal(a, b), (steps, theta)
     for c in ctxs:
         c.close()
+
+
+def test_order_by_cost_is_scheduling_only(gpu):
+    """bh_batch_order_by_cost permutes the work table (heaviest blocks of the last walk first):
+    forces, potential and counts are bit-identical before and after, and it is refused where it
+    does not apply."""
+    from masterthesislucas_amd.workloads import batched_systems
+    ps = batched_systems(12, seed=73)
+    sizes = [len(q.x) for q in ps]
+    cat = [np.concatenate([getattr(q, f) for q in ps]) for f in ("x", "y", "m")]
+    ctx = BHContext(int(sum(sizes)), kernel="gravity", softening=EPS, sizes=sizes)
+    ctx.set_particles(*cat)
+    ctx.build()
+    with pytest.raises(BHError):
+        ctx.order_by_cost()  # no walk yet
+    ax0, ay0, t0 = ctx.forces(0.5)
+    c0 = ctx.counters()
+    ctx.order_by_cost()
+    ax1, ay1, t1 = ctx.forces(0.5)
+    assert np.array_equal(ax0.view(np.uint32), ax1.view(np.uint32))
+    assert np.array_equal(ay0.view(np.uint32), ay1.view(np.uint32))
+    assert t0 == t1 and ctx.counters() == c0
+    ctx.order_by_cost()  # again, from the reordered table's counters
+    ax2, _, _ = ctx.forces(0.5)
+    assert np.array_equal(ax0.view(np.uint32), ax2.view(np.uint32))
+    ctx.close()
+    one = BHContext(1000, kernel="gravity", softening=EPS)
+    one.set_particles(cat[0][:1000], cat[1][:1000], cat[2][:1000])
+    one.build()
+    one.forces(0.5)
+    with pytest.raises(BHError):
+        one.order_by_cost()  # not batched
+    one.close()
