@@ -791,9 +791,7 @@ def main():
     ctx.close()
     if not args.no_batched:
         result["batched"] = batched_comparison(args, eps, theta, dt)
-        result["batched"]["truncated_r3"] = batched_comparison(
-            args, eps, theta, dt, rmax=3.0, variant=result["batched"]["traverse_variant"],
-            split=result["batched"]["traverse_split"])
+        result["batched"]["truncated_r3"] = batched_comparison(args, eps, theta, dt, rmax=3.0)
     if not args.no_learned:
         result["learned"] = learned_comparison(args, eps)
     if not args.no_tsne:

@@ -362,11 +362,15 @@ bh_status bh_set_traverse_split(bh_ctx* ctx, int split);
  * not learned) whose waves all fit on the device at once (at most 8 per SIMD); other launches
  * walk without it. Scheduling only: results and interaction counts are bit-identical either
  * way. */
-/* Batched contexts (no reference counterpart): reorder the work table of the current variant
- * so that the entries whose last walk (split 1) cost the most interactions are walked first, in
- * whole blocks. Waves are otherwise dispatched in table order, and on the 256-system workload a
- * handful of waves over thousands of one-child-chain occupants ran 17 ms of a 20 ms walk
- * starting 3.4 ms in. Scheduling only: results and counts are bit-identical. */
+/* Batched contexts (no reference counterpart): rebuild the work table of the current variant
+ * from the interaction counts of the last walk (split 1): entries in descending cost, and the
+ * entries of at least 16 times the mean cost whose work is mostly particle-particle sums walked
+ * as 8 slices each (every slice takes the path; slice f adds every 8th visit's monopoles and
+ * the f-th eighth of each particle-particle cell's sources; the slices' partial results are
+ * added in slice order). Waves are otherwise dispatched in system order, and on the 256-system
+ * workload a handful of waves over the thousands of occupants of one-child chains ran 17 ms of
+ * a 20 ms walk. Interaction counts are unchanged; forces change only in the summation order of
+ * the sliced entries. bh_set_traverse_split(ctx, F > 1) restores the system order. */
 bh_status bh_batch_order_by_cost(bh_ctx* ctx);
 bh_status bh_set_walk_priority(bh_ctx* ctx, int on);
 bh_status bh_get_walk_priority(bh_ctx* ctx, int* on);

@@ -130,6 +130,10 @@ struct bh_ctx {
     float4* bpart = nullptr;
     uint4* work[3]{};              // traversal work tables for 1, 2, 4 sinks per lane
     uint32_t n_work[3]{};
+    std::vector<uint4> h_work0[3];  // the tables as built (system order, no heavy slices)
+    uint4* heavy[3]{};              // per table: (first sink, count, slices) of its heavy slots
+    uint32_t nheavy[3]{};
+    float* side = nullptr;          // partial results of heavy slices (TraverseBatch::side)
 
     // captured leapfrog steps (bh_leapfrog_graph), one per ping-pong parity
     hipGraph_t graph[2]{};
@@ -370,6 +374,7 @@ TraverseBatch batch_of(const bh_ctx* c, int ns_index) {
         b.work = c->work[ns_index];
         b.n_work = c->n_work[ns_index];
         b.roots = c->roots;
+        b.side = c->side;
     }
     return b;
 }
@@ -418,6 +423,10 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
         launch_sum_parts(s, ox, F, c->n, c->ax);
         launch_sum_parts(s, oy, F, c->n, c->ay);
         if (want_total) launch_sum_parts(s, ot, F, c->n, c->tot);
+    } else if (c->nsys) {
+        const int v = variant_ns_index(c->variant);
+        launch_sum_heavy(s, c->side, c->heavy[v], c->nheavy[v], c->ax, c->ay,
+                         want_total ? c->tot : nullptr);
     }
     phase_end(c, 5);
     if (want_total)
@@ -700,10 +709,14 @@ bh_status bh_create_batched(bh_ctx** out, const bh_config* cfg, const uint32_t* 
             for (uint32_t f = 0; f < sizes[q]; f += per)
                 wk.push_back(make_uint4(c->h_sys_off[q] + f, std::min(per, sizes[q] - f), q, 0u));
         c->n_work[v] = (uint32_t)wk.size();
-        TRYB(dalloc(c, &c->work[v], wk.size()));
+        // room for the heavy entries' extra slices (bh_batch_order_by_cost)
+        TRYB(dalloc(c, &c->work[v], wk.size() + (size_t)kHeavyMax * (kHeavySlices - 1)));
+        TRYB(dalloc(c, &c->heavy[v], kHeavyMax));
         TRYB(hip_check(c, hipMemcpyAsync(c->work[v], wk.data(), wk.size() * sizeof(uint4),
                                          hipMemcpyHostToDevice, s), "work"));
+        c->h_work0[v] = std::move(wk);
     }
+    TRYB(dalloc(c, &c->side, 3ull * kHeavyMax * kHeavySlices * kHeavySinks));
 #undef TRYB
     return hip_check(c, hipStreamSynchronize(s), "batched init");
 }
@@ -1607,6 +1620,32 @@ bh_status bh_set_solver(bh_ctx* c, int solver) {
     return BH_OK;
 }
 
+// Uploads work table v: `order` lists original entries (h_work0[v]) heaviest first; the first
+// `nh` of them are walked as kHeavySlices slices each.
+static bh_status upload_work(bh_ctx* c, int v, const std::vector<uint32_t>& order, uint32_t nh) {
+    const std::vector<uint4>& w0 = c->h_work0[v];
+    std::vector<uint4> out, hv(nh);
+    out.reserve(w0.size() + (size_t)nh * (kHeavySlices - 1));
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint4 e = w0[order[h]];
+        hv[h] = make_uint4(e.x, e.y, kHeavySlices, 0u);
+        for (uint32_t f = 0; f < kHeavySlices; ++f)
+            out.push_back(make_uint4(e.x, e.y, e.z, (h << 8) | (kHeavySlices << 4) | f));
+    }
+    for (size_t i = nh; i < order.size(); ++i) out.push_back(w0[order[i]]);
+    bh_status st = hip_check(c, hipMemcpyAsync(c->work[v], out.data(), out.size() * sizeof(uint4),
+                                               hipMemcpyHostToDevice, c->stream), "work");
+    if (st == BH_OK && nh)
+        st = hip_check(c, hipMemcpyAsync(c->heavy[v], hv.data(), nh * sizeof(uint4),
+                                         hipMemcpyHostToDevice, c->stream), "heavy");
+    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "work table");
+    if (st != BH_OK) return st;
+    c->n_work[v] = (uint32_t)out.size();
+    c->nheavy[v] = nh;
+    c->g_valid = false;  // captured steps hold the old grid
+    return BH_OK;
+}
+
 bh_status bh_batch_order_by_cost(bh_ctx* c) {
     if (!c) return BH_ERR_ARG;
     if (!c->nsys) return fail(c, BH_ERR_STATE, "not a batched context");
@@ -1617,7 +1656,7 @@ bh_status bh_batch_order_by_cost(bh_ctx* c) {
     if (c->trav_blocks != nb)
         return fail(c, BH_ERR_STATE, "the last walk used another variant's work table");
     std::vector<unsigned long long> cnt(2ull * nb);
-    std::vector<uint4> wk(nw), out;
+    std::vector<uint4> wk(nw);
     bh_status st = hip_check(c, hipMemcpyAsync(cnt.data(), c->counters, cnt.size() * 8,
                                                hipMemcpyDeviceToHost, c->stream), "counters");
     if (st == BH_OK)
@@ -1625,20 +1664,46 @@ bh_status bh_batch_order_by_cost(bh_ctx* c) {
                                          hipMemcpyDeviceToHost, c->stream), "work");
     if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "order by cost");
     if (st != BH_OK) return st;
-    // whole blocks of kWavesPerBlock entries, heaviest first; a last, partial block stays last
-    const uint32_t full = nw / kWavesPerBlock;
-    std::vector<std::pair<unsigned long long, uint32_t>> order(full);  // (~cost, block)
-    for (uint32_t b = 0; b < full; ++b) order[b] = {~(cnt[2 * b] + cnt[2 * b + 1]), b};
-    std::sort(order.begin(), order.end());  // descending cost, ties by block
-    out.reserve(nw);
-    for (const auto& ob : order)
-        for (uint32_t e = 0; e < (uint32_t)kWavesPerBlock; ++e)
-            out.push_back(wk[(size_t)ob.second * kWavesPerBlock + e]);
-    for (uint32_t e = full * kWavesPerBlock; e < nw; ++e) out.push_back(wk[e]);
-    st = hip_check(c, hipMemcpyAsync(c->work[v], out.data(), (size_t)nw * sizeof(uint4),
-                                     hipMemcpyHostToDevice, c->stream), "work");
-    if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "order by cost");
-    return st;
+    // cost of each original entry: its share of its block's interactions in the last walk (a
+    // heavy entry's slices add up); entries are identified by their first sink
+    const std::vector<uint4>& w0 = c->h_work0[v];
+    std::vector<std::pair<uint32_t, uint32_t>> by_first(w0.size());
+    for (uint32_t i = 0; i < w0.size(); ++i) by_first[i] = {w0[i].x, i};
+    std::sort(by_first.begin(), by_first.end());
+    std::vector<unsigned long long> cost(w0.size(), 0ull), cost_pp(w0.size(), 0ull);
+    for (uint32_t e = 0; e < nw; ++e) {
+        const uint32_t b = e / kWavesPerBlock;
+        const uint32_t in_block = std::min<uint32_t>(kWavesPerBlock, nw - b * kWavesPerBlock);
+        const auto it = std::lower_bound(by_first.begin(), by_first.end(),
+                                         std::make_pair(wk[e].x, 0u));
+        if (it == by_first.end() || it->first != wk[e].x) continue;
+        cost[it->second] += (cnt[2 * b] + cnt[2 * b + 1]) / in_block;
+        cost_pp[it->second] += cnt[2 * b + 1] / in_block;
+    }
+    std::vector<std::pair<unsigned long long, uint32_t>> ord(w0.size());  // (~cost, entry)
+    unsigned long long total = 0;
+    for (uint32_t i = 0; i < w0.size(); ++i) {
+        ord[i] = {~cost[i], i};
+        total += cost[i];
+    }
+    std::sort(ord.begin(), ord.end());  // descending cost, ties by entry
+    std::vector<uint32_t> order(w0.size());
+    for (uint32_t i = 0; i < w0.size(); ++i) order[i] = ord[i].second;
+    // heavy: at least kHeavyFactor times the mean entry, and mostly particle-particle work (the
+    // slices share the cells' sources; every slice repeats the walk's visits), at most kHeavyMax
+    // of them (the 256-system workload: a handful of entries over one-child chains of thousands
+    // of occupants, one wave of them 17 ms of a 20 ms walk). They go first, as slices.
+    constexpr unsigned long long kHeavyFactor = 16, kHeavyPP = 4;
+    std::vector<uint32_t> heavy, rest;
+    for (uint32_t i : order) {
+        const bool h = heavy.size() < kHeavyMax && cost[i] > 0 &&
+                       cost[i] * w0.size() >= kHeavyFactor * total &&
+                       cost_pp[i] >= kHeavyPP * (cost[i] - cost_pp[i]);
+        (h ? heavy : rest).push_back(i);
+    }
+    const uint32_t nh = (uint32_t)heavy.size();
+    heavy.insert(heavy.end(), rest.begin(), rest.end());
+    return upload_work(c, v, heavy, nh);
 }
 
 bh_status bh_set_walk_priority(bh_ctx* c, int on) {
@@ -1672,6 +1737,16 @@ bh_status bh_set_traverse_split(bh_ctx* c, int split) {
             return st;
         }
         c->part_split = (uint32_t)split;
+    }
+    if (split > 1 && c->nsys) {
+        // heavy-entry slices and split walks do not combine: back to the tables as built
+        for (int v = 0; v < 3; ++v) {
+            if (!c->nheavy[v]) continue;
+            std::vector<uint32_t> order(c->h_work0[v].size());
+            for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+            bh_status st = upload_work(c, v, order, 0);
+            if (st != BH_OK) return st;
+        }
     }
     if ((uint32_t)split != c->split) c->g_valid = false;  // captured steps hold the old walk
     c->split = (uint32_t)split;

@@ -89,11 +89,14 @@ __device__ __forceinline__ NodeBits load_node(const Node* __restrict__ nodes, ui
 __device__ __forceinline__ void wave_work(const TraverseBatch& batch, uint32_t gw, uint32_t n,
                                           int ns, const Node* __restrict__ nodes,
                                           const BuildParams* __restrict__ prm, uint32_t& base,
-                                          uint32_t& count, uint32_t& k0, uint32_t& end0) {
+                                          uint32_t& count, uint32_t& k0, uint32_t& end0,
+                                          uint32_t& slice) {
+    slice = 0;
     if (batch.work) {
         base = count = k0 = end0 = 0;
         if (gw < batch.n_work) {
             const uint4 wk = batch.work[gw];
+            slice = uniform_u32(wk.w);
             const uint32_t r = uniform_u32(batch.roots[uniform_u32(wk.z)]);
             if (r != kNone) {
                 base = uniform_u32(wk.x);
@@ -184,13 +187,14 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     const uint32_t wb = blockIdx.x;
     uint32_t base, count = 0, k0 = 0, end0 = 0;
     uint32_t part = 0;
+    uint32_t slice = 0;  // BATCHED: slice word of a heavy entry's wave (see TraverseBatch)
     if constexpr (BATCHED) {
         uint32_t wv = wb * kWavesPerBlock + w;
         if constexpr (SPLIT) {  // F waves per work entry
             part = wv % F;
             wv /= F;
         }
-        wave_work(batch, wv, n, NS, nodes, prm, base, count, k0, end0);
+        wave_work(batch, wv, n, NS, nodes, prm, base, count, k0, end0, slice);
     } else if constexpr (SPLIT) {
         const uint32_t wv = wb * kWavesPerBlock + w;
         part = wv % F;
@@ -256,6 +260,11 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         if constexpr (SPLIT) {
             owned = vmod == part;
             vmod = vmod + 1u == F ? 0u : vmod + 1u;
+        } else if constexpr (BATCHED) {
+            if (slice) {  // slice (slice & 15) of (slice >> 4 & 15)
+                owned = vmod == (slice & 15u);
+                vmod = vmod + 1u == ((slice >> 4) & 15u) ? 0u : vmod + 1u;
+            }
         }
         const float lim = cur.l2 * inv_t2;
         uint64_t open[NS];
@@ -316,6 +325,12 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                     if constexpr (SPLIT) {
                         f = cur.first + (uint32_t)((uint64_t)cur.count * part / F);
                         e = cur.first + (uint32_t)((uint64_t)cur.count * (part + 1u) / F);
+                    } else if constexpr (BATCHED) {
+                        if (slice) {
+                            const uint32_t sf = slice & 15u, sF = (slice >> 4) & 15u;
+                            f = cur.first + (uint32_t)((uint64_t)cur.count * sf / sF);
+                            e = cur.first + (uint32_t)((uint64_t)cur.count * (sf + 1u) / sF);
+                        }
                     }
                     if constexpr (PPU == 0) {
                         // lane l loads source b + l; the sources are broadcast with readlane,
@@ -459,7 +474,14 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         kahan_add(ax[s], cax[s], bx[s]);
         kahan_add(ay[s], cay[s], by[s]);
         if (TOT) kahan_add(at[s], cat[s], bt[s]);
-        if (valid[s]) {
+        if (BATCHED && slice && valid[s]) {  // a heavy entry's slice: partial results
+            const size_t plane = (size_t)kHeavyMax * kHeavySlices * kHeavySinks;
+            const size_t o = ((size_t)(slice >> 8) * kHeavySlices + (slice & 15u)) * kHeavySinks +
+                             (uint32_t)(s * kWave + lane);
+            batch.side[o] = scale * (ax[s] - cax[s]);
+            batch.side[plane + o] = scale * (ay[s] - cay[s]);
+            if (TOT) batch.side[2 * plane + o] = scale * (at[s] - cat[s]);
+        } else if (valid[s]) {
             const size_t o = (size_t)part * n + idx[s];  // part 0 unless SPLIT
             ax_out[o] = scale * (ax[s] - cax[s]);
             ay_out[o] = scale * (ay[s] - cay[s]);
@@ -801,6 +823,34 @@ uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4,
 #undef BH_TRAVL_T
 #undef BH_TRAVL
     return g;
+}
+
+// heavy batched entries: out[first + i] = sum over slices f of side[slot][f][i], in slice order
+// (one block per slot)
+__global__ __launch_bounds__(kBlock) void k_sum_heavy(const float* __restrict__ side,
+                                                      const uint4* __restrict__ heavy,
+                                                      float* __restrict__ ax,
+                                                      float* __restrict__ ay,
+                                                      float* __restrict__ tot) {
+    const uint4 h = heavy[blockIdx.x];  // first sink, count, F
+    const size_t plane = (size_t)kHeavyMax * kHeavySlices * kHeavySinks;
+    for (uint32_t i = threadIdx.x; i < h.y; i += kBlock) {
+        float sx = 0.f, sy = 0.f, st = 0.f;
+        for (uint32_t f = 0; f < h.z; ++f) {
+            const size_t o = ((size_t)blockIdx.x * kHeavySlices + f) * kHeavySinks + i;
+            sx += side[o];
+            sy += side[plane + o];
+            if (tot) st += side[2 * plane + o];
+        }
+        ax[h.x + i] = sx;
+        ay[h.x + i] = sy;
+        if (tot) tot[h.x + i] = st;
+    }
+}
+
+void launch_sum_heavy(hipStream_t s, const float* side, const uint4* heavy, uint32_t nheavy,
+                      float* ax, float* ay, float* tot) {
+    if (nheavy) k_sum_heavy<<<nheavy, kBlock, 0, s>>>(side, heavy, ax, ay, tot);
 }
 
 // out[i] = sum over parts f of part[f n + i], in part order
