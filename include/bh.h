@@ -364,8 +364,8 @@ bh_status bh_set_traverse_split(bh_ctx* ctx, int split);
  * way. */
 /* Batched contexts (no reference counterpart): rebuild the work table of the current variant
  * from the interaction counts of the last walk (split 1): entries in descending cost, and the
- * entries of at least 16 times the mean cost whose work is mostly particle-particle sums walked
- * as 8 slices each (every slice takes the path; slice f adds every 8th visit's monopoles and
+ * costliest entries (at least 16 times the mean cost, at most 8192 of them) walked as 8 slices
+ * each (every slice takes the path; slice f adds every 8th visit's monopoles and
  * the f-th eighth of each particle-particle cell's sources; the slices' partial results are
  * added in slice order). Waves are otherwise dispatched in system order, and on the 256-system
  * workload a handful of waves over the thousands of occupants of one-child chains ran 17 ms of

@@ -73,7 +73,7 @@ struct bh_ctx {
     float4* bounds_part = nullptr;
     double* red_part = nullptr;
     uint2* stats_part = nullptr;
-    uint32_t max_blocks = 0;  // traversal blocks for n_max (per-block counter partials)
+    uint32_t max_blocks = 0;  // traversal blocks for n_max (counter pairs: one per wave)
     double* dsum = nullptr;
     unsigned long long* counters = nullptr;
     uint32_t *slot_a = nullptr, *slot_b = nullptr, *slot_c = nullptr;
@@ -90,7 +90,7 @@ struct bh_ctx {
     uint32_t sort_spin = kSortSpinLimit;
     BuildParams* h_prm = nullptr;  // pinned
     double* h_dsum = nullptr;
-    unsigned long long* h_counters = nullptr;  // 2 per traversal block
+    unsigned long long* h_counters = nullptr;  // 2 per traversal wave
     uint2* h_stats = nullptr;
 
     // t-SNE state (caller order): P in CSR, embedding triple buffer, repulsion, derivative
@@ -134,6 +134,7 @@ struct bh_ctx {
     uint4* heavy[3]{};              // per table: (first sink, count, slices) of its heavy slots
     uint32_t nheavy[3]{};
     float* side = nullptr;          // partial results of heavy slices (TraverseBatch::side)
+    uint32_t side_cap = 0;          // heavy entries `side` holds room for
 
     // captured leapfrog steps (bh_leapfrog_graph), one per ping-pong parity
     hipGraph_t graph[2]{};
@@ -375,6 +376,7 @@ TraverseBatch batch_of(const bh_ctx* c, int ns_index) {
         b.n_work = c->n_work[ns_index];
         b.roots = c->roots;
         b.side = c->side;
+        b.side_slots = c->side_cap;
     }
     return b;
 }
@@ -425,7 +427,7 @@ bh_status do_forces(bh_ctx* c, double theta, bool want_total) {
         if (want_total) launch_sum_parts(s, ot, F, c->n, c->tot);
     } else if (c->nsys) {
         const int v = variant_ns_index(c->variant);
-        launch_sum_heavy(s, c->side, c->heavy[v], c->nheavy[v], c->ax, c->ay,
+        launch_sum_heavy(s, c->side, c->side_cap, c->heavy[v], c->nheavy[v], c->ax, c->ay,
                          want_total ? c->tot : nullptr);
     }
     phase_end(c, 5);
@@ -476,6 +478,7 @@ void bh_destroy(bh_ctx* c) {
     if (c->dpart) (void)hipFree(c->dpart);
     if (c->part) (void)hipFree(c->part);
     if (c->fmm_buf) (void)hipFree(c->fmm_buf);
+    if (c->side) (void)hipFree(c->side);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_prm) (void)hipHostFree(c->h_prm);
     if (c->h_dsum) (void)hipHostFree(c->h_dsum);
@@ -594,7 +597,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     TRY(dalloc(c, &c->red_part, kMaxReduceBlocks));
     TRY(dalloc(c, &c->dsum, 5));  // [0] the walk's FP64 total, [1..4] state sums
     c->max_blocks = traverse_blocks(N) * kMaxSplit + extra_blocks;
-    TRY(dalloc(c, &c->counters, 2ull * c->max_blocks));
+    TRY(dalloc(c, &c->counters, 2ull * kWavesPerBlock * c->max_blocks));
     TRY(dalloc(c, &c->where, N));
     if (cfg->kernel == BH_KERNEL_TSNE) {
         for (int k = 0; k < 3; ++k) {
@@ -635,7 +638,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_stats, kStatsBlocks * sizeof(uint2), 0),
                   "hipHostMalloc"));
     TRY(hip_check(c, hipHostMalloc((void**)&c->h_counters,
-                                   2ull * c->max_blocks * sizeof(unsigned long long), 0),
+                                   2ull * kWavesPerBlock * c->max_blocks * sizeof(unsigned long long), 0),
                   "hipHostMalloc"));
     launch_fill_f32(c->stream, (float*)c->pos4, 4ull * ((uint64_t)N + kPosPad), 0.0f);
     launch_fill_u32(c->stream, (uint32_t*)c->prm, sizeof(BuildParams) / 4, 0u);
@@ -716,7 +719,6 @@ bh_status bh_create_batched(bh_ctx** out, const bh_config* cfg, const uint32_t* 
                                          hipMemcpyHostToDevice, s), "work"));
         c->h_work0[v] = std::move(wk);
     }
-    TRYB(dalloc(c, &c->side, 3ull * kHeavyMax * kHeavySlices * kHeavySinks));
 #undef TRYB
     return hip_check(c, hipStreamSynchronize(s), "batched init");
 }
@@ -1544,7 +1546,7 @@ bh_status bh_get_tree_slots(bh_ctx* c, int32_t* depth, uint64_t* n_slots, uint32
 bh_status bh_counters(bh_ctx* c, uint64_t* pn, uint64_t* pp) {
     if (!c) return BH_ERR_ARG;
     if (!c->have_acc) return fail(c, BH_ERR_STATE, "no force evaluation yet");
-    const uint32_t nb = c->trav_blocks;
+    const uint32_t nb = c->trav_blocks * kWavesPerBlock;  // waves of the last walk
     bh_status st = hip_check(c, hipMemcpyAsync(c->h_counters, c->counters,
                                                2ull * nb * sizeof(unsigned long long),
                                                hipMemcpyDeviceToHost, c->stream), "counters");
@@ -1624,6 +1626,17 @@ bh_status bh_set_solver(bh_ctx* c, int solver) {
 // `nh` of them are walked as kHeavySlices slices each.
 static bh_status upload_work(bh_ctx* c, int v, const std::vector<uint32_t>& order, uint32_t nh) {
     const std::vector<uint4>& w0 = c->h_work0[v];
+    if (nh > c->side_cap) {  // grown in place of the old buffer: 3 planes x nh x slices x sinks
+        bh_status st = hip_check(c, hipStreamSynchronize(c->stream), "side sync");
+        if (st != BH_OK) return st;
+        if (c->side) (void)hipFree(c->side);
+        c->side = nullptr;
+        c->side_cap = 0;
+        st = hip_check(c, hipMalloc((void**)&c->side, 3ull * nh * kHeavySlices * kHeavySinks * 4),
+                       "hipMalloc heavy slices");
+        if (st != BH_OK) return st;
+        c->side_cap = nh;
+    }
     std::vector<uint4> out, hv(nh);
     out.reserve(w0.size() + (size_t)nh * (kHeavySlices - 1));
     for (uint32_t h = 0; h < nh; ++h) {
@@ -1655,7 +1668,7 @@ bh_status bh_batch_order_by_cost(bh_ctx* c) {
     const uint32_t nw = c->n_work[v], nb = (nw + kWavesPerBlock - 1) / kWavesPerBlock;
     if (c->trav_blocks != nb)
         return fail(c, BH_ERR_STATE, "the last walk used another variant's work table");
-    std::vector<unsigned long long> cnt(2ull * nb);
+    std::vector<unsigned long long> cnt(2ull * nw);  // per wave: entry e's pair
     std::vector<uint4> wk(nw);
     bh_status st = hip_check(c, hipMemcpyAsync(cnt.data(), c->counters, cnt.size() * 8,
                                                hipMemcpyDeviceToHost, c->stream), "counters");
@@ -1664,22 +1677,23 @@ bh_status bh_batch_order_by_cost(bh_ctx* c) {
                                          hipMemcpyDeviceToHost, c->stream), "work");
     if (st == BH_OK) st = hip_check(c, hipStreamSynchronize(c->stream), "order by cost");
     if (st != BH_OK) return st;
-    // cost of each original entry: its share of its block's interactions in the last walk (a
-    // heavy entry's slices add up); entries are identified by their first sink
+    // cost of each original entry: its wave's interactions in the last walk (a heavy entry's
+    // slices add up; entries are identified by their first sink), per lane: the wave's walk is
+    // shared by its lanes, so its time follows a lane's sinks (sink k of an entry sits on lane
+    // k mod 64), not the entry's sink count
     const std::vector<uint4>& w0 = c->h_work0[v];
     std::vector<std::pair<uint32_t, uint32_t>> by_first(w0.size());
     for (uint32_t i = 0; i < w0.size(); ++i) by_first[i] = {w0[i].x, i};
     std::sort(by_first.begin(), by_first.end());
-    std::vector<unsigned long long> cost(w0.size(), 0ull), cost_pp(w0.size(), 0ull);
+    std::vector<unsigned long long> cost(w0.size(), 0ull);
     for (uint32_t e = 0; e < nw; ++e) {
-        const uint32_t b = e / kWavesPerBlock;
-        const uint32_t in_block = std::min<uint32_t>(kWavesPerBlock, nw - b * kWavesPerBlock);
         const auto it = std::lower_bound(by_first.begin(), by_first.end(),
                                          std::make_pair(wk[e].x, 0u));
         if (it == by_first.end() || it->first != wk[e].x) continue;
-        cost[it->second] += (cnt[2 * b] + cnt[2 * b + 1]) / in_block;
-        cost_pp[it->second] += cnt[2 * b + 1] / in_block;
+        cost[it->second] += cnt[2 * e] + cnt[2 * e + 1];
     }
+    for (uint32_t i = 0; i < w0.size(); ++i)
+        if (w0[i].y) cost[i] = cost[i] * ((w0[i].y + kWave - 1) / kWave) / w0[i].y;
     std::vector<std::pair<unsigned long long, uint32_t>> ord(w0.size());  // (~cost, entry)
     unsigned long long total = 0;
     for (uint32_t i = 0; i < w0.size(); ++i) {
@@ -1689,21 +1703,17 @@ bh_status bh_batch_order_by_cost(bh_ctx* c) {
     std::sort(ord.begin(), ord.end());  // descending cost, ties by entry
     std::vector<uint32_t> order(w0.size());
     for (uint32_t i = 0; i < w0.size(); ++i) order[i] = ord[i].second;
-    // heavy: at least kHeavyFactor times the mean entry, and mostly particle-particle work (the
-    // slices share the cells' sources; every slice repeats the walk's visits), at most kHeavyMax
-    // of them (the 256-system workload: a handful of entries over one-child chains of thousands
-    // of occupants, one wave of them 17 ms of a 20 ms walk). They go first, as slices.
-    constexpr unsigned long long kHeavyFactor = 16, kHeavyPP = 4;
-    std::vector<uint32_t> heavy, rest;
-    for (uint32_t i : order) {
-        const bool h = heavy.size() < kHeavyMax && cost[i] > 0 &&
-                       cost[i] * w0.size() >= kHeavyFactor * total &&
-                       cost_pp[i] >= kHeavyPP * (cost[i] - cost_pp[i]);
-        (h ? heavy : rest).push_back(i);
-    }
-    const uint32_t nh = (uint32_t)heavy.size();
-    heavy.insert(heavy.end(), rest.begin(), rest.end());
-    return upload_work(c, v, heavy, nh);
+    // heavy: the costliest entries, at least kHeavyFactor times the mean, at most kHeavyMax of
+    // them (the 256-system workload: a few thousand entries whose sinks sit under one-child
+    // chains of thousands of occupants, 15-50 times the mean cost, 4-17 ms each of a 17 ms walk
+    // on too few waves to fill the device). Each of their slices repeats the path's acceptance tests
+    // but takes an eighth of the monopoles and of the particle-particle sums.
+    constexpr unsigned long long kHeavyFactor = 16;
+    uint32_t nh = 0;
+    while (nh < std::min<size_t>(kHeavyMax, order.size()) && cost[order[nh]] > 0 &&
+           cost[order[nh]] * w0.size() >= kHeavyFactor * total)
+        ++nh;
+    return upload_work(c, v, order, nh);
 }
 
 bh_status bh_set_walk_priority(bh_ctx* c, int on) {

@@ -104,18 +104,19 @@ inline int variant_ppu(int v) {
 // 0-3 the slice f, 4-7 F, 8-31 the heavy slot; its wave takes the monopoles of every F-th visit
 // and the f-th slice of every particle-particle cell, and writes partial results to
 // side[slot][f][sink] (planes x, y, total), which k_sum_heavy adds in slice order.
-constexpr uint32_t kHeavyMax = 256;   // heavy entries of a work table
+constexpr uint32_t kHeavyMax = 8192;  // heavy entries of a work table
 constexpr uint32_t kHeavySlices = 8;  // waves per heavy entry
 constexpr uint32_t kHeavySinks = 256; // sinks per work entry, at most (64 lanes x 4 per lane)
 struct TraverseBatch {
     const uint4* work = nullptr;
     uint32_t n_work = 0;
     const uint32_t* roots = nullptr;
-    float* side = nullptr;  // 3 planes of kHeavyMax x kHeavySlices x kHeavySinks
+    float* side = nullptr;    // 3 planes of side_slots x kHeavySlices x kHeavySinks
+    uint32_t side_slots = 0;  // heavy slots `side` holds room for
 };
 // sums the slices of each heavy entry (heavy: (first sink, count, F, -) per slot) into the outputs
-void launch_sum_heavy(hipStream_t s, const float* side, const uint4* heavy, uint32_t nheavy,
-                      float* ax, float* ay, float* tot);
+void launch_sum_heavy(hipStream_t s, const float* side, uint32_t side_slots, const uint4* heavy,
+                      uint32_t nheavy, float* ax, float* ay, float* tot);
 uint32_t launch_traverse(hipStream_t s, int kernel, int variant, const float4* pos4,
                      const Node* nodes, const BuildParams* prm, uint32_t n, float inv_theta2,
                      float eps2, float scale, float* ax, float* ay, float* tot,

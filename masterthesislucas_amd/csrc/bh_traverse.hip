@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         kahan_add(ay[s], cay[s], by[s]);
         if (TOT) kahan_add(at[s], cat[s], bt[s]);
         if (BATCHED && slice && valid[s]) {  // a heavy entry's slice: partial results
-            const size_t plane = (size_t)kHeavyMax * kHeavySlices * kHeavySinks;
+            const size_t plane = (size_t)batch.side_slots * kHeavySlices * kHeavySinks;
             const size_t o = ((size_t)(slice >> 8) * kHeavySlices + (slice & 15u)) * kHeavySinks +
                              (uint32_t)(s * kWave + lane);
             batch.side[o] = scale * (ax[s] - cax[s]);
@@ -488,8 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
             if (TOT) tot_out[o] = scale * (at[s] - cat[s]);
         }
     }
-    if (counters) {  // one partial per block (no same-address device atomics)
-        __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
+    if (counters) {  // one pair per wave (no same-address device atomics)
         unsigned long long s_pn = 0, s_pp = 0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -499,18 +498,9 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         s_pn = wave_sum_u64(s_pn) + cnt_pn;
         s_pp = wave_sum_u64(s_pp);
         if (lane == 0) {
-            s_cnt[w][0] = s_pn;
-            s_cnt[w][1] = s_pp;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long a = 0, b = 0;
-            for (int k2 = 0; k2 < kWavesPerBlock; ++k2) {
-                a += s_cnt[k2][0];
-                b += s_cnt[k2][1];
-            }
-            counters[2 * (size_t)wb] = a;
-            counters[2 * (size_t)wb + 1] = b;
+            const size_t gw = (size_t)wb * kWavesPerBlock + w;
+            counters[2 * gw] = s_pn;
+            counters[2 * gw + 1] = s_pp;
         }
     }
 }
@@ -828,12 +818,13 @@ uint32_t launch_traverse_learned(hipStream_t s, int variant, const float4* pos4,
 // heavy batched entries: out[first + i] = sum over slices f of side[slot][f][i], in slice order
 // (one block per slot)
 __global__ __launch_bounds__(kBlock) void k_sum_heavy(const float* __restrict__ side,
+                                                      uint32_t side_slots,
                                                       const uint4* __restrict__ heavy,
                                                       float* __restrict__ ax,
                                                       float* __restrict__ ay,
                                                       float* __restrict__ tot) {
     const uint4 h = heavy[blockIdx.x];  // first sink, count, F
-    const size_t plane = (size_t)kHeavyMax * kHeavySlices * kHeavySinks;
+    const size_t plane = (size_t)side_slots * kHeavySlices * kHeavySinks;
     for (uint32_t i = threadIdx.x; i < h.y; i += kBlock) {
         float sx = 0.f, sy = 0.f, st = 0.f;
         for (uint32_t f = 0; f < h.z; ++f) {
@@ -848,9 +839,9 @@ __global__ __launch_bounds__(kBlock) void k_sum_heavy(const float* __restrict__ 
     }
 }
 
-void launch_sum_heavy(hipStream_t s, const float* side, const uint4* heavy, uint32_t nheavy,
-                      float* ax, float* ay, float* tot) {
-    if (nheavy) k_sum_heavy<<<nheavy, kBlock, 0, s>>>(side, heavy, ax, ay, tot);
+void launch_sum_heavy(hipStream_t s, const float* side, uint32_t side_slots, const uint4* heavy,
+                      uint32_t nheavy, float* ax, float* ay, float* tot) {
+    if (nheavy) k_sum_heavy<<<nheavy, kBlock, 0, s>>>(side, side_slots, heavy, ax, ay, tot);
 }
 
 // out[i] = sum over parts f of part[f n + i], in part order
