@@ -11,6 +11,7 @@
 // batch of at most kFoldEvery monopoles is one compensated addition.
 #include "bh_common.h"
 #include "bh_launch.h"
+#include <cfloat>
 #include "bh_mlp.h"
 
 #include <algorithm>
@@ -365,42 +366,69 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         // PPU sources per scalar fetch (pos4 is padded past n, lanes mask the
                         // tail). Only sink slots with open lanes take part: in about half of the
                         // cells a pair of slots has one slot with no lane near the cell.
-                        auto pp_slots = [&](auto lo_c, auto hi_c) {
+                        // Unmasked form (FAST; softened gravity without totals): every lane
+                        // sums every source and the lane mask is applied once per cell below.
+                        // A source past the cell's end gets mass 0 (a scalar select), so adds
+                        // 0; the sink itself has dx = dy = 0 and adds s * 0 = 0, with s
+                        // clamped to FLT_MAX so that it is never inf (r^2 >= eps^2 >= FLT_MIN
+                        // keeps rsq finite). A finite s is unchanged by the clamp, so the sums
+                        // are those of the masked form.
+                        auto pp_slots = [&](auto lo_c, auto hi_c, auto fast_c) {
                             constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
+                            constexpr bool FAST = decltype(fast_c)::value;
                             for (uint32_t j = f; j < e; j += PPU) {
                                 float4 q[PPU > 0 ? PPU : 1];
 #pragma unroll
                                 for (int t = 0; t < PPU; ++t) q[t] = pos4[j + t];
 #pragma unroll
                                 for (int t = 0; t < PPU; ++t) {
+                                    // (integer select of the uniform mass: stays scalar)
+                                    const float qm = !FAST || t == 0 ? q[t].z
+                                        : __uint_as_float(uniform_u32(
+                                              j + t < e ? __float_as_uint(q[t].z) : 0u));
 #pragma unroll
                                     for (int s = LO; s < HI; ++s) {
                                         const float qx = q[t].x - px[s], qy = q[t].y - py[s];
                                         const float q2 = fmaf(qx, qx, qy * qy);
-                                        const bool use =
-                                            mine[s] && (j + t < e) && (j + t != idx[s]);
-                                        accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use, lx[s],
-                                                           ly[s], lt[s]);
+                                        if constexpr (FAST) {
+                                            const float inv = __builtin_amdgcn_rsqf(q2 + eps2);
+                                            const float mi = qm * inv;
+                                            const float sc = __builtin_fminf(mi * inv * inv,
+                                                                             FLT_MAX);
+                                            lx[s] = fmaf(sc, qx, lx[s]);
+                                            ly[s] = fmaf(sc, qy, ly[s]);
+                                        } else {
+                                            const bool use =
+                                                mine[s] && (j + t < e) && (j + t != idx[s]);
+                                            accum<KERNEL, TOT>(qx, qy, q2, q[t].z, eps2, use,
+                                                               lx[s], ly[s], lt[s]);
+                                        }
                                     }
                                 }
                             }
+                        };
+                        auto pp_fast = [&](auto lo_c, auto hi_c) {
+                            if (KERNEL == kKernelGravity && !TOT && eps2 >= FLT_MIN)
+                                pp_slots(lo_c, hi_c, std::true_type{});
+                            else
+                                pp_slots(lo_c, hi_c, std::false_type{});
                         };
                         auto pp_pair = [&](auto p_c) {
                             constexpr int P = decltype(p_c)::value;
                             const bool a = open[P] != 0ull, c = open[P + 1] != 0ull;
                             if (a && c)
-                                pp_slots(std::integral_constant<int, P>{},
-                                         std::integral_constant<int, P + 2>{});
+                                pp_fast(std::integral_constant<int, P>{},
+                                        std::integral_constant<int, P + 2>{});
                             else if (a)
-                                pp_slots(std::integral_constant<int, P>{},
-                                         std::integral_constant<int, P + 1>{});
+                                pp_fast(std::integral_constant<int, P>{},
+                                        std::integral_constant<int, P + 1>{});
                             else if (c)
-                                pp_slots(std::integral_constant<int, P + 1>{},
-                                         std::integral_constant<int, P + 2>{});
+                                pp_fast(std::integral_constant<int, P + 1>{},
+                                        std::integral_constant<int, P + 2>{});
                         };
                         if constexpr (NS == 1) {
-                            pp_slots(std::integral_constant<int, 0>{},
-                                     std::integral_constant<int, 1>{});
+                            pp_fast(std::integral_constant<int, 0>{},
+                                    std::integral_constant<int, 1>{});
                         } else {
                             pp_pair(std::integral_constant<int, 0>{});
                             if constexpr (NS == 4) pp_pair(std::integral_constant<int, 2>{});
@@ -409,6 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         npp[s] += mine[s] ? (e - f) - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
+                        // lanes that did not open the cell add nothing (the unmasked form
+                        // summed for them too)
+                        if (!mine[s]) lx[s] = ly[s] = 0.f;
                         kahan_add(ax[s], cax[s], lx[s] + bx[s]);
                         kahan_add(ay[s], cay[s], ly[s] + by[s]);
                         if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
