@@ -244,6 +244,12 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None, split=None)
     if variant is None:
         from masterthesislucas_amd.autotune import time_variant
 
+        # the variants are timed on cost-ordered work tables, as the steps will run: one walk
+        # and bh_batch_order_by_cost per table (variants 0, 4, 8: 1, 2, 4 sinks per lane)
+        for v0 in (0, 4, 8):
+            ctx.set_variant(v0)
+            ctx.forces_device_only(theta)
+            ctx.order_by_cost()
         variant, times = tune(ctx, theta)
         # then the split (each work entry walked by F waves, each taking a slice of every
         # particle-particle cell) for the chosen variant
@@ -257,7 +263,7 @@ def batched_comparison(args, eps, theta, dt, rmax=0.0, variant=None, split=None)
                 split, best_ms = F, ms
     else:
         ctx.set_variant(variant)
-    ctx.set_split(1)
+    ctx.set_split(1)  # (a split above 1 restored the system-order tables)
     # walk the heaviest work entries first (bh_batch_order_by_cost, from one walk's counters)
     ctx.forces_device_only(theta)
     ctx.order_by_cost()
