@@ -156,6 +156,10 @@ def test_order_by_cost(gpu):
     ctx.order_by_cost()  # again, from the reordered table's counters: the same table
     ax2, ay2, _ = ctx.forces(0.5)
     assert np.array_equal(ax1.view(np.uint32), ax2.view(np.uint32))
+    # the slices without totals (unmasked particle-particle form) add up to the same forces
+    ax4, ay4, _ = ctx.forces(0.5, want_total=False)
+    assert np.array_equal(ax4.view(np.uint32), ax2.view(np.uint32))
+    assert np.array_equal(ay4.view(np.uint32), ay2.view(np.uint32))
     ctx.set_split(2)  # back to system order, each entry walked by 2 waves
     ax3, _, _ = ctx.forces(0.5)
     assert ctx.counters() == c0
