@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     }
     uint32_t idx[NS];
     bool valid[NS];
-    float px[NS], py[NS];
+    float px[NS], py[NS], pm[NS];  // pm: the sink's own mass (t-SNE totals)
     uint64_t active[NS];
     uint64_t any = 0;
 #pragma unroll
@@ -214,6 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
         valid[s] = BATCHED ? (uint32_t)(s * kWave + lane) < count : idx[s] < n;
         const float4 me = valid[s] ? pos4[idx[s]] : make_float4(0.f, 0.f, 0.f, 0.f);
         px[s] = me.x;
+        pm[s] = me.z;
         py[s] = me.y;
         active[s] = __ballot(valid[s]);
         any |= active[s];
@@ -366,13 +367,16 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         // PPU sources per scalar fetch (pos4 is padded past n, lanes mask the
                         // tail). Only sink slots with open lanes take part: in about half of the
                         // cells a pair of slots has one slot with no lane near the cell.
-                        // Unmasked form (FAST; softened gravity without totals): every lane
-                        // sums every source and the lane mask is applied once per cell below.
-                        // A source past the cell's end gets mass 0 (a scalar select), so adds
-                        // 0; the sink itself has dx = dy = 0 and adds s * 0 = 0, with s
-                        // clamped to FLT_MAX so that it is never inf (r^2 >= eps^2 >= FLT_MIN
-                        // keeps rsq finite). A finite s is unchanged by the clamp, so the sums
-                        // are those of the masked form.
+                        // Unmasked form (FAST; softened gravity without totals, and t-SNE):
+                        // every lane sums every source and the lane mask is applied once per
+                        // cell below. A source past the cell's end gets mass 0 (a scalar
+                        // select), so adds 0; the sink itself has dx = dy = 0 and adds s * 0 =
+                        // 0. Gravity: s is clamped to FLT_MAX so that it is never inf (r^2 >=
+                        // eps^2 >= FLT_MIN keeps rsq finite); a finite s is unchanged by the
+                        // clamp, so the forces are those of the masked form. t-SNE: w =
+                        // 1/(1 + r^2) is finite, the forces are those of the masked form, and
+                        // the sink's own term of Z (m * 1) is taken out once per cell below
+                        // (Z then differs from the masked form's by rounding only).
                         auto pp_slots = [&](auto lo_c, auto hi_c, auto fast_c) {
                             constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
                             constexpr bool FAST = decltype(fast_c)::value;
@@ -390,13 +394,20 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                     for (int s = LO; s < HI; ++s) {
                                         const float qx = q[t].x - px[s], qy = q[t].y - py[s];
                                         const float q2 = fmaf(qx, qx, qy * qy);
-                                        if constexpr (FAST) {
+                                        if constexpr (FAST && KERNEL == kKernelGravity) {
                                             const float inv = __builtin_amdgcn_rsqf(q2 + eps2);
                                             const float mi = qm * inv;
                                             const float sc = __builtin_fminf(mi * inv * inv,
                                                                              FLT_MAX);
                                             lx[s] = fmaf(sc, qx, lx[s]);
                                             ly[s] = fmaf(sc, qy, ly[s]);
+                                        } else if constexpr (FAST) {
+                                            const float wq = __builtin_amdgcn_rcpf(1.0f + q2);
+                                            const float mw = qm * wq;
+                                            const float sc = mw * wq;
+                                            lx[s] = fmaf(-sc, qx, lx[s]);
+                                            ly[s] = fmaf(-sc, qy, ly[s]);
+                                            if (TOT) lt[s] += mw;
                                         } else {
                                             const bool use =
                                                 mine[s] && (j + t < e) && (j + t != idx[s]);
@@ -408,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                             }
                         };
                         auto pp_fast = [&](auto lo_c, auto hi_c) {
-                            if (KERNEL == kKernelGravity && !TOT && eps2 >= FLT_MIN)
+                            if (KERNEL != kKernelGravity || (!TOT && eps2 >= FLT_MIN))
                                 pp_slots(lo_c, hi_c, std::true_type{});
                             else
                                 pp_slots(lo_c, hi_c, std::false_type{});
@@ -438,8 +449,12 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                     for (int s = 0; s < NS; ++s) {
                         npp[s] += mine[s] ? (e - f) - ((idx[s] >= f && idx[s] < e) ? 1u : 0u) : 0u;
                         // lanes that did not open the cell add nothing (the unmasked form
-                        // summed for them too)
-                        if (!mine[s]) lx[s] = ly[s] = 0.f;
+                        // summed for them too); t-SNE: the unmasked form added the sink's own
+                        // m * 1 to Z when the sink lies in the summed range
+                        if (!mine[s]) lx[s] = ly[s] = lt[s] = 0.f;
+                        if (KERNEL != kKernelGravity && TOT && PPU > 0 && mine[s] &&
+                            idx[s] >= f && idx[s] < e)
+                            lt[s] -= pm[s];
                         kahan_add(ax[s], cax[s], lx[s] + bx[s]);
                         kahan_add(ay[s], cay[s], ly[s] + by[s]);
                         if (TOT) kahan_add(at[s], cat[s], lt[s] + bt[s]);
