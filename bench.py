@@ -347,34 +347,54 @@ def tsne_run(args):
     y0 = T.initial_embedding(n, seed=SEED_TSNE, std=1e-2)
     run = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
     kl0, _ = run.costFunction()
-    # warm, then tune the walk variant and split on the trees of three snapshots of the
-    # embedding (end of the exaggeration phase, mid-run, end): the walks get more expensive as
-    # the embedding spreads, so the choice minimises their summed time
+    # warm, then tune the walk variant and split per phase of the run: on the trees of two
+    # snapshots inside the early-exaggeration phase (iterations 60 and 180: compact clusters,
+    # crowded leaves) for the first `switch` iterations, and of three later ones (260, mid-run,
+    # end) for the rest; the walks change with the embedding's spread, so each choice minimises
+    # the summed time of its phase's snapshots
+    switch = min(250, args.tsne_iters)
     if args.tsne_split > 0:
         run.run(args.tsne_iters)
-        best, tuning = (args.variant if args.variant >= 0 else 4, args.tsne_split), None
+        best = (args.variant if args.variant >= 0 else 4, args.tsne_split)
+        best_early, tuning, tuning_early = best, None, None
     else:
         from masterthesislucas_amd.autotune import tune_split
-        tuning, done = {}, 0
-        for stop in sorted({min(260, args.tsne_iters), args.tsne_iters // 2, args.tsne_iters}):
+        late = {min(260, args.tsne_iters), args.tsne_iters // 2, args.tsne_iters}
+        early = {min(60, switch), min(180, switch)} - late if switch > 0 else set()
+        tuning, tuning_early, done = {}, {}, 0
+        for stop in sorted(early | late):
             run.run(stop - done)
             done = stop
             _, times = tune_split(run.ctx, 0.5, splits=(1, 2, 4, 8))
+            acc = tuning_early if stop in early else tuning
             for key, ms in times.items():
-                tuning[key] = tuning.get(key, 0.0) + ms
+                acc[key] = acc.get(key, 0.0) + ms
         v, s = min(tuning, key=tuning.get).split("/")
         best = (int(v), int(s))
-    log(f"[bench] t-SNE walk: variant {best[0]}, split {best[1]}")
+        if tuning_early:
+            v, s = min(tuning_early, key=tuning_early.get).split("/")
+            best_early = (int(v), int(s))
+        else:
+            best_early = best
+    log(f"[bench] t-SNE walk: variant {best_early[0]}, split {best_early[1]} for the first "
+        f"{switch} iterations, then variant {best[0]}, split {best[1]}")
     run.close()
     def make():
         r = T.TSNE(rp, col, val, y0, min_theta=0.5, max_theta=0.5, exaggeration=(48.0, 4.0))
+        r.ctx.set_split(max(best_early[1], best[1]))  # the split buffer allocated up front
+        r.ctx.set_variant(best_early[0])
+        r.ctx.set_split(best_early[1])
+        return r
+
+    def drive(r):
+        r.run(switch)
         r.ctx.set_variant(best[0])
         r.ctx.set_split(best[1])
-        return r
+        r.run(args.tsne_iters - switch)
 
     run = make()
     t0 = time.perf_counter()
-    run.run(args.tsne_iters)
+    drive(run)
     wall = time.perf_counter() - t0
     pn, pp = run.ctx.counters()  # last iteration's repulsion (before the KL rebuilds the tree)
     kl1, z = run.costFunction()
@@ -384,12 +404,16 @@ def tsne_run(args):
            "iterations_per_s": args.tsne_iters / wall,
            "interactions_per_point": (pn + pp) / n,
            "traversal_variant": best[0], "traversal_split": best[1],
+           "early_phase": {"iterations": switch, "traversal_variant": best_early[0],
+                           "traversal_split": best_early[1],
+                           "tuning_ms": ({k: round(v, 4) for k, v in tuning_early.items()}
+                                         if tuning_early else None)},
            "tuning_ms": ({k: round(v, 4) for k, v in tuning.items()} if tuning else None),
            "kl_initial": kl0, "kl_final": kl1, "z_final": z,
            "reference_cpu_s_per_iteration_published": 0.23}
     y_end = run.embeddedPoints
     run.close()
-    ph = replay_phases(make, args.tsne_iters)
+    ph = replay_phases(make, args.tsne_iters, drive)
     rep_ms = sum(ph[k][0] for k in ("bounds+morton", "sort", "gather", "tree", "upward",
                                      "traversal")) / args.tsne_iters
     out["repulsion_ms_per_iteration"] = rep_ms
@@ -490,15 +514,19 @@ def tsne_nmae_trajectory(args, T, rp, col, val, y0, theta=0.5):
             "protocol": "tsneTests.h:781-1085 (naive-advanced, NMAE per iteration averaged)"}
 
 
-def replay_phases(make, iters):
+def replay_phases(make, iters, drive=None):
     """Per-phase times of a second run identical to a timed one (the walk, the FMM and every
     reduction sum in a fixed order, so it retraces the same embeddings) with every phase
     bracketed by HIP events; the timed run records none, since the events delay each next
-    kernel's start."""
+    kernel's start. `drive(run)` runs the iterations as the timed run did (default: all of
+    them in one call)."""
     run = make()
     run.ctx.profile_enable(True)
     run.ctx.profile_read(reset=True)
-    run.run(iters)
+    if drive is None:
+        run.run(iters)
+    else:
+        drive(run)
     ph = run.ctx.profile_read(reset=True)
     run.close()
     return ph
