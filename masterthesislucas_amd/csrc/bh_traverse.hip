@@ -240,6 +240,25 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
     int sp = 0, since_fold = 0;
     Node cur = nodes[min(k0, nn - 1)];
     uint32_t vmod = 0;  // SPLIT: node visits mod F
+    // Softened gravity without totals sums particle-particle cells in the unmasked form (below)
+    // when the sink's own term there, s = ((m r) r) r with r = rsq(eps^2), is finite for every
+    // sink of the wave: it is at most the same product of the wave's largest |m|, rounding
+    // being monotone. Otherwise (a huge mass, a tiny softening, a non-finite mass) the wave
+    // takes the masked form. Not in walks with issue priority: the guard's registers would cost
+    // them a wave per SIMD, and the one-system walks gain nothing from the unmasked form.
+    bool grav_unmasked = false;
+    if constexpr (KERNEL == kKernelGravity && !TOT && PPU > 0 && !PRIO) {
+        float mx = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const float am = valid[s] ? fabsf(pm[s]) : 0.f;
+            mx = am <= FLT_MAX ? fmaxf(mx, am) : INFINITY;  // NaN and inf: no bound
+        }
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+        mx = __uint_as_float(uniform_u32(__float_as_uint(mx)));
+        const float r = __builtin_amdgcn_rsqf(eps2);
+        grav_unmasked = eps2 >= FLT_MIN && mx * r * r * r <= FLT_MAX;
+    }
     // priority level (top 2 bits) and the visits left at it (one scalar register: every
     // further register of walk state cost the 1M-sink walk a wave per SIMD)
     uint32_t prio = (3u << 30) | min(max(prio_step, 1u), kPrioLeft);
@@ -367,16 +386,16 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                         // PPU sources per scalar fetch (pos4 is padded past n, lanes mask the
                         // tail). Only sink slots with open lanes take part: in about half of the
                         // cells a pair of slots has one slot with no lane near the cell.
-                        // Unmasked form (FAST; softened gravity without totals, and t-SNE):
-                        // every lane sums every source and the lane mask is applied once per
-                        // cell below. A source past the cell's end gets mass 0 (a scalar
-                        // select), so adds 0; the sink itself has dx = dy = 0 and adds s * 0 =
-                        // 0. Gravity: s is clamped to FLT_MAX so that it is never inf (r^2 >=
-                        // eps^2 >= FLT_MIN keeps rsq finite); a finite s is unchanged by the
-                        // clamp, so the forces are those of the masked form. t-SNE: w =
-                        // 1/(1 + r^2) is finite, the forces are those of the masked form, and
-                        // the sink's own term of Z (m * 1) is taken out once per cell below
-                        // (Z then differs from the masked form's by rounding only).
+                        // Unmasked form (FAST; softened gravity without totals where the
+                        // wave's guard above holds, and t-SNE): every lane sums every source
+                        // and the lane mask is applied once per cell below. A source past the
+                        // cell's end gets mass 0 (a scalar select), so adds 0 (r^2 >= eps^2 >=
+                        // FLT_MIN keeps rsq finite); the sink itself has dx = dy = 0 and adds
+                        // s * 0 = 0, s being finite by the guard. So the forces are those of
+                        // the masked form. t-SNE: w = 1/(1 + r^2) is finite, the forces are
+                        // those of the masked form, and the sink's own term of Z (m * 1) is
+                        // taken out once per cell below (Z then differs from the masked
+                        // form's by rounding only).
                         auto pp_slots = [&](auto lo_c, auto hi_c, auto fast_c) {
                             constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
                             constexpr bool FAST = decltype(fast_c)::value;
@@ -397,8 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                                         if constexpr (FAST && KERNEL == kKernelGravity) {
                                             const float inv = __builtin_amdgcn_rsqf(q2 + eps2);
                                             const float mi = qm * inv;
-                                            const float sc = __builtin_fminf(mi * inv * inv,
-                                                                             FLT_MAX);
+                                            const float sc = mi * inv * inv;
                                             lx[s] = fmaf(sc, qx, lx[s]);
                                             ly[s] = fmaf(sc, qy, ly[s]);
                                         } else if constexpr (FAST) {
@@ -419,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_traverse(const float4* __restrict__ 
                             }
                         };
                         auto pp_fast = [&](auto lo_c, auto hi_c) {
-                            if (KERNEL != kKernelGravity || (!TOT && eps2 >= FLT_MIN))
+                            if (KERNEL != kKernelGravity || grav_unmasked)
                                 pp_slots(lo_c, hi_c, std::true_type{});
                             else
                                 pp_slots(lo_c, hi_c, std::false_type{});

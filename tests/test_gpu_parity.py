@@ -460,27 +460,37 @@ def test_walk_priority_is_scheduling_only(gpu, variant):
     ctx.close()
 
 
-@pytest.mark.parametrize("soft", [EPS, 1.1e-19])
+@pytest.mark.parametrize("case", ["soft", "signed", "tiny"])
 @pytest.mark.parametrize("variant", [1, 4, 5, 11, 22, 23])
-def test_unmasked_particle_particle_form(gpu, variant, soft):
+def test_unmasked_particle_particle_form(gpu, variant, case):
     """Softened gravity without totals sums particle-particle cells in the unmasked form (every
     lane every source, the lane mask once per cell, mass 0 past the cell's end, the sink's own
-    term s * 0 with s clamped to FLT_MAX); with totals the masked form runs. Their forces are
-    bit-identical. Softening 1.1e-19 puts eps^2 just above FLT_MIN: the sink's own term then
-    overflows to inf before the clamp, so the clamp is what keeps it 0 (variants with 1 and 4
-    sources per fetch, 1, 2 and 4 sinks per lane, vector and scalar lane masks)."""
+    term s * 0) where a per-wave guard shows the sink's own s finite; with totals the masked
+    form runs. Their forces are bit-identical, with the walk's issue priority off (walks with
+    it keep the masked form) and on. Cases: softening 1e-3; masses of both signs; softening
+    1.1e-19 (eps^2 just above FLT_MIN), where the sink's own term would overflow and the guard
+    sends every wave to the masked form (variants with 1 and 4 sources per fetch, 1, 2 and 4
+    sinks per lane, vector and scalar lane masks)."""
     p = two_plummer_disks(1 << 16, seed=29)
-    ctx = _ctx(p.n, kernel="gravity", tree_mode="quadtree", softening=soft)
-    ctx.set_particles(p.x, p.y, p.m)
+    m = p.m
+    if case == "signed":  # 2% of the masses negative, at -0.37 of their size
+        neg = np.random.default_rng(30).random(p.n) < 0.02
+        m = np.where(neg, -0.37 * m, m).astype(np.float32)
+    ctx = _ctx(p.n, kernel="gravity", tree_mode="quadtree",
+               softening=1.1e-19 if case == "tiny" else EPS)
+    ctx.set_particles(p.x, p.y, m)
     ctx.set_variant(variant)
     ctx.build()
-    ax0, ay0, _ = ctx.forces(0.5, want_total=True)
-    c0 = ctx.counters()
-    ax1, ay1, _ = ctx.forces(0.5, want_total=False)
-    assert ctx.counters() == c0 and c0[1] > 0
-    assert np.isfinite(ax0).all() and np.isfinite(ay0).all()
-    assert np.array_equal(ax1.view(np.uint32), ax0.view(np.uint32))
-    assert np.array_equal(ay1.view(np.uint32), ay0.view(np.uint32))
+    for prio in (False, True):
+        ctx.set_priority(prio)
+        ax0, ay0, _ = ctx.forces(0.5, want_total=True)
+        c0 = ctx.counters()
+        ax1, ay1, _ = ctx.forces(0.5, want_total=False)
+        assert ctx.counters() == c0 and c0[1] > 0
+        if case != "signed":
+            assert np.isfinite(ax0).all() and np.isfinite(ay0).all()
+        assert np.array_equal(ax1.view(np.uint32), ax0.view(np.uint32))
+        assert np.array_equal(ay1.view(np.uint32), ay0.view(np.uint32))
     ctx.close()
 
 
