@@ -676,6 +676,10 @@ def main():
                     help="skip the batched-systems run (256 systems, graph-captured steps)")
     ap.add_argument("--batched-systems", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-variant", type=int, default=-1,
+                    help="walk variant of the batched runs (default: autotuned), e.g. to profile one")
+    ap.add_argument("--no-batched-truncated", action="store_true",
+                    help="skip the batched run with the Plummer systems truncated at R = 3")
     ap.add_argument("--no-tsne", action="store_true",
                     help="skip the t-SNE run (70,000 points, 1,000 iterations)")
     ap.add_argument("--no-tsne-fmm", action="store_true",
@@ -824,8 +828,11 @@ def main():
             result["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
     ctx.close()
     if not args.no_batched:
-        result["batched"] = batched_comparison(args, eps, theta, dt)
-        result["batched"]["truncated_r3"] = batched_comparison(args, eps, theta, dt, rmax=3.0)
+        bv = args.batched_variant if args.batched_variant >= 0 else None
+        result["batched"] = batched_comparison(args, eps, theta, dt, variant=bv)
+        if not args.no_batched_truncated:
+            result["batched"]["truncated_r3"] = batched_comparison(args, eps, theta, dt, rmax=3.0,
+                                                                   variant=bv)
     if not args.no_learned:
         result["learned"] = learned_comparison(args, eps)
     if not args.no_tsne:
