@@ -67,7 +67,7 @@ struct bh_ctx {
     uint8_t *leaf_lvl = nullptr, *lcp1 = nullptr;
     uint32_t *cnt = nullptr, *noff = nullptr;
     Node* nodes = nullptr;
-    uint32_t *parent = nullptr, *child4 = nullptr, *deferred = nullptr;
+    uint32_t *parent = nullptr, *child4 = nullptr, *deferred = nullptr, *bylevel = nullptr;
     float4* m2 = nullptr;
     BuildParams* prm = nullptr;
     float4* bounds_part = nullptr;
@@ -358,7 +358,7 @@ bh_status do_build(bh_ctx* c, const double* bounds) {
 void launch_node_moments(bh_ctx* c) {
     phase_begin(c, 4);
     launch_upward(c->stream, c->pos4, c->nodes, c->child4, c->learned ? c->m2 : nullptr, c->prm,
-                  c->deferred, c->cap_nodes, !c->wide && !c->nsys);
+                  c->deferred, c->bylevel, c->cap_nodes, !c->wide && !c->nsys);
     phase_end(c, 4);
     c->moments = true;
 }
@@ -590,6 +590,7 @@ static bh_status create_impl(bh_ctx** out, const bh_config* cfg, uint32_t extra_
     TRY(dalloc(c, &c->nodes, c->cap_nodes));
     TRY(dalloc(c, &c->parent, c->cap_nodes));
     TRY(dalloc(c, &c->deferred, (size_t)up_chunks(c->cap_nodes) * kDeferStride));
+    TRY(dalloc(c, &c->bylevel, (size_t)up_chunks(c->cap_nodes) * kDeferStride));
     TRY(dalloc(c, &c->stats_part, kStatsBlocks));
     TRY(dalloc(c, &c->child4, 4ull * c->cap_nodes));
     TRY(dalloc(c, &c->prm, 1));

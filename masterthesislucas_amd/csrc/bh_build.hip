@@ -1023,6 +1023,7 @@ __global__ __launch_bounds__(kUpThreads) void k_up_top(Node* __restrict__ nodes,
                                                        float4* __restrict__ m2,
                                                        const BuildParams* __restrict__ prm,
                                                        const uint32_t* __restrict__ deferred,
+                                                       uint32_t* __restrict__ bylevel,
                                                        uint32_t nchunks, uint32_t cap_nodes) {
     __shared__ uint32_t s_k[kTopMax];
     __shared__ uint8_t s_l[kTopMax];
@@ -1057,23 +1058,43 @@ __global__ __launch_bounds__(kUpThreads) void k_up_top(Node* __restrict__ nodes,
     }
     const uint32_t D = s_count;
     if (D > kTopMax) {
-        // large list: level by level through global memory
-        uint32_t lmax = 0;
+        // large list (many trees: batched systems): the entries bucketed by level with a
+        // counting sort into `bylevel`, then each level's bucket settled, deepest first, through
+        // global memory. Each node combines its settled children in digit order, so the order
+        // inside a bucket does not change any result.
+        __shared__ uint32_t s_lc[256], s_lo[256];  // per level: entries, then bucket start
+        for (uint32_t i = tid; i < 256; i += kUpThreads) s_lc[i] = 0u;
+        __syncthreads();
         const uint32_t entries = nch * (kDeferStride - 1);
         for (uint32_t e = tid; e < entries; e += kUpThreads) {
             const uint32_t b = e / (kDeferStride - 1), slot = e % (kDeferStride - 1);
             const uint32_t* lst = deferred + (size_t)b * kDeferStride;
-            if (slot < lst[0]) lmax = max(lmax, nodes[lst[1 + slot]].meta & kMetaLevelMask);
+            if (slot < lst[0]) atomicAdd(&s_lc[nodes[lst[1 + slot]].meta & kMetaLevelMask], 1u);
         }
-        atomicMax(&s_maxl, lmax);
         __syncthreads();
-        for (int L = (int)s_maxl; L >= 0; --L) {
-            for (uint32_t e = tid; e < entries; e += kUpThreads) {
-                const uint32_t b = e / (kDeferStride - 1), slot = e % (kDeferStride - 1);
-                const uint32_t* lst = deferred + (size_t)b * kDeferStride;
-                if (slot >= lst[0]) continue;
-                const uint32_t k = lst[1 + slot];
-                if ((int)(nodes[k].meta & kMetaLevelMask) != L) continue;
+        if (tid == 0) {
+            uint32_t at = 0;
+            for (int l = 0; l < 256; ++l) {
+                s_lo[l] = at;
+                at += s_lc[l];
+                s_lc[l] = 0u;
+            }
+        }
+        __syncthreads();
+        for (uint32_t e = tid; e < entries; e += kUpThreads) {
+            const uint32_t b = e / (kDeferStride - 1), slot = e % (kDeferStride - 1);
+            const uint32_t* lst = deferred + (size_t)b * kDeferStride;
+            if (slot >= lst[0]) continue;
+            const uint32_t k = lst[1 + slot];
+            const uint32_t l = nodes[k].meta & kMetaLevelMask;
+            bylevel[s_lo[l] + atomicAdd(&s_lc[l], 1u)] = k;
+        }
+        __syncthreads();
+        for (int L = 255; L >= 0; --L) {
+            const uint32_t lo = s_lo[L], cnt = s_lc[L];  // the same in every thread
+            if (cnt == 0u) continue;
+            for (uint32_t i = tid; i < cnt; i += kUpThreads) {
+                const uint32_t k = bylevel[lo + i];
                 const uint4 cs = child4[k];
                 const uint32_t ci[4] = {cs.x, cs.y, cs.z, cs.w};
                 Moments ch[4];
@@ -1419,8 +1440,8 @@ void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
 uint32_t up_chunks(uint32_t cap_nodes) { return div_up(cap_nodes, kUpChunk); }
 
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
-                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes,
-                   bool big_leaves) {
+                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t* bylevel,
+                   uint32_t cap_nodes, bool big_leaves) {
     const uint32_t nch = up_chunks(cap_nodes);
     const bool big_done = big_leaves && !m2;
     if (big_done)
@@ -1428,8 +1449,8 @@ void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_
                                                                                 cap_nodes);
     k_up_chunk<<<nch, kUpThreads, 0, s>>>(pos4, nodes, (const uint4*)child4, m2, prm, deferred,
                                       cap_nodes, big_done ? 1 : 0);
-    k_up_top<<<1, kUpThreads, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, nch,
-                                      cap_nodes);
+    k_up_top<<<1, kUpThreads, 0, s>>>(nodes, (const uint4*)child4, m2, prm, deferred, bylevel,
+                                      nch, cap_nodes);
 }
 
 void launch_tree_stats(hipStream_t s, const Node* nodes, const BuildParams* prm,

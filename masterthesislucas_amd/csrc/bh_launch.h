@@ -58,9 +58,10 @@ void launch_morton_batched(hipStream_t s, const float* xs, const float* ys, uint
 void launch_batch_roots(hipStream_t s, const uint32_t* sys_off, uint32_t nsys,
                         const uint32_t* noff, const BuildParams* prm, uint32_t* roots);
 uint32_t up_chunks(uint32_t cap_nodes);
+// deferred and bylevel: up_chunks(cap_nodes) * kDeferStride words each
 void launch_upward(hipStream_t s, const float4* pos4, Node* nodes, const uint32_t* child4,
-                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t cap_nodes,
-                   bool big_leaves);
+                   float4* m2, const BuildParams* prm, uint32_t* deferred, uint32_t* bylevel,
+                   uint32_t cap_nodes, bool big_leaves);
 void launch_tree_stats(hipStream_t s, const Node* nodes, const BuildParams* prm,
                        uint32_t cap_nodes, int depth_limit, uint32_t leaf_cap, uint2* out,
                        uint32_t blocks);
