@@ -68,6 +68,34 @@ def test_batched_equals_each_system_alone(gpu, variant):
     b.close()
 
 
+def test_many_systems_equal_each_system_alone(gpu):
+    """The 256-system workload (11.1 M particles): its trees defer more top nodes than the
+    upward pass keeps on chip, so they are settled through global memory, bucketed by level.
+    Forces and interaction counts of sampled systems are bit-identical to each system alone
+    (24 code bits, as the batched keys have)."""
+    from masterthesislucas_amd.workloads import batched_systems
+    ps = batched_systems()
+    sizes = [len(q.x) for q in ps]
+    x, y, m, vx, vy = _concat(ps)
+    b = BHContext(len(x), kernel="gravity", softening=EPS, sizes=sizes)
+    b.set_particles(x, y, m, vx, vy)
+    b.build()
+    ax, ay, _ = b.forces(0.5, want_total=False)
+    ns, off = b.batch_info()
+    assert ns == 256
+    b.close()
+    for s in (0, 1, 128, 255, int(np.argmax(sizes))):
+        p = ps[s]
+        a = BHContext(len(p.x), kernel="gravity", softening=EPS, code_bits=24)
+        a.set_particles(p.x, p.y, p.m)
+        a.build()
+        sx, sy, _ = a.forces(0.5, want_total=False)
+        lo, hi = off[s], off[s + 1]
+        assert np.array_equal(ax[lo:hi].view(np.uint32), sx.view(np.uint32)), s
+        assert np.array_equal(ay[lo:hi].view(np.uint32), sy.view(np.uint32)), s
+        a.close()
+
+
 def test_batched_direct_sum_stays_in_system(gpu):
     ps = _systems([4000, 3000, 50])
     x, y, m, _, _ = _concat(ps)
